@@ -1,0 +1,180 @@
+"""Benchmark of the hybrid traversal hot path on one MI355X.
+
+Workload (N=1): plain single-source BFS from 64 seeded non-isolated sources on
+a directed RMAT scale-22 graph, edge factor 16, (a,b,c,d) = (0.57,0.19,0.19,0.05),
+self-loops and duplicates removed, uint32 weights in [1,64). One step = the 64
+queries back to back on one HIP stream; inputs are resident in HBM before the
+timed region. value = aggregate GTEPS = sum of m_r (out-degree of reached
+vertices, Graph500 convention) over all timed queries / wall time.
+
+roofline: algorithmic bytes of the BFS kernel per launch, 8*n_r + 4*m_r + 4*n,
+over its average launch duration from HIP events recorded on the launch stream.
+cpu_baseline: the oracle's single-threaded FIFO BFS (the reference's BFSHost)
+on a bounded sample of the same sources, timed on this host.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "sep-graph_amd"))
+
+HBM_PEAK_GBS = 8000.0
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def cpu_baseline(hg, sources, gpu_levels, budget_s):
+    """Oracle BFS (single thread) over as many sources as fit in budget_s."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle
+
+    deg = np.diff(hg.row.astype(np.int64))
+    tot_edges, tot_t, done, exact = 0, 0.0, 0, 0
+    for i, s in enumerate(sources):
+        t0 = time.perf_counter()
+        lv = pyoracle.bfs(hg.row, hg.col, int(s))
+        dt = time.perf_counter() - t0
+        tot_t += dt
+        tot_edges += int(deg[lv >= 0].sum())
+        done += 1
+        if i in gpu_levels:
+            exact += int(np.array_equal(lv, gpu_levels[i]))
+        if tot_t > budget_s:
+            break
+    return {
+        "value": tot_edges / tot_t / 1e9,
+        "unit": "GTEPS",
+        "cores": 1,
+        "kind": "port",
+        "sample": f"{done} of {len(sources)} BFS sources on the same graph, oracle FIFO BFS "
+                  f"(restating BFSHost), {tot_t:.1f} s",
+        "bit_exact_vs_gpu": f"{exact}/{min(done, len(gpu_levels))}",
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--scale", type=int, default=22)
+    ap.add_argument("--sources", type=int, default=64)
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU only: this build has no multi-GPU path "
+              "(the engine runs on a single MI355X); run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    import sepgraph as sg
+
+    if not torch.cuda.is_available() or sg.lib().sg_device_count() < 1:
+        print("bench.py needs a visible MI355X", file=sys.stderr)
+        sys.exit(1)
+
+    t0 = time.time()
+    hg = sg.rmat_graph(args.scale, 16, symmetrize=False, seed=1, weights=(1, 64), weight_seed=2)
+    log(f"graph {hg.name}: n={hg.n} m={hg.m} built in {time.time() - t0:.1f}s")
+    t0 = time.time()
+    g = sg.Graph(hg)
+    log(f"uploaded (CSR+CSC+weights, {g.info()[2] / 1e9:.2f} GB) in {time.time() - t0:.1f}s")
+    sources = sg.pick_sources(hg.row, args.sources, seed=3)
+    stream = torch.cuda.current_stream()
+    e = sg.Engine(g, stream=stream.cuda_stream)
+
+    # Warm-up pass: per-source reach statistics (deterministic), kept levels for
+    # the CPU-baseline cross-check.
+    n_r, m_r, keep = [], [], {}
+    for i, s in enumerate(sources):
+        e.bfs(int(s))
+        st = e.stats()
+        if st["status"] != 0:
+            raise RuntimeError(f"BFS status {st['status']} for source {s}")
+        n_r.append(st["reached"])
+        m_r.append(st["reached_edges"])
+        if i < 8:
+            keep[i] = e.gather(np.int32)
+    for _ in range(max(0, args.warmup - 1)):
+        for s in sources:
+            e.bfs(int(s))
+    torch.cuda.synchronize()
+    log(f"warm-up done; mean n_r={np.mean(n_r):.0f} m_r={np.mean(m_r):.0f}")
+
+    nq = len(sources)
+    evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(nq * args.steps)]
+    torch.cuda.synchronize()
+    t_start = time.perf_counter()
+    k = 0
+    for _ in range(args.steps):
+        for s in sources:
+            evs[k][0].record(stream)
+            e.bfs(int(s))
+            evs[k][1].record(stream)
+            k += 1
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t_start
+    e.sync()
+    durs_ms = np.array([a.elapsed_time(b) for a, b in evs])
+    total_edges = args.steps * float(np.sum(m_r))
+    gteps = total_edges / wall / 1e9
+    bytes_per = 8.0 * np.array(n_r, np.float64) + 4.0 * np.array(m_r, np.float64) + 4.0 * hg.n
+    alg_bytes = float(np.tile(bytes_per, args.steps).sum())
+    achieved = alg_bytes / (durs_ms.sum() / 1e3) / 1e9
+    log(f"timed {nq * args.steps} BFS in {wall * 1e3:.1f} ms: {gteps:.1f} GTEPS, "
+        f"kernel mean {durs_ms.mean() * 1e3:.1f} us, achieved {achieved:.0f} GB/s")
+
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(hg, sources, keep, args.cpu_seconds)
+        log(f"cpu baseline {cpu['value']:.4f} GTEPS ({cpu['sample']}), exact {cpu['bit_exact_vs_gpu']}")
+
+    out = {
+        "metric": "BFS GTEPS, RMAT-22 directed, 64 sources (bit-exact levels)",
+        "value": round(gteps, 3),
+        "unit": "GTEPS",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(wall * 1e3 / args.steps, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "int32",
+        "data": "synthetic RMAT (seeded, generated in-process)",
+        "config": {
+            "workload": f"bfs-rmat{args.scale}-ef16-directed-{nq}src",
+            "scale": args.scale,
+            "edge_factor": 16,
+            "n": hg.n,
+            "m": hg.m,
+            "sources": nq,
+            "mean_reached_vertices": float(np.mean(n_r)),
+            "mean_reached_edges": float(np.mean(m_r)),
+            "parallelism": "single-gpu",
+        },
+        "roofline": {
+            "bound": "hbm",
+            "kernel": "k_bfs (persistent hybrid BFS, one launch per query)",
+            "achieved": round(achieved, 1),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": None,
+            "mean_launch_us": round(float(durs_ms.mean()) * 1e3, 2),
+        },
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,198 @@
+/*
+ * sepgraph.h — C ABI of the MI355X-native hybrid graph engine (libsepgraph.so).
+ *
+ * The reference exposes its hot path as C++ templates: an app derives from
+ * sepgraph::api::AppBase (include/framework/variants/api.cuh:22-120) and is
+ * driven by sepgraph::engine::Engine (include/framework/framework.cuh:56-797)
+ * configured through EngineOptions (include/framework/common.h:203-321).
+ * Templates with device hooks cannot cross a C ABI, so this header exports the
+ * same life cycle over opaque handles:
+ *
+ *   Engine ctor + LoadGraph          -> sg_graph_create + sg_engine_create
+ *   EngineOptions / SetOptions        -> sg_engine_options + sg_engine_set_options
+ *   InitGraph(source) + Start()       -> sg_bfs / sg_sssp / sg_pagerank / sg_bc
+ *   GatherValue() / GatherBuffer()    -> sg_engine_gather_value / sg_engine_value_ptr
+ *   PrintInfo() / JSON running data   -> sg_engine_get_stats
+ *
+ * Conventions (all entry points):
+ *   - every call returns an int status (SG_OK == 0, negative on error) and
+ *     never exits the process; sg_last_error() gives the thread's last message.
+ *     (The reference prints and calls exit(1) on any CUDA error,
+ *     include/groute/internal/cuda_utils.h:41-65.)
+ *   - vertex ids and CSR/CSC offsets are uint32 (the reference's index_t);
+ *     graphs are limited to n < 2^31 vertices and m < 2^32 edges.
+ *   - BFS levels are int32 with -1 for unreached (the int32 view of the
+ *     reference's UINT32_MAX sentinel, samples/hybrid_bfs/hybrid_bfs_common.h:38);
+ *     SSSP distances are uint32 with UINT32_MAX for unreached.
+ *   - runs are asynchronous on the engine's HIP stream (sg_engine_set_stream
+ *     takes a hipStream_t as void*); gather/stats calls synchronise it.
+ *   - a handle is not safe for concurrent use from several threads; distinct
+ *     handles are independent.
+ */
+#ifndef SEPGRAPH_H
+#define SEPGRAPH_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---------------------------------------------------------------- status */
+enum {
+    SG_OK = 0,
+    SG_ERR_INVALID = -1,     /* bad argument or malformed graph */
+    SG_ERR_RANGE = -2,       /* vertex id / size out of range */
+    SG_ERR_NOMEM = -3,       /* host or device allocation failed */
+    SG_ERR_HIP = -4,         /* HIP runtime error */
+    SG_ERR_CAPACITY = -5,    /* a fixed-capacity buffer would overflow */
+    SG_ERR_TIMEOUT = -6,     /* a device-side bounded wait expired */
+    SG_ERR_STATE = -7,       /* call not valid in the handle's current state */
+    SG_ERR_UNSUPPORTED = -8  /* option combination not implemented */
+};
+
+/* Thread-local message describing the last failing call on this thread. */
+const char *sg_last_error(void);
+/* Library version, e.g. 100 for 0.1.0. */
+int sg_version(void);
+/* Number of visible HIP devices (0 on a host without GPU; never an error). */
+int sg_device_count(void);
+
+/* ------------------------------------------------- host graph construction
+ * Synthetic inputs and CSR/CSC building. The reference loads .gr/.mtx/METIS
+ * files or generates uniform/chain/clique graphs (include/utils/graphs/traversal.h:99-200);
+ * it has no RMAT or grid generator. All generators are counter-based
+ * (SplitMix64 draw k of the seed's stream), so output is independent of the
+ * number of host threads. */
+
+/* R-MAT quadrant descent (Chakrabarti, Zhan & Faloutsos 2004) without noise
+ * or relabelling: edge i, level l (l = 0 is the top bit) picks quadrant
+ * a/b/c/d from draw(seed, i*scale + l). Writes edge_factor << scale tuples. */
+int sg_gen_rmat_edges(int scale, int edge_factor, double a, double b, double c, uint64_t seed,
+                      uint32_t *src, uint32_t *dst);
+
+/* rows x cols 4-neighbour grid, vertex id r*cols + c. Each undirected edge is
+ * kept unless draw(seed, edge_id) falls below p_delete (horizontal edge of
+ * (r,c) has id 2*(r*cols+c), vertical edge id 2*(r*cols+c)+1). Writes each
+ * kept edge once as (u, v) with u < v; *count gets the number written.
+ * Capacity needed: 2*rows*cols. */
+int sg_gen_grid_edges(uint32_t rows, uint32_t cols, double p_delete, uint64_t seed, uint32_t *src,
+                      uint32_t *dst, uint64_t *count);
+
+/* Build CSR (row_start[n+1], edge_dst[*m_out]) from an edge list. flags:
+ * SG_CSR_SYMMETRIZE adds the reverse of every edge; SG_CSR_DROP_SELF_LOOPS;
+ * SG_CSR_DEDUP removes duplicate (u,v). Neighbours end up sorted ascending.
+ * edge_dst must hold `cap` entries (2*count is always enough with
+ * SYMMETRIZE, count without). */
+enum { SG_CSR_SYMMETRIZE = 1, SG_CSR_DROP_SELF_LOOPS = 2, SG_CSR_DEDUP = 4 };
+int sg_build_csr(uint32_t n, uint64_t count, const uint32_t *src, const uint32_t *dst, int flags,
+                 uint32_t *row_start, uint32_t *edge_dst, uint64_t cap, uint64_t *m_out);
+
+/* Per-edge uint32 weights uniform in [lo, hi), weight of (u,v) from
+ * draw(seed, u<<32|v), or from the unordered pair when `symmetric` (so both
+ * directions of an undirected edge agree). */
+int sg_gen_weights(uint32_t n, const uint32_t *row_start, const uint32_t *edge_dst, uint32_t lo,
+                   uint32_t hi, uint64_t seed, int symmetric, uint32_t *weights);
+
+/* CSC (col_start[n+1], edge_src[m], optional weights) with sources ascending in
+ * each column, as csr_tocsc (include/groute/graphs/csr_graph.h:330-397). */
+int sg_csc_from_csr(uint32_t n, const uint32_t *row_start, const uint32_t *edge_dst,
+                    const uint32_t *weights, uint32_t *col_start, uint32_t *edge_src,
+                    uint32_t *csc_weights);
+
+/* k distinct vertices with out-degree >= 1, drawn as draw(seed, i) mod n with
+ * rejection. Fails with SG_ERR_RANGE if fewer than k such vertices exist. */
+int sg_pick_sources(uint32_t n, const uint32_t *row_start, int k, uint64_t seed, uint32_t *out);
+
+/* ------------------------------------------------------------ device graph */
+typedef struct sg_graph sg_graph;
+
+/* Upload a graph to device 0 (the reference's LoadGraph: CSR and CSC mirrors,
+ * include/framework/framework.cuh:99-133). weights may be NULL. When
+ * `undirected` is set the CSC aliases the CSR (FLAGS_undirected,
+ * framework.cuh:111-118); otherwise col_start/edge_src/csc_weights give the
+ * CSC, or are all NULL to have the library build it. */
+int sg_graph_create(uint32_t n, uint64_t m, const uint32_t *row_start, const uint32_t *edge_dst,
+                    const uint32_t *weights, const uint32_t *col_start, const uint32_t *edge_src,
+                    const uint32_t *csc_weights, int undirected, sg_graph **out);
+int sg_graph_destroy(sg_graph *g);
+/* n, m, and bytes of device memory held by the graph. */
+int sg_graph_info(const sg_graph *g, uint32_t *n, uint64_t *m, uint64_t *device_bytes);
+
+/* ------------------------------------------------------------------ engine */
+typedef struct sg_engine sg_engine;
+
+/* AlgoVariant (include/framework/common.h:43-177): model x passing x scheduling. */
+enum {
+    SG_VARIANT_AUTO = -1,
+    SG_ASYNC_PUSH_TD = 0,
+    SG_ASYNC_PUSH_DD = 1,
+    SG_ASYNC_PULL_TD = 2,
+    SG_ASYNC_PULL_DD = 3,
+    SG_SYNC_PUSH_TD = 4,
+    SG_SYNC_PUSH_DD = 5,
+    SG_SYNC_PULL_TD = 6,
+    SG_SYNC_PULL_DD = 7
+};
+/* LoadBalancing (common.h): NONE / COARSE_GRAINED / FINE_GRAINED / HYBRID. */
+enum { SG_LB_DEFAULT = -1, SG_LB_NONE = 0, SG_LB_COARSE = 1, SG_LB_FINE = 2, SG_LB_HYBRID = 3 };
+/* Priority (common.h): NONE / LOW_HIGH (two-level) / SAMPLING. */
+enum { SG_PRIO_NONE = 0, SG_PRIO_LOW_HIGH = 1, SG_PRIO_SAMPLING = 2 };
+
+typedef struct sg_engine_options {
+    int32_t force_variant;      /* EngineOptions::ForceVariant, or SG_VARIANT_AUTO */
+    int32_t fused;              /* EngineOptions::SetFused (persistent multi-round kernel) */
+    int32_t priority;           /* SG_PRIO_* */
+    float priority_threshold;   /* SetTwoLevelBasedPriority delta (SSSP); 0 = heuristic */
+    float cut_threshold;        /* SetSampleBasedPriority */
+    int32_t lb_push, lb_pull;   /* SetLoadBalancing, SG_LB_* */
+    double alpha;               /* FLAGS_alpha, default 15 (include/utils/app_skeleton.h:50) */
+    double beta;                /* FLAGS_beta, default 18 (app_skeleton.h:51) */
+    double edge_factor;         /* FLAGS_edge_factor, default 1 (app_skeleton.h:49) */
+    int32_t max_iteration;      /* FLAGS_max_iteration; 0 = no cap (the reference's 200
+                                   cuts high-diameter traversals short, framework.cuh:359-364) */
+    int32_t block_size;         /* threads per workgroup for traversal kernels; 0 = tuned */
+    int32_t large_degree;       /* out-degree above which a push vertex is split across
+                                   the grid; 0 = tuned */
+    int32_t reserved[8];
+} sg_engine_options;
+
+void sg_engine_options_default(sg_engine_options *opt);
+
+typedef struct sg_stats {
+    uint32_t rounds;            /* traversal rounds (levels) executed */
+    uint32_t push_rounds, pull_rounds;
+    uint32_t reached;           /* n_r: vertices with a finite result */
+    uint64_t reached_edges;     /* m_r: sum of out-degree over reached vertices */
+    uint64_t policy_trace;      /* bit r = 1 if round r ran pull (first 64 rounds) */
+    double kernel_ms;           /* device time of the last run (HIP events) */
+    double residual;            /* PageRank: final L1 change (FP64 on device) */
+    uint32_t iterations;        /* PageRank iterations */
+    uint32_t max_depth;         /* deepest level reached */
+    int32_t status;             /* device-side status of the last run (SG_OK or error) */
+    uint32_t reserved[7];
+} sg_stats;
+
+int sg_engine_create(sg_graph *g, sg_engine **out);
+int sg_engine_destroy(sg_engine *e);
+int sg_engine_set_stream(sg_engine *e, void *hip_stream);
+int sg_engine_set_options(sg_engine *e, const sg_engine_options *opt);
+
+/* BFS from `source` (hybrid_bfs, samples/hybrid_bfs/hybrid_bfs.cu:173-243):
+ * int32 levels, direction chosen per round by the reference's alpha/beta rule
+ * (include/framework/hybrid_policy.h:237-291). Asynchronous. */
+int sg_bfs(sg_engine *e, uint32_t source);
+
+/* Synchronise the engine's stream and report the device-side status. */
+int sg_engine_sync(sg_engine *e);
+/* Device pointer and size in bytes of the last run's per-vertex result. */
+int sg_engine_value_ptr(sg_engine *e, void **dev_ptr, uint64_t *bytes);
+/* Copy the last run's per-vertex result to host memory (synchronises). */
+int sg_engine_gather_value(sg_engine *e, void *host_out, uint64_t bytes);
+/* Statistics of the last run (synchronises). */
+int sg_engine_get_stats(sg_engine *e, sg_stats *out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* SEPGRAPH_H */

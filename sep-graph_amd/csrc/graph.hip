@@ -1,0 +1,356 @@
+// Device graph upload and engine life cycle (C ABI).
+//
+// Mirrors Engine::LoadGraph (include/framework/framework.cuh:99-133) and the
+// single:: CSR/CSC allocators (include/groute/graphs/csr_graph.h:1596-1783):
+// CSR and CSC are mirrored to HBM once and stay resident; an undirected graph
+// aliases its CSC to the CSR. Unlike the reference, no call exits the process.
+#include "sg_internal.hpp"
+
+#include <cstring>
+#include <vector>
+
+using namespace sg;
+
+namespace {
+
+// bit v of `inert` is set when vertex v has in-degree 0.
+__global__ void k_inert_bitmap(const uint32_t *cstart, uint32_t n, uint32_t words, uint64_t *inert)
+{
+    const uint32_t w = blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave;
+    if (w >= words) return;
+    const uint32_t v = w * 64 + lane();
+    bool zero = v < n && cstart[v + 1] == cstart[v];
+    uint64_t m = ballot(zero);
+    if (lane() == 0) inert[w] = m;
+}
+
+int dev_alloc(sg_graph *g, void **p, uint64_t bytes)
+{
+    if (g->nallocs >= 8) SG_FAIL(SG_ERR_STATE, "sg_graph_create: too many allocations");
+    hipError_t e = hipMalloc(p, bytes ? bytes : 4);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        sg::set_error(std::string("sg_graph_create: hipMalloc failed: ") + hipGetErrorString(e));
+        return SG_ERR_NOMEM;
+    }
+    g->allocs[g->nallocs++] = *p;
+    g->device_bytes += bytes;
+    return SG_OK;
+}
+
+}  // namespace
+
+extern "C" int sg_device_count(void)
+{
+    int c = 0;
+    if (hipGetDeviceCount(&c) != hipSuccess) {
+        (void)hipGetLastError();
+        return 0;
+    }
+    return c;
+}
+
+extern "C" int sg_graph_destroy(sg_graph *g)
+{
+    if (!g) return SG_OK;
+    for (int i = 0; i < g->nallocs; i++) (void)hipFree(g->allocs[i]);
+    delete g;
+    return SG_OK;
+}
+
+extern "C" int sg_graph_create(uint32_t n, uint64_t m, const uint32_t *row_start, const uint32_t *edge_dst,
+                               const uint32_t *weights, const uint32_t *col_start, const uint32_t *edge_src,
+                               const uint32_t *csc_weights, int undirected, sg_graph **out)
+{
+    if (!out) SG_FAIL(SG_ERR_INVALID, "sg_graph_create: null output handle");
+    *out = nullptr;
+    if (n == 0 || n >= (1u << 31)) SG_FAIL(SG_ERR_RANGE, "sg_graph_create: need 0 < n < 2^31");
+    if (m >= (1ull << 32)) SG_FAIL(SG_ERR_RANGE, "sg_graph_create: need m < 2^32");
+    if (!row_start || (m && !edge_dst)) SG_FAIL(SG_ERR_INVALID, "sg_graph_create: null CSR array");
+    if (row_start[0] != 0 || row_start[n] != m) SG_FAIL(SG_ERR_INVALID, "sg_graph_create: row_start[0] != 0 or row_start[n] != m");
+    for (uint32_t v = 0; v < n; v++)
+        if (row_start[v + 1] < row_start[v]) SG_FAIL(SG_ERR_INVALID, "sg_graph_create: row_start not monotone");
+    for (uint64_t e = 0; e < m; e++)
+        if (edge_dst[e] >= n) SG_FAIL(SG_ERR_RANGE, "sg_graph_create: edge_dst entry >= n");
+    const bool have_csc = col_start != nullptr;
+    if (!undirected && have_csc && (!edge_src || (weights && !csc_weights)))
+        SG_FAIL(SG_ERR_INVALID, "sg_graph_create: incomplete CSC");
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) {
+        (void)hipGetLastError();
+        SG_FAIL(SG_ERR_HIP, "sg_graph_create: no HIP device visible");
+    }
+    SG_HIP(hipSetDevice(0));
+
+    // Host CSC when the caller did not pass one (directed graphs only).
+    std::vector<uint32_t> h_cs, h_src, h_cw;
+    const uint32_t *cs = col_start, *src = edge_src, *cw = csc_weights;
+    if (!undirected && !have_csc) {
+        h_cs.resize((size_t)n + 1);
+        h_src.resize(m ? m : 1);
+        if (weights) h_cw.resize(m ? m : 1);
+        int rc = sg_csc_from_csr(n, row_start, edge_dst, weights, h_cs.data(), h_src.data(),
+                                 weights ? h_cw.data() : nullptr);
+        if (rc) return rc;
+        cs = h_cs.data();
+        src = h_src.data();
+        cw = weights ? h_cw.data() : nullptr;
+    }
+
+    sg_graph *g = new sg_graph();
+    g->undirected = undirected != 0;
+    g->weighted = weights != nullptr;
+    DevGraph &d = g->dev;
+    d.n = n;
+    d.m = m;
+    d.words = (n + 63) / 64;
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, 0) != hipSuccess) {
+        delete g;
+        SG_FAIL(SG_ERR_HIP, "sg_graph_create: hipGetDeviceProperties failed");
+    }
+    g->num_cus = prop.multiProcessorCount;
+
+    auto up = [&](const uint32_t *h, uint64_t count, const uint32_t **dptr) -> int {
+        void *p = nullptr;
+        int rc = dev_alloc(g, &p, count * sizeof(uint32_t));
+        if (rc) return rc;
+        if (count) SG_HIP(hipMemcpy(p, h, count * sizeof(uint32_t), hipMemcpyHostToDevice));
+        *dptr = (const uint32_t *)p;
+        return SG_OK;
+    };
+    int rc = up(row_start, (uint64_t)n + 1, &d.row);
+    if (!rc) rc = up(edge_dst, m, &d.col);
+    if (!rc && weights) rc = up(weights, m, &d.w);
+    if (!rc) {
+        if (undirected) {
+            d.cstart = d.row;
+            d.csrc = d.col;
+            d.cw = d.w;
+        } else {
+            rc = up(cs, (uint64_t)n + 1, &d.cstart);
+            if (!rc) rc = up(src, m, &d.csrc);
+            if (!rc && cw) rc = up(cw, m, &d.cw);
+        }
+    }
+    if (!rc) {
+        void *p = nullptr;
+        rc = dev_alloc(g, &p, (uint64_t)d.words * 8);
+        if (!rc) {
+            d.inert = (const uint64_t *)p;
+            const int waves_per_block = 4;
+            dim3 grid((d.words + waves_per_block - 1) / waves_per_block);
+            hipLaunchKernelGGL(k_inert_bitmap, grid, dim3(waves_per_block * kWave), 0, 0, d.cstart, n, d.words,
+                               (uint64_t *)p);
+            hipError_t e = hipDeviceSynchronize();
+            if (e != hipSuccess) {
+                sg::set_error(std::string("sg_graph_create: inert bitmap kernel: ") + hipGetErrorString(e));
+                rc = SG_ERR_HIP;
+            }
+        }
+    }
+    if (rc) {
+        sg_graph_destroy(g);
+        return rc;
+    }
+    if (weights && m) {
+        double s = 0;
+        for (uint64_t e = 0; e < m; e++) s += weights[e];
+        g->avg_weight = s / (double)m;
+    }
+    *out = g;
+    return SG_OK;
+}
+
+extern "C" int sg_graph_info(const sg_graph *g, uint32_t *n, uint64_t *m, uint64_t *device_bytes)
+{
+    if (!g) SG_FAIL(SG_ERR_INVALID, "sg_graph_info: null graph");
+    if (n) *n = g->dev.n;
+    if (m) *m = g->dev.m;
+    if (device_bytes) *device_bytes = g->device_bytes;
+    return SG_OK;
+}
+
+extern "C" void sg_engine_options_default(sg_engine_options *o)
+{
+    if (!o) return;
+    std::memset(o, 0, sizeof(*o));
+    o->force_variant = SG_VARIANT_AUTO;
+    o->fused = 0;
+    o->priority = SG_PRIO_NONE;
+    o->priority_threshold = 0.f;
+    o->cut_threshold = 0.f;
+    o->lb_push = SG_LB_DEFAULT;
+    o->lb_pull = SG_LB_DEFAULT;
+    o->alpha = 15.0;
+    o->beta = 18.0;
+    o->edge_factor = 1.0;
+    o->max_iteration = 0;
+    o->block_size = 0;
+    o->large_degree = 0;
+}
+
+extern "C" int sg_engine_create(sg_graph *g, sg_engine **out)
+{
+    if (!g || !out) SG_FAIL(SG_ERR_INVALID, "sg_engine_create: null argument");
+    *out = nullptr;
+    SG_HIP(hipSetDevice(0));
+    sg_engine *e = new sg_engine();
+    e->g = g;
+    sg_engine_options_default(&e->opt);
+    if (hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete e;
+        SG_FAIL(SG_ERR_HIP, "sg_engine_create: hipStreamCreate failed");
+    }
+    e->own_stream = true;
+    if (hipEventCreate(&e->ev0) != hipSuccess || hipEventCreate(&e->ev1) != hipSuccess) {
+        sg_engine_destroy(e);
+        SG_FAIL(SG_ERR_HIP, "sg_engine_create: hipEventCreate failed");
+    }
+    *out = e;
+    return SG_OK;
+}
+
+extern "C" int sg_engine_destroy(sg_engine *e)
+{
+    if (!e) return SG_OK;
+    if (e->stream) (void)hipStreamSynchronize(e->stream);
+    void *ptrs[] = {e->value, e->bitmaps, e->queues, e->large, e->ctr, e->bar_words, e->abort_flag, e->stats};
+    for (void *p : ptrs)
+        if (p) (void)hipFree(p);
+    if (e->ev0) (void)hipEventDestroy(e->ev0);
+    if (e->ev1) (void)hipEventDestroy(e->ev1);
+    if (e->own_stream && e->stream) (void)hipStreamDestroy(e->stream);
+    delete e;
+    return SG_OK;
+}
+
+extern "C" int sg_engine_set_stream(sg_engine *e, void *hip_stream)
+{
+    if (!e) SG_FAIL(SG_ERR_INVALID, "sg_engine_set_stream: null engine");
+    SG_HIP(hipStreamSynchronize(e->stream));
+    if (e->own_stream) (void)hipStreamDestroy(e->stream);
+    e->stream = (hipStream_t)hip_stream;
+    e->own_stream = false;
+    return SG_OK;
+}
+
+extern "C" int sg_engine_set_options(sg_engine *e, const sg_engine_options *o)
+{
+    if (!e || !o) SG_FAIL(SG_ERR_INVALID, "sg_engine_set_options: null argument");
+    if (o->force_variant < SG_VARIANT_AUTO || o->force_variant > SG_SYNC_PULL_DD)
+        SG_FAIL(SG_ERR_INVALID, "sg_engine_set_options: unknown variant");
+    if (o->alpha < 0 || o->beta <= 0) SG_FAIL(SG_ERR_INVALID, "sg_engine_set_options: need alpha >= 0, beta > 0");
+    if (o->block_size != 0 && (o->block_size % 64 || o->block_size < 64 || o->block_size > 1024))
+        SG_FAIL(SG_ERR_INVALID, "sg_engine_set_options: block_size must be a multiple of 64 in [64, 1024]");
+    if (o->max_iteration < 0) SG_FAIL(SG_ERR_INVALID, "sg_engine_set_options: max_iteration < 0");
+    int old_bs = e->opt.block_size;
+    e->opt = *o;
+    if (old_bs != o->block_size) e->coop_blocks = 0;  // re-derive the persistent grid
+    return SG_OK;
+}
+
+namespace sg {
+
+int engine_reset_barrier(sg_engine *e)
+{
+    SG_HIP(hipMemsetAsync(e->bar_words, 0, (size_t)e->bar_words_len * 4, e->stream));
+    SG_HIP(hipMemsetAsync(e->abort_flag, 0, 4, e->stream));
+    e->needs_bar_reset = false;
+    return SG_OK;
+}
+
+int engine_ensure_bfs_state(sg_engine *e)
+{
+    const DevGraph &d = e->g->dev;
+    if (e->bitmaps) return SG_OK;
+    auto alloc = [&](void **p, uint64_t bytes) -> int {
+        hipError_t err = hipMalloc(p, bytes ? bytes : 4);
+        if (err != hipSuccess) {
+            (void)hipGetLastError();
+            sg::set_error(std::string("engine state: hipMalloc failed: ") + hipGetErrorString(err));
+            return SG_ERR_NOMEM;
+        }
+        return SG_OK;
+    };
+    int rc;
+    e->value_bytes = (uint64_t)d.n * 4;
+    if ((rc = alloc(&e->value, e->value_bytes))) return rc;
+    if ((rc = alloc((void **)&e->bitmaps, (uint64_t)d.words * 8 * 4))) return rc;
+    if ((rc = alloc((void **)&e->queues, (uint64_t)d.n * 4 * 2))) return rc;
+    // Large-degree list: at most one entry per vertex.
+    e->large_cap = d.n;
+    if ((rc = alloc((void **)&e->large, (uint64_t)e->large_cap * sizeof(uint2) * 2))) return rc;
+    if ((rc = alloc((void **)&e->ctr, kCtrBytes))) return rc;
+    e->bar_words_len = kBarLine * (2 + 4096 / kBarGroup + 1);
+    if ((rc = alloc((void **)&e->bar_words, (uint64_t)e->bar_words_len * 4))) return rc;
+    if ((rc = alloc((void **)&e->abort_flag, 4))) return rc;
+    if ((rc = alloc((void **)&e->stats, sizeof(DevStats)))) return rc;
+    SG_HIP(hipMemsetAsync(e->stats, 0, sizeof(DevStats), e->stream));
+    e->needs_bar_reset = true;
+    return SG_OK;
+}
+
+}  // namespace sg
+
+extern "C" int sg_engine_sync(sg_engine *e)
+{
+    if (!e) SG_FAIL(SG_ERR_INVALID, "sg_engine_sync: null engine");
+    SG_HIP(hipStreamSynchronize(e->stream));
+    if (e->stats) {
+        DevStats s;
+        SG_HIP(hipMemcpy(&s, e->stats, sizeof(s), hipMemcpyDeviceToHost));
+        if (s.status != SG_OK) {
+            e->needs_bar_reset = true;
+            SG_FAIL(s.status, s.status == SG_ERR_TIMEOUT ? "device: grid barrier wait expired"
+                                                         : "device: run reported an error");
+        }
+    }
+    return SG_OK;
+}
+
+extern "C" int sg_engine_value_ptr(sg_engine *e, void **dev_ptr, uint64_t *bytes)
+{
+    if (!e || !dev_ptr) SG_FAIL(SG_ERR_INVALID, "sg_engine_value_ptr: null argument");
+    if (!e->value) SG_FAIL(SG_ERR_STATE, "sg_engine_value_ptr: nothing has run yet");
+    *dev_ptr = e->value;
+    if (bytes) *bytes = e->value_bytes;
+    return SG_OK;
+}
+
+extern "C" int sg_engine_gather_value(sg_engine *e, void *host_out, uint64_t bytes)
+{
+    if (!e || !host_out) SG_FAIL(SG_ERR_INVALID, "sg_engine_gather_value: null argument");
+    if (!e->value) SG_FAIL(SG_ERR_STATE, "sg_engine_gather_value: nothing has run yet");
+    if (bytes < e->value_bytes) SG_FAIL(SG_ERR_CAPACITY, "sg_engine_gather_value: host buffer too small");
+    int rc = sg_engine_sync(e);
+    if (rc) return rc;
+    SG_HIP(hipMemcpy(host_out, e->value, e->value_bytes, hipMemcpyDeviceToHost));
+    return SG_OK;
+}
+
+extern "C" int sg_engine_get_stats(sg_engine *e, sg_stats *out)
+{
+    if (!e || !out) SG_FAIL(SG_ERR_INVALID, "sg_engine_get_stats: null argument");
+    std::memset(out, 0, sizeof(*out));
+    if (!e->stats) SG_FAIL(SG_ERR_STATE, "sg_engine_get_stats: nothing has run yet");
+    SG_HIP(hipStreamSynchronize(e->stream));
+    DevStats s;
+    SG_HIP(hipMemcpy(&s, e->stats, sizeof(s), hipMemcpyDeviceToHost));
+    out->rounds = s.rounds;
+    out->push_rounds = s.push_rounds;
+    out->pull_rounds = s.pull_rounds;
+    out->reached = s.reached;
+    out->reached_edges = s.reached_edges;
+    out->policy_trace = s.trace;
+    out->max_depth = s.max_depth;
+    out->status = s.status;
+    out->iterations = s.iterations;
+    out->residual = s.residual;
+    if (e->timed) {
+        float ms = 0.f;
+        SG_HIP(hipEventElapsedTime(&ms, e->ev0, e->ev1));
+        out->kernel_ms = ms;
+    }
+    return SG_OK;
+}

@@ -1,0 +1,341 @@
+"""ctypes binding of libsepgraph.so (the C ABI declared in include/sepgraph.h).
+
+This module is plumbing for tests, the smoke check and bench.py: every call
+goes straight to the native library, which runs on the GPU. There is no CPU
+fallback: if the library is missing or no GPU is visible, the GPU entry points
+raise.
+
+The life cycle mirrors the reference's Engine (include/framework/framework.cuh:56-797):
+Graph(...) = LoadGraph, Engine.set_options = SetOptions, Engine.bfs(src) =
+InitGraph(src) + Start(), Engine.gather() = GatherValue().
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from dataclasses import dataclass
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "lib", "libsepgraph.so")
+
+SG_OK = 0
+ERRORS = {
+    -1: "SG_ERR_INVALID",
+    -2: "SG_ERR_RANGE",
+    -3: "SG_ERR_NOMEM",
+    -4: "SG_ERR_HIP",
+    -5: "SG_ERR_CAPACITY",
+    -6: "SG_ERR_TIMEOUT",
+    -7: "SG_ERR_STATE",
+    -8: "SG_ERR_UNSUPPORTED",
+}
+CSR_SYMMETRIZE, CSR_DROP_SELF_LOOPS, CSR_DEDUP = 1, 2, 4
+
+VARIANTS = {
+    "auto": -1,
+    "async_push_td": 0,
+    "async_push_dd": 1,
+    "async_pull_td": 2,
+    "async_pull_dd": 3,
+    "sync_push_td": 4,
+    "sync_push_dd": 5,
+    "sync_pull_td": 6,
+    "sync_pull_dd": 7,
+}
+
+
+class SepGraphError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"{ERRORS.get(code, code)}: {msg}")
+        self.code = code
+
+
+class EngineOptions(C.Structure):
+    """sg_engine_options (EngineOptions, include/framework/common.h:203-321)."""
+
+    _fields_ = [
+        ("force_variant", C.c_int32),
+        ("fused", C.c_int32),
+        ("priority", C.c_int32),
+        ("priority_threshold", C.c_float),
+        ("cut_threshold", C.c_float),
+        ("lb_push", C.c_int32),
+        ("lb_pull", C.c_int32),
+        ("alpha", C.c_double),
+        ("beta", C.c_double),
+        ("edge_factor", C.c_double),
+        ("max_iteration", C.c_int32),
+        ("block_size", C.c_int32),
+        ("large_degree", C.c_int32),
+        ("reserved", C.c_int32 * 8),
+    ]
+
+
+class Stats(C.Structure):
+    _fields_ = [
+        ("rounds", C.c_uint32),
+        ("push_rounds", C.c_uint32),
+        ("pull_rounds", C.c_uint32),
+        ("reached", C.c_uint32),
+        ("reached_edges", C.c_uint64),
+        ("policy_trace", C.c_uint64),
+        ("kernel_ms", C.c_double),
+        ("residual", C.c_double),
+        ("iterations", C.c_uint32),
+        ("max_depth", C.c_uint32),
+        ("status", C.c_int32),
+        ("reserved", C.c_uint32 * 7),
+    ]
+
+    def as_dict(self) -> dict:
+        return {k: getattr(self, k) for k, _ in self._fields_ if k != "reserved"}
+
+
+_P = C.c_void_p
+_U32P = C.POINTER(C.c_uint32)
+_lib = None
+
+
+def lib() -> C.CDLL:
+    """Load libsepgraph.so once; raise if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise FileNotFoundError(f"{LIB_PATH} not built (run __graft_entry__.build())")
+        L = C.CDLL(LIB_PATH)
+        sig = {
+            "sg_last_error": ([], C.c_char_p),
+            "sg_version": ([], C.c_int),
+            "sg_device_count": ([], C.c_int),
+            "sg_gen_rmat_edges": ([C.c_int, C.c_int, C.c_double, C.c_double, C.c_double, C.c_uint64, _P, _P], C.c_int),
+            "sg_gen_grid_edges": ([C.c_uint32, C.c_uint32, C.c_double, C.c_uint64, _P, _P, C.POINTER(C.c_uint64)], C.c_int),
+            "sg_build_csr": ([C.c_uint32, C.c_uint64, _P, _P, C.c_int, _P, _P, C.c_uint64, C.POINTER(C.c_uint64)], C.c_int),
+            "sg_gen_weights": ([C.c_uint32, _P, _P, C.c_uint32, C.c_uint32, C.c_uint64, C.c_int, _P], C.c_int),
+            "sg_csc_from_csr": ([C.c_uint32, _P, _P, _P, _P, _P, _P], C.c_int),
+            "sg_pick_sources": ([C.c_uint32, _P, C.c_int, C.c_uint64, _P], C.c_int),
+            "sg_graph_create": ([C.c_uint32, C.c_uint64, _P, _P, _P, _P, _P, _P, C.c_int, C.POINTER(_P)], C.c_int),
+            "sg_graph_destroy": ([_P], C.c_int),
+            "sg_graph_info": ([_P, _U32P, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)], C.c_int),
+            "sg_engine_options_default": ([C.POINTER(EngineOptions)], None),
+            "sg_engine_create": ([_P, C.POINTER(_P)], C.c_int),
+            "sg_engine_destroy": ([_P], C.c_int),
+            "sg_engine_set_stream": ([_P, _P], C.c_int),
+            "sg_engine_set_options": ([_P, C.POINTER(EngineOptions)], C.c_int),
+            "sg_bfs": ([_P, C.c_uint32], C.c_int),
+            "sg_engine_sync": ([_P], C.c_int),
+            "sg_engine_value_ptr": ([_P, C.POINTER(_P), C.POINTER(C.c_uint64)], C.c_int),
+            "sg_engine_gather_value": ([_P, _P, C.c_uint64], C.c_int),
+            "sg_engine_get_stats": ([_P, C.POINTER(Stats)], C.c_int),
+        }
+        for name, (args, res) in sig.items():
+            f = getattr(L, name)
+            f.argtypes = args
+            f.restype = res
+        _lib = L
+    return _lib
+
+
+def declared_symbols() -> list[str]:
+    """Names of every function declared in include/sepgraph.h."""
+    import re
+
+    hdr = os.path.join(_HERE, "..", "include", "sepgraph.h")
+    text = open(hdr).read()
+    return sorted(set(re.findall(r"\b(sg_[a-z0-9_]+)\s*\(", text)))
+
+
+def _check(rc: int) -> None:
+    if rc != SG_OK:
+        raise SepGraphError(rc, lib().sg_last_error().decode())
+
+
+def _ptr(a: np.ndarray | None):
+    return None if a is None else a.ctypes.data_as(_P)
+
+
+def _u32(a) -> np.ndarray:
+    return np.ascontiguousarray(a, dtype=np.uint32)
+
+
+# ------------------------------------------------------------------ generators
+def gen_rmat_edges(scale: int, edge_factor: int = 16, a=0.57, b=0.19, c=0.19, seed: int = 1):
+    count = edge_factor << scale
+    src = np.empty(count, np.uint32)
+    dst = np.empty(count, np.uint32)
+    _check(lib().sg_gen_rmat_edges(scale, edge_factor, a, b, c, seed, _ptr(src), _ptr(dst)))
+    return src, dst
+
+
+def gen_grid_edges(rows: int, cols: int, p_delete: float = 0.1, seed: int = 1):
+    cap = 2 * rows * cols
+    src = np.empty(cap, np.uint32)
+    dst = np.empty(cap, np.uint32)
+    cnt = C.c_uint64()
+    _check(lib().sg_gen_grid_edges(rows, cols, p_delete, seed, _ptr(src), _ptr(dst), C.byref(cnt)))
+    return src[: cnt.value], dst[: cnt.value]
+
+
+def build_csr(n: int, src, dst, symmetrize=False, drop_self_loops=True, dedup=True):
+    src, dst = _u32(src), _u32(dst)
+    flags = (CSR_SYMMETRIZE if symmetrize else 0) | (CSR_DROP_SELF_LOOPS if drop_self_loops else 0)
+    flags |= CSR_DEDUP if dedup else 0
+    cap = len(src) * (2 if symmetrize else 1)
+    row = np.empty(n + 1, np.uint32)
+    col = np.empty(max(cap, 1), np.uint32)
+    m = C.c_uint64()
+    _check(lib().sg_build_csr(n, len(src), _ptr(src), _ptr(dst), flags, _ptr(row), _ptr(col), cap, C.byref(m)))
+    return row, col[: m.value].copy()
+
+
+def gen_weights(row, col, lo: int, hi: int, seed: int = 2, symmetric=False):
+    row, col = _u32(row), _u32(col)
+    w = np.empty(max(len(col), 1), np.uint32)
+    _check(lib().sg_gen_weights(len(row) - 1, _ptr(row), _ptr(col), lo, hi, seed, int(symmetric), _ptr(w)))
+    return w[: len(col)]
+
+
+def csc_from_csr(row, col, w=None):
+    row, col = _u32(row), _u32(col)
+    n = len(row) - 1
+    cs = np.empty(n + 1, np.uint32)
+    src = np.empty(max(len(col), 1), np.uint32)
+    cw = None if w is None else np.empty(max(len(col), 1), np.uint32)
+    _check(lib().sg_csc_from_csr(n, _ptr(row), _ptr(col), _ptr(None if w is None else _u32(w)), _ptr(cs), _ptr(src), _ptr(cw)))
+    return cs, src[: len(col)], (None if cw is None else cw[: len(col)])
+
+
+def pick_sources(row, k: int, seed: int = 3):
+    row = _u32(row)
+    out = np.empty(k, np.uint32)
+    _check(lib().sg_pick_sources(len(row) - 1, _ptr(row), k, seed, _ptr(out)))
+    return out
+
+
+@dataclass
+class HostGraph:
+    """A host CSR (+ optional weights) with the metadata of its generator."""
+
+    n: int
+    row: np.ndarray
+    col: np.ndarray
+    w: np.ndarray | None = None
+    undirected: bool = False
+    name: str = ""
+
+    @property
+    def m(self) -> int:
+        return int(self.row[-1])
+
+
+def rmat_graph(scale: int, edge_factor: int = 16, symmetrize=False, seed=1, weights=None, weight_seed=2) -> HostGraph:
+    """RMAT (a,b,c,d)=(0.57,0.19,0.19,0.05), self-loops and duplicates removed."""
+    s, d = gen_rmat_edges(scale, edge_factor, seed=seed)
+    n = 1 << scale
+    row, col = build_csr(n, s, d, symmetrize=symmetrize)
+    del s, d
+    w = None
+    if weights is not None:
+        w = gen_weights(row, col, weights[0], weights[1], seed=weight_seed, symmetric=symmetrize)
+    kind = "sym" if symmetrize else "dir"
+    return HostGraph(n, row, col, w, undirected=symmetrize, name=f"rmat{scale}-ef{edge_factor}-{kind}")
+
+
+def grid_graph(rows=2048, cols=2048, p_delete=0.1, seed=1, weights=(1, 1000), weight_seed=2) -> HostGraph:
+    """4-neighbour grid, each undirected edge deleted with probability p_delete."""
+    s, d = gen_grid_edges(rows, cols, p_delete, seed)
+    n = rows * cols
+    row, col = build_csr(n, s, d, symmetrize=True)
+    w = None if weights is None else gen_weights(row, col, weights[0], weights[1], seed=weight_seed, symmetric=True)
+    return HostGraph(n, row, col, w, undirected=True, name=f"grid{rows}x{cols}-p{p_delete}")
+
+
+# --------------------------------------------------------------- device side
+class Graph:
+    """A device-resident graph (sg_graph)."""
+
+    def __init__(self, hg: HostGraph, with_weights: bool = True):
+        self.host = hg
+        h = _P()
+        w = _u32(hg.w) if (with_weights and hg.w is not None) else None
+        row, col = _u32(hg.row), _u32(hg.col)
+        _check(lib().sg_graph_create(hg.n, hg.m, _ptr(row), _ptr(col), _ptr(w), None, None, None, int(hg.undirected), C.byref(h)))
+        self.h = h
+
+    def info(self):
+        n, m, b = C.c_uint32(), C.c_uint64(), C.c_uint64()
+        _check(lib().sg_graph_info(self.h, C.byref(n), C.byref(m), C.byref(b)))
+        return n.value, m.value, b.value
+
+    def close(self):
+        if self.h:
+            lib().sg_graph_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def default_options(**kw) -> EngineOptions:
+    o = EngineOptions()
+    lib().sg_engine_options_default(C.byref(o))
+    for k, v in kw.items():
+        if k == "force_variant" and isinstance(v, str):
+            v = VARIANTS[v]
+        setattr(o, k, v)
+    return o
+
+
+class Engine:
+    """sg_engine bound to one Graph."""
+
+    def __init__(self, graph: Graph, stream: int | None = None, **opts):
+        self.graph = graph
+        h = _P()
+        _check(lib().sg_engine_create(graph.h, C.byref(h)))
+        self.h = h
+        if stream is not None:
+            _check(lib().sg_engine_set_stream(self.h, _P(stream)))
+        if opts:
+            self.set_options(**opts)
+
+    def set_options(self, **kw):
+        o = default_options(**kw)
+        _check(lib().sg_engine_set_options(self.h, C.byref(o)))
+
+    def bfs(self, source: int):
+        _check(lib().sg_bfs(self.h, int(source)))
+
+    def sync(self):
+        _check(lib().sg_engine_sync(self.h))
+
+    def value_ptr(self):
+        p, b = _P(), C.c_uint64()
+        _check(lib().sg_engine_value_ptr(self.h, C.byref(p), C.byref(b)))
+        return p.value, b.value
+
+    def gather(self, dtype=np.int32) -> np.ndarray:
+        _, nbytes = self.value_ptr()
+        out = np.empty(nbytes // np.dtype(dtype).itemsize, dtype)
+        _check(lib().sg_engine_gather_value(self.h, _ptr(out), out.nbytes))
+        return out
+
+    def stats(self) -> dict:
+        s = Stats()
+        _check(lib().sg_engine_get_stats(self.h, C.byref(s)))
+        return s.as_dict()
+
+    def close(self):
+        if self.h:
+            lib().sg_engine_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
