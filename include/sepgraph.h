@@ -173,6 +173,17 @@ typedef struct sg_stats {
     uint32_t reserved[7];
 } sg_stats;
 
+/* One row of the per-round trace (the reference's --trace line:
+ * round, policy, in-nodes, in-edges, time; framework.cuh:334-348). */
+typedef struct sg_round_trace {
+    uint32_t round;
+    int32_t variant;      /* SG_ASYNC_PUSH_DD / SG_ASYNC_PULL_DD ...; -1 on the final row */
+    uint32_t in_nodes;    /* frontier size at the start of the round */
+    uint32_t pad;
+    uint64_t in_edges;    /* sum of out-degree over that frontier */
+    double time_us;       /* device wall clock from kernel start to the start of the round */
+} sg_round_trace;
+
 int sg_engine_create(sg_graph *g, sg_engine **out);
 int sg_engine_destroy(sg_engine *e);
 int sg_engine_set_stream(sg_engine *e, void *hip_stream);
@@ -191,6 +202,9 @@ int sg_engine_value_ptr(sg_engine *e, void **dev_ptr, uint64_t *bytes);
 int sg_engine_gather_value(sg_engine *e, void *host_out, uint64_t bytes);
 /* Statistics of the last run (synchronises). */
 int sg_engine_get_stats(sg_engine *e, sg_stats *out);
+/* Per-round trace of the last traversal (synchronises): up to `max` rows,
+ * the last one (variant -1) marking the end; *n_out gets the rows written. */
+int sg_engine_get_trace(sg_engine *e, sg_round_trace *out, int max, int *n_out);
 
 #ifdef __cplusplus
 }

@@ -1,4 +1,4 @@
-// Hybrid (direction-switching) BFS as one persistent cooperative kernel.
+// Hybrid (direction-switching) BFS as one persistent kernel per query.
 //
 // Reference path (samples/hybrid_bfs/hybrid_bfs.cu:173-243 driving
 // Engine::Start, include/framework/framework.cuh:315-375): each round the host
@@ -11,21 +11,25 @@
 //
 // Here the whole traversal is one launch. Every block evaluates the same
 // alpha/beta rule on device from counters read after a grid barrier, so no
-// round needs the host. Frontier state per round r:
-//   fbm[r%3]   bitmap of the frontier (always maintained; pull reads it)
-//   q[r%2]     queue of frontier vertices with out-degree <= large_degree
-//              (only produced by push rounds)
-//   lq[r%2]    list of frontier vertices with out-degree > large_degree, each
-//              with a prefix of its chunk count, so a push round splits their
-//              adjacency into fixed-size chunks spread over every wave
-//   ctr[r%3]   counters (queue lengths, striped count/scout sums)
-// Push round: large-list chunks, then the small queue (or, right after a pull
-// round, the frontier bitmap) with wave-level prefix-sum edge scheduling;
-// claims go through atomicOr on the visited bitmap, so every vertex enters a
-// frontier exactly once (the reference instead tolerates duplicates from a
-// non-atomic min, hybrid_bfs.cu:128-134). Pull round: one wave per 64-vertex
-// bitmap word; unvisited lanes scan CSC in-edges serially for a few edges,
-// then wave-cooperatively, stopping at the first parent in the frontier.
+// round needs the host. The level array is the only per-vertex state a push
+// round writes:
+//   * discovery is a plain check-and-store, level[v] == -1 -> L+1. Racing
+//     writers store the same value (the reference's non-atomic BFS min,
+//     hybrid_bfs.cu:128-134, on by default, hybrid_bfs/main.cpp:37), so a
+//     vertex may enter the next queue more than once, exactly as in the
+//     reference, and frontier membership is level[u] == L;
+//   * the next queue is staged in LDS per block and published with one
+//     atomic per block per round (same-address atomics serialise on MI355X);
+//   * vertices with out-degree > kHugeDegree are expanded through the graph's
+//     static chunk list, filtered by membership, so hub adjacencies spread
+//     over every wave without building any list at run time.
+// A pull round sweeps 64-vertex words (one wave per word at a time, each wave
+// owning a contiguous range); every unvisited lane first tests its first
+// in-edge, then the wave walks the concatenated remaining in-edge lists 64
+// edges per step, re-planning after each step so a vertex stops at its first
+// parent (the reference's early break, pull_functor.h:138-188). Pull rounds
+// write whole words of the next frontier bitmap and of a visited filter, so
+// they need no atomics either.
 #include "sg_internal.hpp"
 
 using namespace sg;
@@ -33,7 +37,6 @@ using namespace sg;
 namespace {
 
 constexpr int kStripes = 16;
-constexpr uint32_t kPullSerial = 8;  // in-edges scanned per lane before going wave-cooperative
 
 struct alignas(128) Stripe {
     unsigned long long scout;
@@ -42,29 +45,29 @@ struct alignas(128) Stripe {
 };
 
 struct alignas(128) BfsCtr {
-    uint32_t nsmall;
-    uint32_t pad0;
-    unsigned long long large;  // (entries << 32) | chunks
-    uint32_t pad1[28];
+    uint32_t qlen;      // entries appended to this round's queue
+    uint32_t overflow;  // set when appends ran past the queue capacity
+    uint32_t has_huge;  // the frontier holds a vertex above kHugeDegree
+    uint32_t pad[29];
     Stripe stripe[kStripes];
 };
 static_assert(sizeof(BfsCtr) == 128 * (1 + kStripes), "BfsCtr layout");
+static_assert(sizeof(BfsCtr) * 3 <= kCtrBytes, "BFS counters exceed the engine's counter area");
 
 struct BfsArgs {
     DevGraph g;
     int32_t *level;
-    uint64_t *vis;
-    uint64_t *fbm;  // 3 * words
-    uint32_t *q0, *q1;
-    uint2 *lq0, *lq1;
-    BfsCtr *ctr;    // 3 slots
+    uint64_t *vis;    // W words: visited filter kept by pull rounds (set bit => visited)
+    uint64_t *fbit;   // 2 * W words: frontier bitmaps written by pull rounds
+    uint32_t *queue;  // 3 * qcap entries
+    uint32_t qcap;
+    BfsCtr *ctr;      // 3 slots
     GridBarrier bar;
     uint32_t *abort;
     DevStats *stats;
+    DevTrace *trace;
     uint32_t source;
-    uint32_t large_deg;
-    uint32_t chunk;
-    int32_t force;  // -1 auto, 0 push, 1 pull
+    int32_t force;    // -1 auto, 0 push, 1 pull
     double alpha, beta;
     uint32_t total_workload0;
     uint32_t max_rounds;
@@ -82,47 +85,188 @@ __device__ void clear_ctr(BfsCtr *c)
     for (uint32_t i = threadIdx.x; i < sizeof(BfsCtr) / 4; i += blockDim.x) w[i] = 0;
 }
 
-struct Emit {
+__device__ __forceinline__ bool test_bit(const uint64_t *bm, uint32_t u) { return (bm[u >> 6] >> (u & 63)) & 1; }
+
+// Index of the lane owning position j of a wave-wide concatenation whose
+// inclusive prefix sums are `incl` (the number of lanes with incl <= j).
+__device__ __forceinline__ int owner_lane(uint32_t incl, uint32_t j)
+{
+    int k = 0;
+#pragma unroll
+    for (int step = 32; step > 0; step >>= 1) {
+        const uint32_t t = shfl(incl, k + step - 1);
+        if (t <= j) k += step;
+    }
+    return k;
+}
+
+struct Round {
+    const BfsArgs *a;
+    int32_t L;              // level being expanded
+    bool fbit_valid;        // frontier L is given by fcur (it came from a pull round)
+    const uint64_t *fcur;
+    uint64_t *fnext;
     uint32_t *qout;
-    uint2 *lout;
     BfsCtr *nxt;
-    uint32_t large_deg, chunk;
+    uint32_t *s_q;          // LDS staging of the next queue
+    uint32_t *s_qn;
+    uint32_t lds_cap;
+    unsigned long long scout, count;  // this lane's share of the next frontier
+    bool huge;                        // this lane discovered a vertex above kHugeDegree
 };
 
-// Route newly discovered vertices of this wave into the next frontier's small
-// queue or large list (one atomic per wave per list). Wave-convergent.
-__device__ __forceinline__ void emit_wave(const Emit &em, bool found, uint32_t v, uint32_t dv, bool want_queue)
+__device__ __forceinline__ bool in_frontier(const Round &R, uint32_t u)
 {
-    const bool big = found && dv > em.large_deg;
-    if (want_queue) wave_append(found && !big, v, em.qout, &em.nxt->nsmall);
-    uint64_t bm = ballot(big);
-    if (bm) {
-        uint32_t nch = big ? (dv + em.chunk - 1) / em.chunk : 0;
-        uint32_t incl = wave_incl_scan(nch);
-        uint32_t tot = shfl(incl, kWave - 1);
-        int leader = first_lane(bm);
-        unsigned long long old = 0;
-        if (lane() == leader)
-            old = atomicAdd(&em.nxt->large, ((unsigned long long)__popcll(bm) << 32) | tot);
-        old = shfl(old, leader);
-        if (big) em.lout[(uint32_t)(old >> 32) + rank_in(bm)] = make_uint2(v, (uint32_t)old + incl - nch);
+    return R.fbit_valid ? test_bit(R.fcur, u) : R.a->level[u] == R.L;
+}
+
+// Stage this wave's discoveries for the next queue: into the block's LDS
+// buffer while it has room, the remainder straight to the global queue.
+__device__ __forceinline__ void stage(Round &R, bool found, uint32_t v)
+{
+    const uint64_t m = ballot(found);
+    if (!m) return;
+    const int leader = first_lane(m);
+    uint32_t base = 0;
+    if (lane() == leader) base = atomicAdd(R.s_qn, (uint32_t)__popcll(m));
+    base = shfl(base, leader);
+    const uint32_t slot = base + rank_in(m);
+    const bool to_lds = found && slot < R.lds_cap;
+    if (to_lds) R.s_q[slot] = v;
+    const bool to_glb = found && !to_lds;
+    const uint64_t gm = ballot(to_glb);
+    if (gm) {
+        const int l2 = first_lane(gm);
+        uint32_t gb = 0;
+        if (lane() == l2) gb = atomicAdd(&R.nxt->qlen, (uint32_t)__popcll(gm));
+        gb = shfl(gb, l2);
+        const uint32_t pos = gb + rank_in(gm);
+        if (to_glb) {
+            if (pos < R.a->qcap) R.qout[pos] = v;
+            else R.nxt->overflow = 1;
+        }
     }
 }
 
-// Claim v (not yet visited) for the next level; returns true for the unique winner.
-__device__ __forceinline__ bool claim(uint64_t *vis, uint32_t v)
+// Push edge -> v for `act` lanes; wave-convergent.
+__device__ __forceinline__ void push_visit(Round &R, bool act, uint32_t v)
 {
-    const uint64_t bit = 1ull << (v & 63);
-    unsigned long long *wp = (unsigned long long *)&vis[v >> 6];
-    if (*wp & bit) return false;
-    unsigned long long old = atomicOr(wp, (unsigned long long)bit);
-    return !(old & bit);
+    const BfsArgs &a = *R.a;
+    bool found = false;
+    if (act && a.level[v] == -1) {
+        a.level[v] = R.L + 1;
+        const uint32_t dv = a.g.row[v + 1] - a.g.row[v];
+        R.scout += dv;
+        R.count++;
+        R.huge |= dv > kHugeDegree;
+        found = true;
+    }
+    stage(R, found, v);
+}
+
+// Expand the out-edges of up to 64 vertices (one per lane) with wave-level
+// prefix-sum scheduling: 64 consecutive edges of the concatenated adjacency
+// per step, so CSR reads are coalesced and lanes stay busy. Vertices above
+// kHugeDegree are left to the chunk list.
+__device__ __forceinline__ void push_expand(Round &R, bool valid, uint32_t u)
+{
+    const DevGraph &g = R.a->g;
+    uint32_t b = 0, d = 0;
+    if (valid) {
+        b = g.row[u];
+        d = g.row[u + 1] - b;
+        if (d > kHugeDegree) d = 0;
+    }
+    const uint32_t incl = wave_incl_scan(d);
+    const uint32_t total = shfl(incl, kWave - 1);
+    const uint32_t off = b - (incl - d);
+    const int ln = lane();
+    for (uint32_t jb = 0; jb < total; jb += kWave) {
+        const uint32_t j = jb + ln;
+        const bool act = j < total;
+        const int k = owner_lane(incl, j);
+        const uint32_t e = shfl(off, k) + j;
+        push_visit(R, act, act ? g.col[e] : 0u);
+    }
+}
+
+// Bottom-up step for the 64 vertices of each of ng (<= G) words wg[i] (vwg[i]:
+// visited filter word). The first-edge tests of all G words are issued
+// together so their dependent loads overlap; the chunked walks follow per word.
+template <int G>
+__device__ __forceinline__ void pull_group(Round &R, const uint32_t (&wg)[G], const uint64_t (&vwg)[G], int ng)
+{
+    const BfsArgs &a = *R.a;
+    const DevGraph &g = a.g;
+    const int ln = lane();
+    uint32_t v[G], b[G], e[G], u[G];
+    int32_t lv[G];
+    uint64_t visited[G];
+    bool found[G];
+#pragma unroll
+    for (int i = 0; i < G; i++) {
+        v[i] = wg[i] * 64 + ln;
+        lv[i] = (i < ng && v[i] < g.n) ? a.level[v[i]] : 0;
+    }
+#pragma unroll
+    for (int i = 0; i < G; i++) {
+        visited[i] = i < ng ? (ballot(v[i] >= g.n || lv[i] != -1) | vwg[i]) : ~0ull;
+        const bool want = !((visited[i] >> ln) & 1);
+        b[i] = want ? g.cstart[v[i]] : 0u;
+        e[i] = want ? g.cstart[v[i] + 1] : 0u;
+    }
+#pragma unroll
+    for (int i = 0; i < G; i++) u[i] = b[i] < e[i] ? g.csrc[b[i]] : 0u;
+#pragma unroll
+    for (int i = 0; i < G; i++) {
+        found[i] = b[i] < e[i] && in_frontier(R, u[i]);
+        if (b[i] < e[i]) b[i]++;
+    }
+#pragma unroll
+    for (int i = 0; i < G; i++) {
+        for (;;) {
+            const uint32_t r = found[i] ? 0u : e[i] - b[i];
+            const uint32_t incl = wave_incl_scan(r);
+            const uint32_t total = shfl(incl, kWave - 1);
+            if (total == 0) break;
+            const uint32_t excl = incl - r;
+            const uint32_t j = (uint32_t)ln;
+            const int k = owner_lane(incl, j);
+            const uint32_t bk = shfl(b[i], k), xk = shfl(excl, k);
+            bool hit = false;
+            if (j < total) hit = in_frontier(R, g.csrc[bk + (j - xk)]);
+            const uint64_t H = ballot(hit);
+            const uint32_t lo = min(excl, 64u), hi = min(incl, 64u), cov = hi - lo;
+            if (cov) {
+                const uint64_t msk = (cov == 64 ? ~0ull : ((1ull << cov) - 1)) << lo;
+                if (H & msk) found[i] = true;
+                b[i] += cov;
+            }
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < G; i++) {
+        if (i >= ng) break;
+        const uint64_t fm = ballot(found[i]);
+        if (ln == 0) {
+            a.vis[wg[i]] = visited[i] | fm;
+            R.fnext[wg[i]] = fm;
+            R.count += __popcll(fm);
+        }
+        if (found[i]) {
+            a.level[v[i]] = R.L + 1;
+            const uint32_t dv = g.row[v[i] + 1] - g.row[v[i]];
+            R.scout += dv;
+            R.huge |= dv > kHugeDegree;
+        }
+    }
 }
 
 template <int BS>
 __global__ __launch_bounds__(BS) void k_bfs(BfsArgs a)
 {
     constexpr int WPB = BS / kWave;
+    constexpr uint32_t kLdsQ = 8u * BS;
     const DevGraph &g = a.g;
     const uint32_t n = g.n, W = g.words;
     const uint32_t gtid = blockIdx.x * BS + threadIdx.x;
@@ -131,10 +275,17 @@ __global__ __launch_bounds__(BS) void k_bfs(BfsArgs a)
     const uint32_t gwave = blockIdx.x * WPB + wid;
     const uint32_t nwaves = gridDim.x * WPB;
     const int ln = lane();
-    uint64_t *fbm0 = a.fbm, *fbm1 = a.fbm + W, *fbm2 = a.fbm + 2 * (size_t)W;
+    const uint32_t xcc = xcc_id();
+    uint32_t gen = threadIdx.x == 0 ? grid_gen(a.bar) : 0u;
+    const long long t_start = wall_clock64();
+    // contiguous range of bitmap words swept by this wave
+    const uint32_t wpw = (W + nwaves - 1) / nwaves;
+    const uint32_t wbeg = min(W, gwave * wpw), wend = min(W, wbeg + wpw);
 
     __shared__ unsigned long long s_red[2][WPB];
-    __shared__ unsigned long long s_lvl[4];
+    __shared__ unsigned long long s_lvl[5];
+    __shared__ uint32_t s_q[kLdsQ];
+    __shared__ uint32_t s_qn, s_flush[2];
 
     // ---- prologue: reset per-query state, seed the source (one barrier)
     const uint32_t src = a.source;
@@ -142,9 +293,6 @@ __global__ __launch_bounds__(BS) void k_bfs(BfsArgs a)
     for (uint32_t w = gtid; w < W; w += nthreads) {
         const uint64_t sb = (w == (src >> 6)) ? (1ull << (src & 63)) : 0ull;
         a.vis[w] = (g.inert ? g.inert[w] : 0ull) | sb;
-        fbm0[w] = sb;
-        fbm1[w] = 0;
-        fbm2[w] = 0;
     }
     if (blockIdx.x == 0) {
         clear_ctr(&a.ctr[0]);
@@ -153,46 +301,47 @@ __global__ __launch_bounds__(BS) void k_bfs(BfsArgs a)
         __syncthreads();
         if (threadIdx.x == 0) {
             const uint32_t deg = g.row[src + 1] - g.row[src];
-            if (deg > a.large_deg) {
-                a.lq0[0] = make_uint2(src, 0u);
-                a.ctr[0].large = (1ull << 32) | ((deg + a.chunk - 1) / a.chunk);
-            } else {
-                a.q0[0] = src;
-                a.ctr[0].nsmall = 1;
-            }
+            a.queue[0] = src;
+            a.ctr[0].qlen = 1;
+            a.ctr[0].has_huge = deg > kHugeDegree;
             a.ctr[0].stripe[0].scout = deg;
             a.ctr[0].stripe[0].count = 1;
         }
     }
-    bool ok = grid_sync(a.bar, a.abort);
+    bool ok = grid_sync(a.bar, xcc, gen, a.abort);
 
     // ---- replicated policy state (identical in every block)
     bool pull_prev = false;
     uint32_t total_wl = a.total_workload0;
-    unsigned long long trace = 0, reached_edges = 0;
-    uint32_t pushes = 0, pulls = 0, reached = 0;
+    unsigned long long trace = 0;
+    uint32_t pushes = 0, pulls = 0;
     uint32_t L = 0;
     while (ok) {
         BfsCtr *cur = &a.ctr[L % 3];
-        if (threadIdx.x == 0) {
+        if (threadIdx.x < kWave) {
             unsigned long long cnt = 0, sc = 0;
-            for (int s = 0; s < kStripes; s++) {
-                cnt += ld_relaxed64(&cur->stripe[s].count);
-                sc += ld_relaxed64(&cur->stripe[s].scout);
+            if (ln < kStripes) {
+                cnt = ld_relaxed64(&cur->stripe[ln].count);
+                sc = ld_relaxed64(&cur->stripe[ln].scout);
             }
-            s_lvl[0] = cnt;
-            s_lvl[1] = sc;
-            s_lvl[2] = __hip_atomic_load(&cur->nsmall, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            s_lvl[3] = ld_relaxed64(&cur->large);
+            cnt = wave_sum_u64(cnt);
+            sc = wave_sum_u64(sc);
+            if (ln == 0) {
+                s_lvl[0] = cnt;
+                s_lvl[1] = sc;
+                s_lvl[2] = __hip_atomic_load(&cur->qlen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                s_lvl[3] = __hip_atomic_load(&cur->overflow, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                s_lvl[4] = __hip_atomic_load(&cur->has_huge, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                s_qn = 0;
+            }
         }
         __syncthreads();
         const unsigned long long count = s_lvl[0], scout = s_lvl[1];
-        const uint32_t nsmall = (uint32_t)s_lvl[2];
-        const uint32_t ne = (uint32_t)(s_lvl[3] >> 32), nc = (uint32_t)s_lvl[3];
+        const uint32_t qlen = (uint32_t)min((unsigned long long)a.qcap, s_lvl[2]);
+        const bool overflow = s_lvl[3] != 0;
+        const bool has_huge = s_lvl[4] != 0;
         __syncthreads();
         if (count == 0) break;
-        reached += (uint32_t)count;
-        reached_edges += scout;
         if (a.max_rounds && L >= a.max_rounds) break;
 
         // Direction for this round: GetInitPolicy / GetNextPolicy with the
@@ -215,192 +364,179 @@ __global__ __launch_bounds__(BS) void k_bfs(BfsArgs a)
         } else {
             pushes++;
         }
-
-        // Slot and bitmap for round L+2 are free now: clear them.
-        uint64_t *fcur = (L % 3 == 0) ? fbm0 : (L % 3 == 1) ? fbm1 : fbm2;
-        uint64_t *fnext = (L % 3 == 0) ? fbm1 : (L % 3 == 1) ? fbm2 : fbm0;
-        uint64_t *fclr = (L % 3 == 0) ? fbm2 : (L % 3 == 1) ? fbm0 : fbm1;
+        if (gtid == 0 && L < kTraceRows) {
+            DevTrace t;
+            t.round = L;
+            t.variant = pull ? SG_ASYNC_PULL_DD : SG_ASYNC_PUSH_DD;
+            t.in_nodes = (uint32_t)count;
+            t.pad = 0;
+            t.in_edges = scout;
+            t.ticks = (unsigned long long)(wall_clock64() - t_start);
+            a.trace[L] = t;
+        }
         if (blockIdx.x == 0) clear_ctr(&a.ctr[(L + 2) % 3]);
-        for (uint32_t w = gtid; w < W; w += nthreads) fclr[w] = 0;
 
-        BfsCtr *nxt = &a.ctr[(L + 1) % 3];
-        const uint32_t *qin = (L & 1) ? a.q1 : a.q0;
-        const uint2 *lin = (L & 1) ? a.lq1 : a.lq0;
-        Emit em{(L & 1) ? a.q0 : a.q1, (L & 1) ? a.lq0 : a.lq1, nxt, a.large_deg, a.chunk};
-        const int32_t next_level = (int32_t)L + 1;
-        unsigned long long my_scout = 0, my_count = 0;
+        Round R;
+        R.a = &a;
+        R.L = (int32_t)L;
+        R.fbit_valid = pull_prev;
+        R.fcur = a.fbit + (size_t)(L & 1) * W;
+        R.fnext = a.fbit + (size_t)((L + 1) & 1) * W;
+        R.qout = a.queue + (size_t)((L + 1) % 3) * a.qcap;
+        R.nxt = &a.ctr[(L + 1) % 3];
+        R.s_q = s_q;
+        R.s_qn = &s_qn;
+        R.lds_cap = kLdsQ;
+        R.scout = 0;
+        R.count = 0;
+        R.huge = false;
+        const uint32_t *qin = a.queue + (size_t)(L % 3) * a.qcap;
 
         if (!pull) {
-            // (1) large-degree vertices: fixed-size chunks spread over all waves
-            for (uint32_t c = gwave; c < nc; c += nwaves) {
-                // 64-ary search for the last entry whose chunk base is <= c
-                // (entries are sorted by base; entry 0 has base 0).
-                uint32_t lo = 0, hi = ne;
-                while (hi - lo > 1) {
-                    const uint32_t step = (hi - lo + kWave - 1) / kWave;
-                    const uint32_t idx = lo + (uint32_t)ln * step;
-                    const uint64_t le = ballot(idx < hi && lin[idx].y <= c);
-                    const uint32_t k = (uint32_t)__popcll(le) - 1;
-                    lo = lo + k * step;
-                    hi = min(hi, lo + step);
+            // (1) high-degree frontier vertices through the static chunk list;
+            //     lane ln of wave gw takes chunk base + ln*nwaves + gw, so the
+            //     chunks of one hub land on different waves.
+            const uint32_t nhuge = has_huge ? g.nhuge : 0u;
+            for (uint32_t base = 0; base < nhuge; base += nwaves * kWave) {
+                const uint32_t c = base + (uint32_t)ln * nwaves + gwave;
+                uint2 dsc = make_uint2(0u, 0u);
+                bool mem = false;
+                if (c < nhuge) {
+                    dsc = g.huge[c];
+                    mem = in_frontier(R, dsc.x);
                 }
-                const uint2 ent = lin[lo];
-                const uint32_t u = ent.x;
-                const uint32_t beg = g.row[u] + (c - ent.y) * a.chunk;
-                const uint32_t end = min(g.row[u + 1], beg + a.chunk);
-                for (uint32_t e0 = beg; e0 < end; e0 += kWave) {
-                    const uint32_t e = e0 + ln;
-                    const bool act = e < end;
-                    const uint32_t v = act ? g.col[e] : 0u;
-                    bool found = act && claim(a.vis, v);
-                    uint32_t dv = 0;
-                    if (found) {
-                        a.level[v] = next_level;
-                        atomicOr((unsigned long long *)&fnext[v >> 6], 1ull << (v & 63));
-                        dv = g.row[v + 1] - g.row[v];
-                        my_scout += dv;
-                        my_count++;
+                uint64_t mm = ballot(mem);
+                while (mm) {
+                    const int t = first_lane(mm);
+                    mm &= mm - 1;
+                    const uint32_t u = shfl(dsc.x, t), beg = shfl(dsc.y, t);
+                    const uint32_t end = min(g.row[u + 1], beg + kHugeChunk);
+                    for (uint32_t e0 = beg; e0 < end; e0 += kWave) {
+                        const uint32_t e = e0 + ln;
+                        const bool act = e < end;
+                        push_visit(R, act, act ? g.col[e] : 0u);
                     }
-                    emit_wave(em, found, v, dv, true);
                 }
             }
-            // (2) small-degree frontier: from the queue after a push round, from
-            //     the bitmap after a pull round (large ones are in the list).
-            const bool from_bitmap = pull_prev;
-            const uint32_t nitems = from_bitmap ? W : (nsmall + kWave - 1) / kWave;
-            for (uint32_t it = gwave; it < nitems; it += nwaves) {
-                uint32_t u = 0;
-                bool valid;
-                if (from_bitmap) {
-                    const uint64_t fw = fcur[it];
-                    if (!fw) continue;  // wave-uniform
-                    u = it * 64 + ln;
-                    valid = (fw >> ln) & 1;
-                } else {
-                    const uint32_t i = it * kWave + ln;
-                    valid = i < nsmall;
-                    if (valid) u = qin[i];
-                }
-                uint32_t b = 0, d = 0;
-                if (valid) {
-                    b = g.row[u];
-                    d = g.row[u + 1] - b;
-                    if (from_bitmap && d > a.large_deg) d = 0;  // handled via the large list
-                }
-                const uint32_t incl = wave_incl_scan(d);
-                const uint32_t total = shfl(incl, kWave - 1);
-                const uint32_t off = b - (incl - d);
-                for (uint32_t jb = 0; jb < total; jb += kWave) {
-                    const uint32_t j = jb + ln;
-                    const bool act = j < total;
-                    int k = 0;
-#pragma unroll
-                    for (int step = 32; step > 0; step >>= 1) {
-                        const uint32_t t = shfl(incl, k + step - 1);
-                        if (t <= j) k += step;
+            // (2) the rest of the frontier: from the bitmap after a pull round,
+            //     from the queue after a push round, from the level array if the
+            //     queue overflowed.
+            if (pull_prev) {
+                for (uint32_t wb = wbeg; wb < wend; wb += kWave) {
+                    const uint32_t nw = min((uint32_t)kWave, wend - wb);
+                    const uint64_t fl = (uint32_t)ln < nw ? R.fcur[wb + ln] : 0ull;
+                    uint64_t todo = ballot(fl != 0);
+                    while (todo) {
+                        const int t = first_lane(todo);
+                        todo &= todo - 1;
+                        const uint64_t fw = shfl(fl, t);
+                        push_expand(R, (fw >> ln) & 1, (wb + t) * 64 + ln);
                     }
-                    const uint32_t e = shfl(off, k) + j;
-                    const uint32_t v = act ? g.col[e] : 0u;
-                    bool found = act && claim(a.vis, v);
-                    uint32_t dv = 0;
-                    if (found) {
-                        a.level[v] = next_level;
-                        atomicOr((unsigned long long *)&fnext[v >> 6], 1ull << (v & 63));
-                        dv = g.row[v + 1] - g.row[v];
-                        my_scout += dv;
-                        my_count++;
-                    }
-                    emit_wave(em, found, v, dv, true);
+                }
+            } else if (!overflow) {
+                // an even slice of the queue per wave (a short queue puts one
+                // vertex on each of many waves rather than 64 on a few)
+                const uint32_t per = (qlen + nwaves - 1) / nwaves;
+                const uint32_t qb = min(qlen, gwave * per), qe = min(qlen, qb + per);
+                for (uint32_t i0 = qb; i0 < qe; i0 += kWave) {
+                    const uint32_t i = i0 + ln;
+                    const bool valid = i < qe;
+                    push_expand(R, valid, valid ? qin[i] : 0u);
+                }
+            } else {
+                for (uint32_t w = gwave; w < W; w += nwaves) {
+                    const uint32_t v = w * 64 + ln;
+                    push_expand(R, v < n && a.level[v] == (int32_t)L, v);
                 }
             }
+            // Publish the block's staged queue with one atomic.
+            __syncthreads();
+            if (threadIdx.x == 0) {
+                const uint32_t k = min(s_qn, kLdsQ);
+                s_flush[0] = k ? atomicAdd(&R.nxt->qlen, k) : 0u;
+                s_flush[1] = k;
+            }
+            __syncthreads();
+            const uint32_t fb = s_flush[0], fk = s_flush[1];
+            for (uint32_t i = threadIdx.x; i < fk; i += BS) {
+                const uint32_t pos = fb + i;
+                if (pos < a.qcap) R.qout[pos] = s_q[i];
+            }
+            if (threadIdx.x == 0 && fb + fk > a.qcap) R.nxt->overflow = 1;
         } else {
-            // Pull: each wave owns one 64-vertex word of the visited bitmap.
-            for (uint32_t w = gwave; w < W; w += nwaves) {
-                const uint64_t vw = a.vis[w];
-                if (vw == ~0ull) continue;  // wave-uniform
-                const uint32_t v = w * 64 + ln;
-                const bool want = v < n && !((vw >> ln) & 1);
-                bool found = false;
-                uint32_t b = 0, e = 0;
-                if (want) {
-                    b = g.cstart[v];
-                    e = g.cstart[v + 1];
-                    const uint32_t lim = min(e, b + kPullSerial);
-                    for (; b < lim; b++) {
-                        const uint32_t u = g.csrc[b];
-                        if ((fcur[u >> 6] >> (u & 63)) & 1) {
-                            found = true;
-                            break;
+            // Pull: sweep this wave's words, skipping ones the filter marks full.
+            for (uint32_t wb = wbeg; wb < wend; wb += kWave) {
+                const uint32_t nw = min((uint32_t)kWave, wend - wb);
+                const uint64_t vl = (uint32_t)ln < nw ? a.vis[wb + ln] : ~0ull;
+                if ((uint32_t)ln < nw && vl == ~0ull) R.fnext[wb + ln] = 0;
+                uint64_t todo = ballot(vl != ~0ull);
+                while (todo) {
+                    constexpr int G = 4;
+                    uint32_t wg[G];
+                    uint64_t vwg[G];
+                    int ng = 0;
+#pragma unroll
+                    for (int i = 0; i < G; i++) {
+                        wg[i] = wb;
+                        vwg[i] = ~0ull;
+                        if (todo) {
+                            const int t = first_lane(todo);
+                            todo &= todo - 1;
+                            wg[i] = wb + t;
+                            vwg[i] = shfl(vl, t);
+                            ng++;
                         }
                     }
+                    pull_group<G>(R, wg, vwg, ng);
                 }
-                uint64_t rest = ballot(want && !found && b < e);
-                while (rest) {
-                    const int t = first_lane(rest);
-                    rest &= rest - 1;
-                    const uint32_t tb = shfl(b, t), te = shfl(e, t);
-                    bool hit = false;
-                    for (uint32_t x = tb; x < te; x += kWave) {
-                        const uint32_t y = x + ln;
-                        bool h = false;
-                        if (y < te) {
-                            const uint32_t u = g.csrc[y];
-                            h = (fcur[u >> 6] >> (u & 63)) & 1;
-                        }
-                        if (ballot(h)) {
-                            hit = true;
-                            break;
-                        }
-                    }
-                    if (ln == t) found = hit;
-                }
-                const uint64_t fm = ballot(found);
-                if (!fm) continue;
-                if (ln == 0) {
-                    a.vis[w] = vw | fm;
-                    fnext[w] = fm;
-                }
-                uint32_t dv = 0;
-                if (found) {
-                    a.level[v] = next_level;
-                    dv = g.row[v + 1] - g.row[v];
-                    my_scout += dv;
-                }
-                if (ln == 0) my_count += __popcll(fm);
-                emit_wave(em, found, v, dv, false);
             }
         }
 
         // Block-reduce count and scout, one atomic per block into a stripe.
-        my_scout = wave_sum_u64(my_scout);
-        my_count = wave_sum_u64(my_count);
+        const unsigned long long bsc = wave_sum_u64(R.scout);
+        const unsigned long long bcn = wave_sum_u64(R.count);
+        const bool bhg = ballot(R.huge) != 0;
         if (ln == 0) {
-            s_red[0][wid] = my_scout;
-            s_red[1][wid] = my_count;
+            s_red[0][wid] = bsc;
+            s_red[1][wid] = bcn | (bhg ? (1ull << 63) : 0ull);
         }
         __syncthreads();
         if (threadIdx.x == 0) {
             unsigned long long sc = 0, cn = 0;
+            bool hg = false;
             for (int i = 0; i < WPB; i++) {
                 sc += s_red[0][i];
-                cn += s_red[1][i];
+                cn += s_red[1][i] & ~(1ull << 63);
+                hg |= (s_red[1][i] >> 63) != 0;
             }
-            Stripe &st = nxt->stripe[blockIdx.x % kStripes];
+            BfsCtr *nc = &a.ctr[(L + 1) % 3];
+            Stripe &st = nc->stripe[blockIdx.x % kStripes];
             if (sc) atomicAdd(&st.scout, sc);
             if (cn) atomicAdd(&st.count, cn);
+            if (hg) nc->has_huge = 1;  // same value from every writer
         }
         pull_prev = pull;
         L++;
-        ok = grid_sync(a.bar, a.abort);
+        ok = grid_sync(a.bar, xcc, gen, a.abort);
     }
 
     if (gtid == 0) {
+        if (L < kTraceRows) {
+            DevTrace t;
+            t.round = L;
+            t.variant = -1;
+            t.in_nodes = 0;
+            t.pad = 0;
+            t.in_edges = 0;
+            t.ticks = (unsigned long long)(wall_clock64() - t_start);
+            a.trace[L] = t;
+        }
         DevStats s;
         s.rounds = L;
         s.push_rounds = pushes;
         s.pull_rounds = pulls;
-        s.reached = reached;
-        s.reached_edges = reached_edges;
+        s.reached = 0;        // exact reach statistics come from sg_engine_get_stats
+        s.reached_edges = 0;
         s.trace = trace;
         s.max_depth = L ? L - 1 : 0;
         s.status = ok ? SG_OK : SG_ERR_TIMEOUT;
@@ -414,24 +550,19 @@ __global__ __launch_bounds__(BS) void k_bfs(BfsArgs a)
 template <int BS>
 int launch_bfs(sg_engine *e, BfsArgs &args)
 {
-    if (e->coop_blocks == 0) {
-        int per_cu = 0;
-        SG_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_bfs<BS>, BS, 0));
-        if (per_cu < 1) SG_FAIL(SG_ERR_HIP, "sg_bfs: kernel cannot be resident");
-        int blocks = per_cu * e->g->num_cus;
-        const int max_blocks = (int)((e->bar_words_len / kBarLine - 3) * kBarGroup);
-        e->coop_blocks = std::min(blocks, max_blocks);
-    }
-    args.bar.nblocks = (uint32_t)e->coop_blocks;
-    void *params[] = {&args};
-    SG_HIP(hipLaunchCooperativeKernel((const void *)k_bfs<BS>, dim3(e->coop_blocks), dim3(BS), params, 0,
-                                      e->stream));
+    sg_engine::PGrid *pg = nullptr;
+    int rc = engine_persistent_grid(e, (const void *)k_bfs<BS>, BS, &pg);
+    if (rc) return rc;
+    args.bar.words = pg->words;
+    args.bar.nxcd = pg->nxcd;
+    SG_HIP(hipEventRecord(e->ev0, e->stream));
+    hipLaunchKernelGGL(k_bfs<BS>, dim3(pg->blocks), dim3(BS), 0, e->stream, args);
+    SG_HIP(hipGetLastError());
+    SG_HIP(hipEventRecord(e->ev1, e->stream));
     return SG_OK;
 }
 
 }  // namespace
-
-static_assert(sizeof(BfsCtr) * 3 <= kCtrBytes, "BFS counters exceed the engine's counter area");
 
 extern "C" int sg_bfs(sg_engine *e, uint32_t source)
 {
@@ -440,25 +571,20 @@ extern "C" int sg_bfs(sg_engine *e, uint32_t source)
     if (source >= d.n) SG_FAIL(SG_ERR_RANGE, "sg_bfs: source >= n");
     int rc = engine_ensure_bfs_state(e);
     if (rc) return rc;
-    if (e->needs_bar_reset && (rc = engine_reset_barrier(e))) return rc;
     const sg_engine_options &o = e->opt;
     BfsArgs a{};
     a.g = d;
     a.level = (int32_t *)e->value;
     a.vis = e->bitmaps;
-    a.fbm = e->bitmaps + d.words;
-    a.q0 = e->queues;
-    a.q1 = e->queues + d.n;
-    a.lq0 = e->large;
-    a.lq1 = e->large + e->large_cap;
+    a.fbit = e->bitmaps + d.words;
+    a.queue = e->queues;
+    a.qcap = e->qcap;
     a.ctr = (BfsCtr *)e->ctr;
-    a.bar.words = e->bar_words;
-    a.bar.spin_limit = 1u << 24;
+    a.bar.spin_limit = 1u << 22;
     a.abort = e->abort_flag;
     a.stats = e->stats;
+    a.trace = e->trace;
     a.source = source;
-    a.large_deg = o.large_degree > 0 ? (uint32_t)o.large_degree : 256u;
-    a.chunk = 256;
     if (o.alpha == 0.0 || o.force_variant == SG_ASYNC_PUSH_DD || o.force_variant == SG_SYNC_PUSH_DD)
         a.force = 0;
     else if (o.force_variant == SG_ASYNC_PULL_DD || o.force_variant == SG_SYNC_PULL_DD)
@@ -472,8 +598,7 @@ extern "C" int sg_bfs(sg_engine *e, uint32_t source)
     a.total_workload0 = (uint32_t)((double)d.m * o.edge_factor);
     a.max_rounds = (uint32_t)o.max_iteration;
     e->last_algo = 0;
-    SG_HIP(hipEventRecord(e->ev0, e->stream));
-    const int bs = o.block_size ? o.block_size : 512;
+    const int bs = o.block_size ? o.block_size : 1024;
     switch (bs) {
     case 256: rc = launch_bfs<256>(e, a); break;
     case 512: rc = launch_bfs<512>(e, a); break;
@@ -481,7 +606,6 @@ extern "C" int sg_bfs(sg_engine *e, uint32_t source)
     default: SG_FAIL(SG_ERR_UNSUPPORTED, "sg_bfs: block_size must be 256, 512 or 1024");
     }
     if (rc) return rc;
-    SG_HIP(hipEventRecord(e->ev1, e->stream));
     e->timed = true;
     return SG_OK;
 }

@@ -24,9 +24,46 @@ __global__ void k_inert_bitmap(const uint32_t *cstart, uint32_t n, uint32_t word
     if (lane() == 0) inert[w] = m;
 }
 
+// Exact reach statistics of a traversal result: vertices whose 32-bit value
+// is not the 0xFFFFFFFF sentinel (-1 level / UINT32_MAX distance) and the sum
+// of their out-degrees. out[0], out[1] must be zero on entry.
+__global__ void k_reach_stats(const uint32_t *value, const uint32_t *row, uint32_t n, unsigned long long *out)
+{
+    __shared__ unsigned long long s[2][16];
+    unsigned long long cnt = 0, deg = 0;
+    for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < n; v += gridDim.x * blockDim.x) {
+        if (value[v] != 0xFFFFFFFFu) {
+            cnt++;
+            deg += row[v + 1] - row[v];
+        }
+    }
+    cnt = wave_sum_u64(cnt);
+    deg = wave_sum_u64(deg);
+    const int wid = threadIdx.x / kWave;
+    if (lane() == 0) {
+        s[0][wid] = cnt;
+        s[1][wid] = deg;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int i = 1; i < (int)(blockDim.x / kWave); i++) {
+            cnt += s[0][i];
+            deg += s[1][i];
+        }
+        atomicAdd(&out[0], cnt);
+        atomicAdd(&out[1], deg);
+    }
+}
+
+// Census for a persistent grid: blocks that land on each XCD.
+__global__ void k_xcd_census(uint32_t *words)
+{
+    if (threadIdx.x == 0) atomicAdd(&words[bar_count(xcc_id())], 1u);
+}
+
 int dev_alloc(sg_graph *g, void **p, uint64_t bytes)
 {
-    if (g->nallocs >= 8) SG_FAIL(SG_ERR_STATE, "sg_graph_create: too many allocations");
+    if (g->nallocs >= 10) SG_FAIL(SG_ERR_STATE, "sg_graph_create: too many allocations");
     hipError_t e = hipMalloc(p, bytes ? bytes : 4);
     if (e != hipSuccess) {
         (void)hipGetLastError();
@@ -149,6 +186,26 @@ extern "C" int sg_graph_create(uint32_t n, uint64_t m, const uint32_t *row_start
             }
         }
     }
+    if (!rc) {
+        // static chunk list of high-degree vertices (host side, from the CSR)
+        std::vector<uint2> huge;
+        for (uint32_t u = 0; u < n; u++) {
+            const uint32_t deg = row_start[u + 1] - row_start[u];
+            if (deg <= kHugeDegree) continue;
+            for (uint32_t c = 0; c < deg; c += kHugeChunk) huge.push_back(make_uint2(u, row_start[u] + c));
+        }
+        void *p = nullptr;
+        rc = dev_alloc(g, &p, (uint64_t)huge.size() * sizeof(uint2));
+        if (!rc && !huge.empty()) {
+            hipError_t e = hipMemcpy(p, huge.data(), huge.size() * sizeof(uint2), hipMemcpyHostToDevice);
+            if (e != hipSuccess) {
+                sg::set_error(std::string("sg_graph_create: huge list upload: ") + hipGetErrorString(e));
+                rc = SG_ERR_HIP;
+            }
+        }
+        d.huge = (const uint2 *)p;
+        d.nhuge = (uint32_t)huge.size();
+    }
     if (rc) {
         sg_graph_destroy(g);
         return rc;
@@ -215,9 +272,10 @@ extern "C" int sg_engine_destroy(sg_engine *e)
 {
     if (!e) return SG_OK;
     if (e->stream) (void)hipStreamSynchronize(e->stream);
-    void *ptrs[] = {e->value, e->bitmaps, e->queues, e->large, e->ctr, e->bar_words, e->abort_flag, e->stats};
+    void *ptrs[] = {e->value, e->bitmaps, e->queues, e->ctr, e->abort_flag, e->stats, e->trace};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
+    for (int i = 0; i < e->ngrids; i++) (void)hipFree(e->grids[i].words);
     if (e->ev0) (void)hipEventDestroy(e->ev0);
     if (e->ev1) (void)hipEventDestroy(e->ev1);
     if (e->own_stream && e->stream) (void)hipStreamDestroy(e->stream);
@@ -244,19 +302,67 @@ extern "C" int sg_engine_set_options(sg_engine *e, const sg_engine_options *o)
     if (o->block_size != 0 && (o->block_size % 64 || o->block_size < 64 || o->block_size > 1024))
         SG_FAIL(SG_ERR_INVALID, "sg_engine_set_options: block_size must be a multiple of 64 in [64, 1024]");
     if (o->max_iteration < 0) SG_FAIL(SG_ERR_INVALID, "sg_engine_set_options: max_iteration < 0");
-    int old_bs = e->opt.block_size;
     e->opt = *o;
-    if (old_bs != o->block_size) e->coop_blocks = 0;  // re-derive the persistent grid
     return SG_OK;
 }
 
 namespace sg {
 
-int engine_reset_barrier(sg_engine *e)
+static int census(sg_engine *e, sg_engine::PGrid &pg)
 {
-    SG_HIP(hipMemsetAsync(e->bar_words, 0, (size_t)e->bar_words_len * 4, e->stream));
-    SG_HIP(hipMemsetAsync(e->abort_flag, 0, 4, e->stream));
-    e->needs_bar_reset = false;
+    SG_HIP(hipMemsetAsync(pg.words, 0, kBarWords * 4, e->stream));
+    hipLaunchKernelGGL(k_xcd_census, dim3(pg.blocks), dim3(pg.bs), 0, e->stream, pg.words);
+    SG_HIP(hipGetLastError());
+    uint32_t counts[kMaxXcd];
+    SG_HIP(hipMemcpyAsync(counts, pg.words + bar_count(0), sizeof(counts), hipMemcpyDeviceToHost, e->stream));
+    SG_HIP(hipStreamSynchronize(e->stream));
+    uint32_t nx = 0, total = 0;
+    for (int x = 0; x < kMaxXcd; x++) {
+        nx += counts[x] > 0;
+        total += counts[x];
+    }
+    if (total != (uint32_t)pg.blocks) SG_FAIL(SG_ERR_HIP, "persistent grid: XCD census incomplete");
+    pg.nxcd = nx;
+    return SG_OK;
+}
+
+int engine_persistent_grid(sg_engine *e, const void *fn, int bs, sg_engine::PGrid **out)
+{
+    if (e->needs_bar_reset) {
+        // an aborted run leaves counters mid-way: zero and re-measure every grid
+        for (int i = 0; i < e->ngrids; i++) {
+            int rc = census(e, e->grids[i]);
+            if (rc) return rc;
+        }
+        SG_HIP(hipMemsetAsync(e->abort_flag, 0, 4, e->stream));
+        e->needs_bar_reset = false;
+    }
+    for (int i = 0; i < e->ngrids; i++) {
+        if (e->grids[i].fn == fn && e->grids[i].bs == bs) {
+            *out = &e->grids[i];
+            return SG_OK;
+        }
+    }
+    if (e->ngrids == 8) SG_FAIL(SG_ERR_STATE, "persistent grid: too many kernel configurations");
+    int per_cu = 0;
+    SG_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, bs, 0));
+    if (per_cu < 1) SG_FAIL(SG_ERR_HIP, "persistent grid: kernel cannot be resident");
+    sg_engine::PGrid pg;
+    pg.fn = fn;
+    pg.bs = bs;
+    pg.blocks = per_cu * e->g->num_cus;
+    hipError_t err = hipMalloc((void **)&pg.words, kBarWords * 4);
+    if (err != hipSuccess) {
+        (void)hipGetLastError();
+        SG_FAIL(SG_ERR_NOMEM, "persistent grid: hipMalloc failed");
+    }
+    int rc = census(e, pg);
+    if (rc) {
+        (void)hipFree(pg.words);
+        return rc;
+    }
+    e->grids[e->ngrids] = pg;
+    *out = &e->grids[e->ngrids++];
     return SG_OK;
 }
 
@@ -276,18 +382,21 @@ int engine_ensure_bfs_state(sg_engine *e)
     int rc;
     e->value_bytes = (uint64_t)d.n * 4;
     if ((rc = alloc(&e->value, e->value_bytes))) return rc;
-    if ((rc = alloc((void **)&e->bitmaps, (uint64_t)d.words * 8 * 4))) return rc;
-    if ((rc = alloc((void **)&e->queues, (uint64_t)d.n * 4 * 2))) return rc;
-    // Large-degree list: at most one entry per vertex.
-    e->large_cap = d.n;
-    if ((rc = alloc((void **)&e->large, (uint64_t)e->large_cap * sizeof(uint2) * 2))) return rc;
+    if ((rc = alloc((void **)&e->bitmaps, (uint64_t)d.words * 8 * 3))) return rc;
+    // Frontier queues may hold a vertex more than once (non-atomic discovery
+    // across XCDs); past qcap a round falls back to scanning the level array.
+    e->qcap = (uint32_t)std::min<uint64_t>(2ull * d.n + 4096, 0xFFFFFFF0ull);
+    if ((rc = alloc((void **)&e->queues, (uint64_t)e->qcap * 4 * 3))) return rc;
     if ((rc = alloc((void **)&e->ctr, kCtrBytes))) return rc;
-    e->bar_words_len = kBarLine * (2 + 4096 / kBarGroup + 1);
-    if ((rc = alloc((void **)&e->bar_words, (uint64_t)e->bar_words_len * 4))) return rc;
     if ((rc = alloc((void **)&e->abort_flag, 4))) return rc;
+    SG_HIP(hipMemsetAsync(e->abort_flag, 0, 4, e->stream));
     if ((rc = alloc((void **)&e->stats, sizeof(DevStats)))) return rc;
+    if ((rc = alloc((void **)&e->trace, sizeof(DevTrace) * kTraceRows))) return rc;
     SG_HIP(hipMemsetAsync(e->stats, 0, sizeof(DevStats), e->stream));
-    e->needs_bar_reset = true;
+    e->needs_bar_reset = false;
+    int khz = 0;
+    if (hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, 0) == hipSuccess && khz > 0)
+        e->wall_khz = khz;
     return SG_OK;
 }
 
@@ -347,10 +456,50 @@ extern "C" int sg_engine_get_stats(sg_engine *e, sg_stats *out)
     out->status = s.status;
     out->iterations = s.iterations;
     out->residual = s.residual;
+    if ((e->last_algo == 0 || e->last_algo == 1) && e->value) {
+        // traversals: exact reach statistics from the result array, computed
+        // here rather than inside the timed traversal
+        unsigned long long *acc = (unsigned long long *)((char *)e->ctr + kCtrBytes - 64);
+        SG_HIP(hipMemsetAsync(acc, 0, 16, e->stream));
+        const DevGraph &d = e->g->dev;
+        const uint32_t blocks = std::min<uint32_t>((d.n + 255) / 256, 4096);
+        hipLaunchKernelGGL(k_reach_stats, dim3(blocks), dim3(256), 0, e->stream, (const uint32_t *)e->value, d.row,
+                           d.n, acc);
+        SG_HIP(hipGetLastError());
+        unsigned long long h[2];
+        SG_HIP(hipMemcpyAsync(h, acc, 16, hipMemcpyDeviceToHost, e->stream));
+        SG_HIP(hipStreamSynchronize(e->stream));
+        out->reached = (uint32_t)h[0];
+        out->reached_edges = h[1];
+    }
     if (e->timed) {
         float ms = 0.f;
         SG_HIP(hipEventElapsedTime(&ms, e->ev0, e->ev1));
         out->kernel_ms = ms;
     }
+    return SG_OK;
+}
+
+extern "C" int sg_engine_get_trace(sg_engine *e, sg_round_trace *out, int max, int *n_out)
+{
+    if (!e || !out || !n_out || max < 0) SG_FAIL(SG_ERR_INVALID, "sg_engine_get_trace: bad argument");
+    *n_out = 0;
+    if (!e->stats || !e->trace) SG_FAIL(SG_ERR_STATE, "sg_engine_get_trace: nothing has run yet");
+    SG_HIP(hipStreamSynchronize(e->stream));
+    DevStats s;
+    SG_HIP(hipMemcpy(&s, e->stats, sizeof(s), hipMemcpyDeviceToHost));
+    uint32_t rows = std::min<uint32_t>(std::min<uint32_t>(s.rounds + 1, kTraceRows), (uint32_t)max);
+    std::vector<DevTrace> t(rows ? rows : 1);
+    if (rows) SG_HIP(hipMemcpy(t.data(), e->trace, rows * sizeof(DevTrace), hipMemcpyDeviceToHost));
+    const double us_per_tick = 1000.0 / e->wall_khz;
+    for (uint32_t i = 0; i < rows; i++) {
+        out[i].round = t[i].round;
+        out[i].variant = t[i].variant;
+        out[i].in_nodes = t[i].in_nodes;
+        out[i].pad = 0;
+        out[i].in_edges = t[i].in_edges;
+        out[i].time_us = (double)t[i].ticks * us_per_tick;
+    }
+    *n_out = (int)rows;
     return SG_OK;
 }

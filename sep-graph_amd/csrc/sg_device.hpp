@@ -72,19 +72,40 @@ __device__ __forceinline__ void wave_append(bool pred, uint32_t v, uint32_t *q, 
 }
 
 // ---------------------------------------------------------------- grid barrier
-// Blocks are grouped by kBarGroup; the last arrival of a group signals the top
-// counter, the last group flips the generation word. Counters sit on separate
-// 128-byte lines. All memory effects before the barrier are released at agent
-// scope and acquired after it, which on CDNA writes back / invalidates the
-// per-XCD L2 so data produced on one XCD is visible on every other.
-constexpr int kBarGroup = 16;
-constexpr int kBarLine = 32;  // uint32 per 128-byte line
+// A grid barrier for persistent kernels whose blocks are all resident.
+//
+// Blocks are grouped by the XCD they physically run on (hardware register
+// XCC_ID; each XCD has its own L2). Data produced before the barrier reaches
+// the writer's L2 when the block's waves pass __syncthreads (vector L1 is
+// write-through); the last block to arrive on an XCD writes that L2 back once
+// (agent-scope release) and signals the top counter. The last XCD flips the
+// generation word. On the way out, each XCD's leader invalidates its L2 once
+// (agent-scope acquire) and releases the XCD's other blocks, which only
+// invalidate their CU's L1. This costs one L2 write-back and one invalidate per
+// XCD per barrier instead of one per block.
+//
+// The number of blocks each XCD receives for a given grid is measured once per
+// engine by a census kernel (dispatch is deterministic); a launch whose blocks
+// land differently cannot complete a group, runs into the bounded spin and
+// reports a timeout instead of hanging.
+constexpr int kBarLine = 32;      // uint32 per 128-byte line
+constexpr int kMaxXcd = 16;       // XCC_ID is 4 bits wide
+constexpr int kBarWords = kBarLine * (2 + 3 * kMaxXcd);
+
+// Word offsets inside the barrier area.
+constexpr int kBarGen = 0;
+constexpr int kBarTop = kBarLine;
+__host__ __device__ constexpr int bar_arrive(int x) { return kBarLine * (2 + x); }
+__host__ __device__ constexpr int bar_release(int x) { return kBarLine * (2 + kMaxXcd + x); }
+__host__ __device__ constexpr int bar_count(int x) { return kBarLine * (2 + 2 * kMaxXcd) + x; }
 
 struct GridBarrier {
-    uint32_t *words;  // [0]: generation, [kBarLine]: top counter, [kBarLine*(2+g)]: group g
-    uint32_t nblocks;
+    uint32_t *words;      // kBarWords, zero-initialised; counts at bar_count(x)
+    uint32_t nxcd;        // XCDs that hold at least one block of the grid
     uint32_t spin_limit;  // max polls before giving up (sets *abort)
 };
+
+__device__ __forceinline__ uint32_t xcc_id() { return __builtin_amdgcn_s_getreg((3 << 11) | 20) & 15; }
 
 __device__ __forceinline__ uint32_t atomic_load_relaxed(const uint32_t *p)
 {
@@ -96,43 +117,69 @@ __device__ __forceinline__ void atomic_store_relaxed(uint32_t *p, uint32_t v)
     __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// Returns false if the wait timed out (the caller must then stop its loop);
-// *abort is set so every block observes the failure.
-__device__ __forceinline__ bool grid_sync(const GridBarrier &gb, uint32_t *abort)
+__device__ __forceinline__ uint32_t atomic_add_relaxed(uint32_t *p, uint32_t v)
+{
+    return __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Bounded wait until *p != v (want_ne) or *p == v (!want_ne).
+__device__ __forceinline__ bool spin_wait(const uint32_t *p, uint32_t v, bool want_ne, uint32_t limit,
+                                          uint32_t *abort)
+{
+    uint32_t polls = 0;
+    for (;;) {
+        const uint32_t x = atomic_load_relaxed(p);
+        if (want_ne ? x != v : x == v) return true;
+        if ((++polls & 63) == 0 && atomic_load_relaxed(abort)) return false;
+        if (polls > limit) {
+            atomic_store_relaxed(abort, 1u);
+            return false;
+        }
+        __builtin_amdgcn_s_sleep(1);
+    }
+}
+
+// Generation the barrier words are at when a kernel starts (thread 0 keeps it
+// in a register and advances it per barrier, so no barrier re-reads it).
+__device__ __forceinline__ uint32_t grid_gen(const GridBarrier &gb) { return atomic_load_relaxed(gb.words + kBarGen); }
+
+// Returns false if the wait timed out or another block aborted; the caller
+// must then leave its loop. `gen` is this block's generation (grid_gen at
+// kernel start), advanced on success; only thread 0's copy is used.
+__device__ __forceinline__ bool grid_sync(const GridBarrier &gb, uint32_t xcc, uint32_t &gen, uint32_t *abort)
 {
     __shared__ uint32_t s_ok;
+    // __syncthreads() alone only waits for LDS traffic: every wave must first
+    // drain its outstanding global stores/atomics (vmcnt) so they are in the
+    // XCD's L2 before the leader writes that L2 back.
+    __builtin_amdgcn_s_waitcnt(0);
     __syncthreads();
     if (threadIdx.x == 0) {
-        uint32_t *gen = gb.words;
-        uint32_t *top = gb.words + kBarLine;
-        const uint32_t g = blockIdx.x / kBarGroup;
-        const uint32_t ngroups = (gb.nblocks + kBarGroup - 1) / kBarGroup;
-        const uint32_t gsize = min((uint32_t)kBarGroup, gb.nblocks - g * kBarGroup);
-        uint32_t *grp = gb.words + kBarLine * (2 + g);
-        const uint32_t gen0 = atomic_load_relaxed(gen);
+        uint32_t *w = gb.words;
+        const uint32_t gen0 = gen;
+        gen = gen0 + 1;
+        const uint32_t cnt = w[bar_count(xcc)];
         uint32_t ok = 1;
-        bool released = false;
-        uint32_t a = __hip_atomic_fetch_add(grp, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-        if (a == gsize - 1) {
-            atomic_store_relaxed(grp, 0u);
-            uint32_t t = __hip_atomic_fetch_add(top, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-            if (t == ngroups - 1) {
-                atomic_store_relaxed(top, 0u);
-                __hip_atomic_store(gen, gen0 + 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-                released = true;
+        const uint32_t a = atomic_add_relaxed(w + bar_arrive(xcc), 1u);
+        if (a == cnt - 1) {
+            atomic_store_relaxed(w + bar_arrive(xcc), 0u);
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+            const uint32_t t = atomic_add_relaxed(w + kBarTop, 1u);
+            if (t == gb.nxcd - 1) {
+                // the reset must complete before the generation flips, or a
+                // leader's next-barrier arrival could be overwritten by it
+                atomic_store_relaxed(w + kBarTop, 0u);
+                __builtin_amdgcn_s_waitcnt(0);
+                atomic_store_relaxed(w + kBarGen, gen0 + 1);
             }
-        }
-        if (!released) {
-            uint32_t polls = 0;
-            while (atomic_load_relaxed(gen) == gen0) {
-                if (++polls > gb.spin_limit || atomic_load_relaxed(abort)) {
-                    atomic_store_relaxed(abort, 1u);
-                    ok = 0;
-                    break;
-                }
-                __builtin_amdgcn_s_sleep(2);
-            }
+            ok = spin_wait(w + kBarGen, gen0, true, gb.spin_limit, abort);
             __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+            __builtin_amdgcn_s_waitcnt(0);
+            atomic_store_relaxed(w + bar_release(xcc), gen0 + 1);
+        } else {
+            ok = spin_wait(w + bar_release(xcc), gen0 + 1, false, gb.spin_limit, abort);
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+            __builtin_amdgcn_s_waitcnt(0);
         }
         s_ok = ok;
     }

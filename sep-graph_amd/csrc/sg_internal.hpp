@@ -40,7 +40,17 @@ struct DevGraph {
     const uint32_t *csrc;     // CSC edge_src [m]
     const uint32_t *cw;       // CSC weights [m] or null
     const uint64_t *inert;    // bitmap: in-degree 0 (never reached unless source)
+    // Static split of high-degree vertices: every vertex with out-degree >
+    // kHugeDegree contributes ceil(deg / kHugeChunk) chunk descriptors
+    // (vertex, first edge), so a push round can spread its edges over the grid.
+    const uint2 *huge;
+    uint32_t nhuge;           // number of chunk descriptors
 };
+
+// A vertex whose out-degree exceeds this is expanded through the static chunk
+// list instead of by a single wave.
+constexpr uint32_t kHugeDegree = 1024;
+constexpr uint32_t kHugeChunk = 256;
 
 }  // namespace sg
 
@@ -49,7 +59,7 @@ struct sg_graph {
     bool undirected = false;
     bool weighted = false;
     uint64_t device_bytes = 0;
-    void *allocs[8] = {};
+    void *allocs[10] = {};
     int nallocs = 0;
     int device = 0;
     int num_cus = 0;
@@ -73,6 +83,18 @@ struct DevStats {
     double residual;
 };
 
+// Rows of the per-round device trace kept per engine.
+constexpr uint32_t kTraceRows = 4096;
+
+struct DevTrace {
+    uint32_t round;
+    int32_t variant;
+    uint32_t in_nodes;
+    uint32_t pad;
+    unsigned long long in_edges;
+    unsigned long long ticks;  // wall_clock64() - kernel start
+};
+
 }  // namespace sg
 
 struct sg_engine {
@@ -85,19 +107,27 @@ struct sg_engine {
     // device allocations
     void *value = nullptr;        // per-vertex result (levels / distances / ranks)
     uint64_t value_bytes = 0;
-    uint64_t *bitmaps = nullptr;  // visited + 3 frontier bitmaps (4 * words)
-    uint32_t *queues = nullptr;   // 2 small queues [n] each
-    uint2 *large = nullptr;       // 2 large lists
-    uint32_t large_cap = 0;
+    uint64_t *bitmaps = nullptr;  // visited + 2 frontier bitmaps (3 * words)
+    uint32_t *queues = nullptr;   // 3 frontier queues of qcap entries
+    uint32_t qcap = 0;
     void *ctr = nullptr;          // kCtrBytes of per-round counters
-    uint32_t *bar_words = nullptr;
     uint32_t *abort_flag = nullptr;
     sg::DevStats *stats = nullptr;
-    uint32_t bar_words_len = 0;
-
-    int coop_blocks = 0;          // persistent grid size for the BFS kernel
-    int block_size = 256;
-    bool needs_bar_reset = true;
+    sg::DevTrace *trace = nullptr;    // kTraceRows rows
+    uint32_t trace_rows = 0;          // rows written by the last run (host copy)
+    double wall_khz = 100000.0;       // device wall clock rate
+    // Persistent-grid configurations (one per kernel and block size): grid
+    // size from occupancy, blocks per XCD from a census, barrier words.
+    struct PGrid {
+        const void *fn = nullptr;
+        int bs = 0;
+        int blocks = 0;
+        uint32_t nxcd = 0;
+        uint32_t *words = nullptr;
+    };
+    PGrid grids[8];
+    int ngrids = 0;
+    bool needs_bar_reset = false;
 
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     bool timed = false;
@@ -105,5 +135,7 @@ struct sg_engine {
 
 namespace sg {
 int engine_ensure_bfs_state(sg_engine *e);
-int engine_reset_barrier(sg_engine *e);
+// Grid for a persistent kernel: blocks = resident blocks per CU x CUs, with a
+// barrier calibrated for it. Barriers are re-zeroed after an aborted run.
+int engine_persistent_grid(sg_engine *e, const void *fn, int bs, sg_engine::PGrid **out);
 }  // namespace sg

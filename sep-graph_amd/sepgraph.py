@@ -93,6 +93,17 @@ class Stats(C.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_ if k != "reserved"}
 
 
+class RoundTrace(C.Structure):
+    _fields_ = [
+        ("round", C.c_uint32),
+        ("variant", C.c_int32),
+        ("in_nodes", C.c_uint32),
+        ("pad", C.c_uint32),
+        ("in_edges", C.c_uint64),
+        ("time_us", C.c_double),
+    ]
+
+
 _P = C.c_void_p
 _U32P = C.POINTER(C.c_uint32)
 _lib = None
@@ -128,6 +139,7 @@ def lib() -> C.CDLL:
             "sg_engine_value_ptr": ([_P, C.POINTER(_P), C.POINTER(C.c_uint64)], C.c_int),
             "sg_engine_gather_value": ([_P, _P, C.c_uint64], C.c_int),
             "sg_engine_get_stats": ([_P, C.POINTER(Stats)], C.c_int),
+            "sg_engine_get_trace": ([_P, C.POINTER(RoundTrace), C.c_int, C.POINTER(C.c_int)], C.c_int),
         }
         for name, (args, res) in sig.items():
             f = getattr(L, name)
@@ -328,6 +340,18 @@ class Engine:
         s = Stats()
         _check(lib().sg_engine_get_stats(self.h, C.byref(s)))
         return s.as_dict()
+
+    def trace(self, max_rows: int = 4096) -> list[dict]:
+        """Per-round rows of the last traversal: round, variant, in_nodes, in_edges, time_us."""
+        rows = (RoundTrace * max_rows)()
+        nrow = C.c_int()
+        _check(lib().sg_engine_get_trace(self.h, rows, max_rows, C.byref(nrow)))
+        inv = {v: k for k, v in VARIANTS.items()}
+        return [
+            {"round": r.round, "variant": inv.get(r.variant, "end"), "in_nodes": r.in_nodes,
+             "in_edges": r.in_edges, "time_us": r.time_us}
+            for r in rows[: nrow.value]
+        ]
 
     def close(self):
         if self.h:
