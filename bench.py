@@ -30,6 +30,21 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
+def measured_traffic(name):
+    """HBM bytes per launch from a committed rocprofv3 counter summary
+    (profiles/<name>.txt, a 'traffic_bytes_per_launch = X' line), or None."""
+    path = os.path.join(ROOT, "profiles", name + ".txt")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.strip().startswith("traffic_bytes_per_launch"):
+            try:
+                return float(line.split("=", 1)[1])
+            except ValueError:
+                return None
+    return None
+
+
 def cpu_baseline(hg, sources, gpu_levels, budget_s):
     """Oracle BFS (single thread) over as many sources as fit in budget_s."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
@@ -68,6 +83,8 @@ def main():
     ap.add_argument("--sources", type=int, default=64)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-mb", type=int, default=0,
+                    help="only run a plain device copy of this many MiB (counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
@@ -80,6 +97,17 @@ def main():
     if not torch.cuda.is_available() or sg.lib().sg_device_count() < 1:
         print("bench.py needs a visible MI355X", file=sys.stderr)
         sys.exit(1)
+    if args.copy_mb:
+        # A copy of known size: its FETCH_SIZE / WRITE_SIZE readings calibrate
+        # the counters used for roofline.traffic.
+        nbytes = args.copy_mb << 20
+        x = torch.empty(nbytes, dtype=torch.uint8, device="cuda")
+        y = torch.empty_like(x)
+        for _ in range(4):
+            y.copy_(x)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes": nbytes, "launches": 4}))
+        return
 
     t0 = time.time()
     hg = sg.rmat_graph(args.scale, 16, symmetrize=False, seed=1, weights=(1, 64), weight_seed=2)
@@ -168,7 +196,7 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": None,
+            "traffic": measured_traffic(f"k_bfs_rmat{args.scale}_traffic"),
             "mean_launch_us": round(float(durs_ms.mean()) * 1e3, 2),
         },
         "cpu_baseline": cpu,

@@ -194,6 +194,19 @@ int sg_engine_set_options(sg_engine *e, const sg_engine_options *opt);
  * (include/framework/hybrid_policy.h:237-291). Asynchronous. */
 int sg_bfs(sg_engine *e, uint32_t source);
 
+/* Single-source shortest paths from `source` (hybrid_sssp,
+ * samples/hybrid_sssp/hybrid_sssp.cu:151-242): uint32 distances, UINT32_MAX
+ * for unreached, relaxation d(u)+w(u,v) with atomicMin as the reference's
+ * AccumulateBuffer (hybrid_sssp.cu:98-118). A graph without weights uses
+ * weight 1 on every edge (the reference's all-ones fallback,
+ * include/groute/graphs/csr_graph.h:1840-1846). Direction per round follows
+ * the same alpha/beta rule as BFS. With options.priority == SG_PRIO_LOW_HIGH
+ * the two-level (near/far) schedule of SetTwoLevelBasedPriority is used, with
+ * delta = options.priority_threshold, or the reference's heuristic
+ * 32 * mean weight / mean degree (hybrid_sssp.cu:183-199) when that is 0.
+ * Asynchronous. */
+int sg_sssp(sg_engine *e, uint32_t source);
+
 /* Synchronise the engine's stream and report the device-side status. */
 int sg_engine_sync(sg_engine *e);
 /* Device pointer and size in bytes of the last run's per-vertex result. */
@@ -205,6 +218,12 @@ int sg_engine_get_stats(sg_engine *e, sg_stats *out);
 /* Per-round trace of the last traversal (synchronises): up to `max` rows,
  * the last one (variant -1) marking the end; *n_out gets the rows written. */
 int sg_engine_get_trace(sg_engine *e, sg_round_trace *out, int max, int *n_out);
+
+/* Diagnostics: grid configuration and barrier words of the engine's
+ * persistent kernel configuration `index` (0-based). info[0] = blocks,
+ * info[1] = block size, info[2] = XCDs used, info[3..18] = blocks per XCD;
+ * words gets up to `nwords` barrier words. Synchronises. */
+int sg_engine_debug_grid(sg_engine *e, int index, uint32_t *info, uint32_t *words, int nwords);
 
 #ifdef __cplusplus
 }

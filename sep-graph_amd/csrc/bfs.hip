@@ -23,36 +23,18 @@
 //   * vertices with out-degree > kHugeDegree are expanded through the graph's
 //     static chunk list, filtered by membership, so hub adjacencies spread
 //     over every wave without building any list at run time.
-// A pull round sweeps 64-vertex words (one wave per word at a time, each wave
-// owning a contiguous range); every unvisited lane first tests its first
-// in-edge, then the wave walks the concatenated remaining in-edge lists 64
-// edges per step, re-planning after each step so a vertex stops at its first
-// parent (the reference's early break, pull_functor.h:138-188). Pull rounds
-// write whole words of the next frontier bitmap and of a visited filter, so
-// they need no atomics either.
-#include "sg_internal.hpp"
+// A pull round sweeps 64-vertex words (each wave owning a contiguous range);
+// every unvisited lane first tests its first in-edge (four words' tests in
+// flight at once), then the wave walks the concatenated remaining in-edge
+// lists 64 edges per step, re-planning after each step so a vertex stops at
+// its first parent (the reference's early break, pull_functor.h:138-188).
+// Pull rounds write whole words of the next frontier bitmap and of a visited
+// filter, so they need no atomics either.
+#include "frontier.hpp"
 
 using namespace sg;
 
 namespace {
-
-constexpr int kStripes = 16;
-
-struct alignas(128) Stripe {
-    unsigned long long scout;
-    unsigned long long count;
-    uint32_t pad[28];
-};
-
-struct alignas(128) BfsCtr {
-    uint32_t qlen;      // entries appended to this round's queue
-    uint32_t overflow;  // set when appends ran past the queue capacity
-    uint32_t has_huge;  // the frontier holds a vertex above kHugeDegree
-    uint32_t pad[29];
-    Stripe stripe[kStripes];
-};
-static_assert(sizeof(BfsCtr) == 128 * (1 + kStripes), "BfsCtr layout");
-static_assert(sizeof(BfsCtr) * 3 <= kCtrBytes, "BFS counters exceed the engine's counter area");
 
 struct BfsArgs {
     DevGraph g;
@@ -61,7 +43,7 @@ struct BfsArgs {
     uint64_t *fbit;   // 2 * W words: frontier bitmaps written by pull rounds
     uint32_t *queue;  // 3 * qcap entries
     uint32_t qcap;
-    BfsCtr *ctr;      // 3 slots
+    FrontierCtr *ctr; // 3 slots
     GridBarrier bar;
     uint32_t *abort;
     DevStats *stats;
@@ -71,34 +53,8 @@ struct BfsArgs {
     double alpha, beta;
     uint32_t total_workload0;
     uint32_t max_rounds;
+    uint32_t probe;   // launch is a residency probe of the grid
 };
-
-__device__ __forceinline__ unsigned long long ld_relaxed64(const unsigned long long *p)
-{
-    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-
-// Zero one counter slot (called by one whole block).
-__device__ void clear_ctr(BfsCtr *c)
-{
-    uint32_t *w = (uint32_t *)c;
-    for (uint32_t i = threadIdx.x; i < sizeof(BfsCtr) / 4; i += blockDim.x) w[i] = 0;
-}
-
-__device__ __forceinline__ bool test_bit(const uint64_t *bm, uint32_t u) { return (bm[u >> 6] >> (u & 63)) & 1; }
-
-// Index of the lane owning position j of a wave-wide concatenation whose
-// inclusive prefix sums are `incl` (the number of lanes with incl <= j).
-__device__ __forceinline__ int owner_lane(uint32_t incl, uint32_t j)
-{
-    int k = 0;
-#pragma unroll
-    for (int step = 32; step > 0; step >>= 1) {
-        const uint32_t t = shfl(incl, k + step - 1);
-        if (t <= j) k += step;
-    }
-    return k;
-}
 
 struct Round {
     const BfsArgs *a;
@@ -106,11 +62,7 @@ struct Round {
     bool fbit_valid;        // frontier L is given by fcur (it came from a pull round)
     const uint64_t *fcur;
     uint64_t *fnext;
-    uint32_t *qout;
-    BfsCtr *nxt;
-    uint32_t *s_q;          // LDS staging of the next queue
-    uint32_t *s_qn;
-    uint32_t lds_cap;
+    Stager S;
     unsigned long long scout, count;  // this lane's share of the next frontier
     bool huge;                        // this lane discovered a vertex above kHugeDegree
 };
@@ -118,34 +70,6 @@ struct Round {
 __device__ __forceinline__ bool in_frontier(const Round &R, uint32_t u)
 {
     return R.fbit_valid ? test_bit(R.fcur, u) : R.a->level[u] == R.L;
-}
-
-// Stage this wave's discoveries for the next queue: into the block's LDS
-// buffer while it has room, the remainder straight to the global queue.
-__device__ __forceinline__ void stage(Round &R, bool found, uint32_t v)
-{
-    const uint64_t m = ballot(found);
-    if (!m) return;
-    const int leader = first_lane(m);
-    uint32_t base = 0;
-    if (lane() == leader) base = atomicAdd(R.s_qn, (uint32_t)__popcll(m));
-    base = shfl(base, leader);
-    const uint32_t slot = base + rank_in(m);
-    const bool to_lds = found && slot < R.lds_cap;
-    if (to_lds) R.s_q[slot] = v;
-    const bool to_glb = found && !to_lds;
-    const uint64_t gm = ballot(to_glb);
-    if (gm) {
-        const int l2 = first_lane(gm);
-        uint32_t gb = 0;
-        if (lane() == l2) gb = atomicAdd(&R.nxt->qlen, (uint32_t)__popcll(gm));
-        gb = shfl(gb, l2);
-        const uint32_t pos = gb + rank_in(gm);
-        if (to_glb) {
-            if (pos < R.a->qcap) R.qout[pos] = v;
-            else R.nxt->overflow = 1;
-        }
-    }
 }
 
 // Push edge -> v for `act` lanes; wave-convergent.
@@ -161,7 +85,7 @@ __device__ __forceinline__ void push_visit(Round &R, bool act, uint32_t v)
         R.huge |= dv > kHugeDegree;
         found = true;
     }
-    stage(R, found, v);
+    stage(R.S, found, v);
 }
 
 // Expand the out-edges of up to 64 vertices (one per lane) with wave-level
@@ -265,14 +189,17 @@ __device__ __forceinline__ void pull_group(Round &R, const uint32_t (&wg)[G], co
 template <int BS>
 __global__ __launch_bounds__(BS) void k_bfs(BfsArgs a)
 {
+    if (a.probe) {
+        residency_probe(a.bar.words, 1u << 20);
+        return;
+    }
     constexpr int WPB = BS / kWave;
     constexpr uint32_t kLdsQ = 8u * BS;
     const DevGraph &g = a.g;
     const uint32_t n = g.n, W = g.words;
     const uint32_t gtid = blockIdx.x * BS + threadIdx.x;
     const uint32_t nthreads = gridDim.x * BS;
-    const uint32_t wid = threadIdx.x / kWave;
-    const uint32_t gwave = blockIdx.x * WPB + wid;
+    const uint32_t gwave = blockIdx.x * WPB + threadIdx.x / kWave;
     const uint32_t nwaves = gridDim.x * WPB;
     const int ln = lane();
     const uint32_t xcc = xcc_id();
@@ -298,7 +225,7 @@ __global__ __launch_bounds__(BS) void k_bfs(BfsArgs a)
         clear_ctr(&a.ctr[0]);
         clear_ctr(&a.ctr[1]);
         clear_ctr(&a.ctr[2]);
-        __syncthreads();
+        block_sync_global();
         if (threadIdx.x == 0) {
             const uint32_t deg = g.row[src + 1] - g.row[src];
             a.queue[0] = src;
@@ -311,82 +238,36 @@ __global__ __launch_bounds__(BS) void k_bfs(BfsArgs a)
     bool ok = grid_sync(a.bar, xcc, gen, a.abort);
 
     // ---- replicated policy state (identical in every block)
-    bool pull_prev = false;
-    uint32_t total_wl = a.total_workload0;
-    unsigned long long trace = 0;
+    DirectionPolicy pol{a.alpha, a.beta, a.total_workload0, n, a.force};
+    unsigned long long ptrace = 0;
     uint32_t pushes = 0, pulls = 0;
     uint32_t L = 0;
     while (ok) {
-        BfsCtr *cur = &a.ctr[L % 3];
-        if (threadIdx.x < kWave) {
-            unsigned long long cnt = 0, sc = 0;
-            if (ln < kStripes) {
-                cnt = ld_relaxed64(&cur->stripe[ln].count);
-                sc = ld_relaxed64(&cur->stripe[ln].scout);
-            }
-            cnt = wave_sum_u64(cnt);
-            sc = wave_sum_u64(sc);
-            if (ln == 0) {
-                s_lvl[0] = cnt;
-                s_lvl[1] = sc;
-                s_lvl[2] = __hip_atomic_load(&cur->qlen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                s_lvl[3] = __hip_atomic_load(&cur->overflow, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                s_lvl[4] = __hip_atomic_load(&cur->has_huge, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                s_qn = 0;
-            }
-        }
+        read_counts(&a.ctr[L % 3], s_lvl);
+        if (threadIdx.x == 0) s_qn = 0;
         __syncthreads();
-        const unsigned long long count = s_lvl[0], scout = s_lvl[1];
-        const uint32_t qlen = (uint32_t)min((unsigned long long)a.qcap, s_lvl[2]);
-        const bool overflow = s_lvl[3] != 0;
-        const bool has_huge = s_lvl[4] != 0;
+        const RoundCounts rc = unpack_counts(s_lvl, a.qcap);
         __syncthreads();
-        if (count == 0) break;
+        if (rc.count == 0) break;
         if (a.max_rounds && L >= a.max_rounds) break;
 
-        // Direction for this round: GetInitPolicy / GetNextPolicy with the
-        // reference's uint32 workload counters.
-        bool pull;
-        if (a.force >= 0) {
-            pull = a.force == 1;
-        } else if (L == 0) {
-            pull = false;
-        } else if (pull_prev && (double)count > (double)n / a.beta) {
-            pull = true;
-        } else {
-            const uint32_t out_wl = pull_prev ? 1u : (uint32_t)scout;
-            pull = (double)out_wl > (double)total_wl / a.alpha;
-        }
-        if (!pull) total_wl -= (uint32_t)scout;
+        const bool pull = pol.next(L, rc.count, rc.scout);
         if (pull) {
             pulls++;
-            if (L < 64) trace |= 1ull << L;
+            if (L < 64) ptrace |= 1ull << L;
         } else {
             pushes++;
         }
-        if (gtid == 0 && L < kTraceRows) {
-            DevTrace t;
-            t.round = L;
-            t.variant = pull ? SG_ASYNC_PULL_DD : SG_ASYNC_PUSH_DD;
-            t.in_nodes = (uint32_t)count;
-            t.pad = 0;
-            t.in_edges = scout;
-            t.ticks = (unsigned long long)(wall_clock64() - t_start);
-            a.trace[L] = t;
-        }
+        if (gtid == 0) write_trace(a.trace, L, pull, rc.count, rc.scout, t_start);
         if (blockIdx.x == 0) clear_ctr(&a.ctr[(L + 2) % 3]);
 
         Round R;
         R.a = &a;
         R.L = (int32_t)L;
-        R.fbit_valid = pull_prev;
+        R.fbit_valid = pol.pull_prev;
         R.fcur = a.fbit + (size_t)(L & 1) * W;
         R.fnext = a.fbit + (size_t)((L + 1) & 1) * W;
-        R.qout = a.queue + (size_t)((L + 1) % 3) * a.qcap;
-        R.nxt = &a.ctr[(L + 1) % 3];
-        R.s_q = s_q;
-        R.s_qn = &s_qn;
-        R.lds_cap = kLdsQ;
+        R.S = Stager{a.queue + (size_t)((L + 1) % 3) * a.qcap, a.qcap, &a.ctr[(L + 1) % 3], s_q, &s_qn, kLdsQ};
         R.scout = 0;
         R.count = 0;
         R.huge = false;
@@ -396,7 +277,7 @@ __global__ __launch_bounds__(BS) void k_bfs(BfsArgs a)
             // (1) high-degree frontier vertices through the static chunk list;
             //     lane ln of wave gw takes chunk base + ln*nwaves + gw, so the
             //     chunks of one hub land on different waves.
-            const uint32_t nhuge = has_huge ? g.nhuge : 0u;
+            const uint32_t nhuge = rc.has_huge ? g.nhuge : 0u;
             for (uint32_t base = 0; base < nhuge; base += nwaves * kWave) {
                 const uint32_t c = base + (uint32_t)ln * nwaves + gwave;
                 uint2 dsc = make_uint2(0u, 0u);
@@ -421,7 +302,7 @@ __global__ __launch_bounds__(BS) void k_bfs(BfsArgs a)
             // (2) the rest of the frontier: from the bitmap after a pull round,
             //     from the queue after a push round, from the level array if the
             //     queue overflowed.
-            if (pull_prev) {
+            if (pol.pull_prev) {
                 for (uint32_t wb = wbeg; wb < wend; wb += kWave) {
                     const uint32_t nw = min((uint32_t)kWave, wend - wb);
                     const uint64_t fl = (uint32_t)ln < nw ? R.fcur[wb + ln] : 0ull;
@@ -433,11 +314,11 @@ __global__ __launch_bounds__(BS) void k_bfs(BfsArgs a)
                         push_expand(R, (fw >> ln) & 1, (wb + t) * 64 + ln);
                     }
                 }
-            } else if (!overflow) {
+            } else if (!rc.overflow) {
                 // an even slice of the queue per wave (a short queue puts one
                 // vertex on each of many waves rather than 64 on a few)
-                const uint32_t per = (qlen + nwaves - 1) / nwaves;
-                const uint32_t qb = min(qlen, gwave * per), qe = min(qlen, qb + per);
+                const uint32_t per = (rc.qlen + nwaves - 1) / nwaves;
+                const uint32_t qb = min(rc.qlen, gwave * per), qe = min(rc.qlen, qb + per);
                 for (uint32_t i0 = qb; i0 < qe; i0 += kWave) {
                     const uint32_t i = i0 + ln;
                     const bool valid = i < qe;
@@ -449,20 +330,7 @@ __global__ __launch_bounds__(BS) void k_bfs(BfsArgs a)
                     push_expand(R, v < n && a.level[v] == (int32_t)L, v);
                 }
             }
-            // Publish the block's staged queue with one atomic.
-            __syncthreads();
-            if (threadIdx.x == 0) {
-                const uint32_t k = min(s_qn, kLdsQ);
-                s_flush[0] = k ? atomicAdd(&R.nxt->qlen, k) : 0u;
-                s_flush[1] = k;
-            }
-            __syncthreads();
-            const uint32_t fb = s_flush[0], fk = s_flush[1];
-            for (uint32_t i = threadIdx.x; i < fk; i += BS) {
-                const uint32_t pos = fb + i;
-                if (pos < a.qcap) R.qout[pos] = s_q[i];
-            }
-            if (threadIdx.x == 0 && fb + fk > a.qcap) R.nxt->overflow = 1;
+            flush_stage(R.S, s_flush);
         } else {
             // Pull: sweep this wave's words, skipping ones the filter marks full.
             for (uint32_t wb = wbeg; wb < wend; wb += kWave) {
@@ -492,69 +360,32 @@ __global__ __launch_bounds__(BS) void k_bfs(BfsArgs a)
             }
         }
 
-        // Block-reduce count and scout, one atomic per block into a stripe.
-        const unsigned long long bsc = wave_sum_u64(R.scout);
-        const unsigned long long bcn = wave_sum_u64(R.count);
-        const bool bhg = ballot(R.huge) != 0;
-        if (ln == 0) {
-            s_red[0][wid] = bsc;
-            s_red[1][wid] = bcn | (bhg ? (1ull << 63) : 0ull);
-        }
-        __syncthreads();
-        if (threadIdx.x == 0) {
-            unsigned long long sc = 0, cn = 0;
-            bool hg = false;
-            for (int i = 0; i < WPB; i++) {
-                sc += s_red[0][i];
-                cn += s_red[1][i] & ~(1ull << 63);
-                hg |= (s_red[1][i] >> 63) != 0;
-            }
-            BfsCtr *nc = &a.ctr[(L + 1) % 3];
-            Stripe &st = nc->stripe[blockIdx.x % kStripes];
-            if (sc) atomicAdd(&st.scout, sc);
-            if (cn) atomicAdd(&st.count, cn);
-            if (hg) nc->has_huge = 1;  // same value from every writer
-        }
-        pull_prev = pull;
+        publish_counts<WPB>(R.S.nxt, R.scout, R.count, R.huge, s_red);
+        pol.pull_prev = pull;
         L++;
         ok = grid_sync(a.bar, xcc, gen, a.abort);
     }
 
-    if (gtid == 0) {
-        if (L < kTraceRows) {
-            DevTrace t;
-            t.round = L;
-            t.variant = -1;
-            t.in_nodes = 0;
-            t.pad = 0;
-            t.in_edges = 0;
-            t.ticks = (unsigned long long)(wall_clock64() - t_start);
-            a.trace[L] = t;
-        }
-        DevStats s;
-        s.rounds = L;
-        s.push_rounds = pushes;
-        s.pull_rounds = pulls;
-        s.reached = 0;        // exact reach statistics come from sg_engine_get_stats
-        s.reached_edges = 0;
-        s.trace = trace;
-        s.max_depth = L ? L - 1 : 0;
-        s.status = ok ? SG_OK : SG_ERR_TIMEOUT;
-        s.iterations = L;
-        s.pad = 0;
-        s.residual = 0.0;
-        *a.stats = s;
-    }
+    if (gtid == 0) write_final(a.trace, a.stats, L, pushes, pulls, ptrace, ok, t_start);
 }
 
 template <int BS>
 int launch_bfs(sg_engine *e, BfsArgs &args)
 {
     sg_engine::PGrid *pg = nullptr;
-    int rc = engine_persistent_grid(e, (const void *)k_bfs<BS>, BS, &pg);
+    auto probe = [e, args](int blocks, uint32_t *words) -> int {
+        BfsArgs p = args;
+        p.probe = 1;
+        p.bar.words = words;
+        hipLaunchKernelGGL(k_bfs<BS>, dim3(blocks), dim3(BS), 0, e->stream, p);
+        SG_HIP(hipGetLastError());
+        return SG_OK;
+    };
+    int rc = engine_persistent_grid(e, (const void *)k_bfs<BS>, BS, probe, &pg);
     if (rc) return rc;
     args.bar.words = pg->words;
     args.bar.nxcd = pg->nxcd;
+    for (int x = 0; x < kMaxXcd; x++) args.bar.cnt[x] = pg->cnt[x];
     SG_HIP(hipEventRecord(e->ev0, e->stream));
     hipLaunchKernelGGL(k_bfs<BS>, dim3(pg->blocks), dim3(BS), 0, e->stream, args);
     SG_HIP(hipGetLastError());
@@ -569,9 +400,11 @@ extern "C" int sg_bfs(sg_engine *e, uint32_t source)
     if (!e) SG_FAIL(SG_ERR_INVALID, "sg_bfs: null engine");
     const DevGraph &d = e->g->dev;
     if (source >= d.n) SG_FAIL(SG_ERR_RANGE, "sg_bfs: source >= n");
+    const sg_engine_options &o = e->opt;
+    const int force = forced_direction(o);
+    if (force < -1) SG_FAIL(SG_ERR_UNSUPPORTED, "sg_bfs: only data-driven push/pull variants apply to BFS");
     int rc = engine_ensure_bfs_state(e);
     if (rc) return rc;
-    const sg_engine_options &o = e->opt;
     BfsArgs a{};
     a.g = d;
     a.level = (int32_t *)e->value;
@@ -579,20 +412,13 @@ extern "C" int sg_bfs(sg_engine *e, uint32_t source)
     a.fbit = e->bitmaps + d.words;
     a.queue = e->queues;
     a.qcap = e->qcap;
-    a.ctr = (BfsCtr *)e->ctr;
+    a.ctr = (FrontierCtr *)e->ctr;
     a.bar.spin_limit = 1u << 22;
     a.abort = e->abort_flag;
     a.stats = e->stats;
     a.trace = e->trace;
     a.source = source;
-    if (o.alpha == 0.0 || o.force_variant == SG_ASYNC_PUSH_DD || o.force_variant == SG_SYNC_PUSH_DD)
-        a.force = 0;
-    else if (o.force_variant == SG_ASYNC_PULL_DD || o.force_variant == SG_SYNC_PULL_DD)
-        a.force = 1;
-    else if (o.force_variant == SG_VARIANT_AUTO)
-        a.force = -1;
-    else
-        SG_FAIL(SG_ERR_UNSUPPORTED, "sg_bfs: only data-driven push/pull variants apply to BFS");
+    a.force = force;
     a.alpha = o.alpha == 0.0 ? 1.0 : o.alpha;
     a.beta = o.beta;
     a.total_workload0 = (uint32_t)((double)d.m * o.edge_factor);

@@ -1,0 +1,276 @@
+// Frontier machinery shared by the persistent traversal kernels (BFS, SSSP,
+// BC forward): per-round counter slots, LDS-staged queue appends and the
+// wave-level owner search used by prefix-sum edge scheduling.
+//
+// These replace the reference's device worklist (groute::dev::Queue with one
+// atomicAdd per appended item, include/groute/device/queue.cuh:84-88) and its
+// per-round count/out-degree read-backs (framework.cuh:451-571, 700-796).
+#pragma once
+
+#include "sg_internal.hpp"
+
+namespace sg {
+
+constexpr int kStripes = 16;
+
+struct alignas(128) Stripe {
+    unsigned long long scout;  // sum of out-degree over appended vertices
+    unsigned long long count;  // appended vertices
+    uint32_t pad[28];
+};
+
+// Counters describing one round's frontier; three slots rotate.
+struct alignas(128) FrontierCtr {
+    uint32_t qlen;      // entries appended to this round's queue
+    uint32_t overflow;  // set when appends ran past the queue capacity
+    uint32_t has_huge;  // the frontier holds a vertex above kHugeDegree
+    uint32_t pad[29];
+    Stripe stripe[kStripes];
+};
+static_assert(sizeof(FrontierCtr) == 128 * (1 + kStripes), "FrontierCtr layout");
+static_assert(sizeof(FrontierCtr) * 3 <= kCtrBytes, "frontier counters exceed the engine's counter area");
+
+__device__ __forceinline__ unsigned long long ld_relaxed64(const unsigned long long *p)
+{
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Zero one counter slot (called by one whole block).
+__device__ inline void clear_ctr(FrontierCtr *c)
+{
+    uint32_t *w = (uint32_t *)c;
+    for (uint32_t i = threadIdx.x; i < sizeof(FrontierCtr) / 4; i += blockDim.x) w[i] = 0;
+}
+
+// Block barrier that also orders global memory between the block's waves:
+// __syncthreads() alone waits only for LDS traffic, so a global store issued
+// by one wave could otherwise land after another wave's later store to the
+// same word.
+__device__ __forceinline__ void block_sync_global()
+{
+    __builtin_amdgcn_s_waitcnt(0);
+    __syncthreads();
+}
+
+__device__ __forceinline__ bool test_bit(const uint64_t *bm, uint32_t u) { return (bm[u >> 6] >> (u & 63)) & 1; }
+
+// Index of the lane owning position j of a wave-wide concatenation whose
+// inclusive prefix sums are `incl` (the number of lanes with incl <= j).
+__device__ __forceinline__ int owner_lane(uint32_t incl, uint32_t j)
+{
+    int k = 0;
+#pragma unroll
+    for (int step = 32; step > 0; step >>= 1) {
+        const uint32_t t = shfl(incl, k + step - 1);
+        if (t <= j) k += step;
+    }
+    return k;
+}
+
+// Snapshot of a counter slot, read by wave 0 of each block after a barrier.
+struct RoundCounts {
+    unsigned long long count, scout;
+    uint32_t qlen;
+    bool overflow, has_huge;
+};
+
+// Wave 0 reads the slot into LDS; the caller must __syncthreads() before use.
+__device__ __forceinline__ void read_counts(const FrontierCtr *c, unsigned long long *s_lvl)
+{
+    if (threadIdx.x < kWave) {
+        const int ln = lane();
+        unsigned long long cnt = 0, sc = 0;
+        if (ln < kStripes) {
+            cnt = ld_relaxed64(&c->stripe[ln].count);
+            sc = ld_relaxed64(&c->stripe[ln].scout);
+        }
+        cnt = wave_sum_u64(cnt);
+        sc = wave_sum_u64(sc);
+        if (ln == 0) {
+            s_lvl[0] = cnt;
+            s_lvl[1] = sc;
+            s_lvl[2] = __hip_atomic_load(&c->qlen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            s_lvl[3] = __hip_atomic_load(&c->overflow, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            s_lvl[4] = __hip_atomic_load(&c->has_huge, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+    }
+}
+
+__device__ __forceinline__ RoundCounts unpack_counts(const unsigned long long *s_lvl, uint32_t qcap)
+{
+    RoundCounts r;
+    r.count = s_lvl[0];
+    r.scout = s_lvl[1];
+    r.qlen = (uint32_t)min((unsigned long long)qcap, s_lvl[2]);
+    r.overflow = s_lvl[3] != 0;
+    r.has_huge = s_lvl[4] != 0;
+    return r;
+}
+
+// Per-block staging of the next queue in LDS.
+struct Stager {
+    uint32_t *qout;        // global queue of the next round
+    uint32_t qcap;
+    FrontierCtr *nxt;
+    uint32_t *s_q;         // LDS buffer
+    uint32_t *s_qn;        // LDS fill count
+    uint32_t lds_cap;
+};
+
+// Stage this wave's `found` vertices: into LDS while the block buffer has
+// room, the remainder straight to the global queue. Wave-convergent.
+__device__ __forceinline__ void stage(const Stager &S, bool found, uint32_t v)
+{
+    const uint64_t m = ballot(found);
+    if (!m) return;
+    const int leader = first_lane(m);
+    uint32_t base = 0;
+    if (lane() == leader) base = atomicAdd(S.s_qn, (uint32_t)__popcll(m));
+    base = shfl(base, leader);
+    const uint32_t slot = base + rank_in(m);
+    const bool to_lds = found && slot < S.lds_cap;
+    if (to_lds) S.s_q[slot] = v;
+    const bool to_glb = found && !to_lds;
+    const uint64_t gm = ballot(to_glb);
+    if (gm) {
+        const int l2 = first_lane(gm);
+        uint32_t gb = 0;
+        if (lane() == l2) gb = atomicAdd(&S.nxt->qlen, (uint32_t)__popcll(gm));
+        gb = shfl(gb, l2);
+        const uint32_t pos = gb + rank_in(gm);
+        if (to_glb) {
+            if (pos < S.qcap) S.qout[pos] = v;
+            else S.nxt->overflow = 1;
+        }
+    }
+}
+
+// Publish the block's staged entries with one atomic (whole block).
+__device__ __forceinline__ void flush_stage(const Stager &S, uint32_t *s_flush)
+{
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const uint32_t k = min(*S.s_qn, S.lds_cap);
+        s_flush[0] = k ? atomicAdd(&S.nxt->qlen, k) : 0u;
+        s_flush[1] = k;
+    }
+    __syncthreads();
+    const uint32_t fb = s_flush[0], fk = s_flush[1];
+    for (uint32_t i = threadIdx.x; i < fk; i += blockDim.x) {
+        const uint32_t pos = fb + i;
+        if (pos < S.qcap) S.qout[pos] = S.s_q[i];
+    }
+    if (threadIdx.x == 0 && fb + fk > S.qcap) S.nxt->overflow = 1;
+}
+
+// Block-reduce this lane's (scout, count, huge) contribution and add it to a
+// stripe of the next slot (one atomic per block per field). s_red: 2 x WPB.
+template <int WPB>
+__device__ __forceinline__ void publish_counts(FrontierCtr *nxt, unsigned long long scout, unsigned long long count,
+                                               bool huge, unsigned long long (*s_red)[WPB])
+{
+    const unsigned long long bsc = wave_sum_u64(scout);
+    const unsigned long long bcn = wave_sum_u64(count);
+    const bool bhg = ballot(huge) != 0;
+    const int wid = threadIdx.x / kWave;
+    if (lane() == 0) {
+        s_red[0][wid] = bsc;
+        s_red[1][wid] = bcn | (bhg ? (1ull << 63) : 0ull);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned long long sc = 0, cn = 0;
+        bool hg = false;
+        for (int i = 0; i < WPB; i++) {
+            sc += s_red[0][i];
+            cn += s_red[1][i] & ~(1ull << 63);
+            hg |= (s_red[1][i] >> 63) != 0;
+        }
+        Stripe &st = nxt->stripe[blockIdx.x % kStripes];
+        if (sc) atomicAdd(&st.scout, sc);
+        if (cn) atomicAdd(&st.count, cn);
+        if (hg) nxt->has_huge = 1;  // same value from every writer
+    }
+}
+
+// The reference's direction rule (PolicyDecisionMaker::GetInitPolicy /
+// GetNextPolicy, include/framework/hybrid_policy.h:218-291) with its uint32
+// workload counters. Returns true for pull.
+struct DirectionPolicy {
+    double alpha, beta;
+    uint32_t total_wl;
+    uint32_t n;
+    int32_t force;  // -1 auto, 0 push, 1 pull
+    bool pull_prev = false;
+
+    __device__ __forceinline__ bool next(uint32_t round, unsigned long long count, unsigned long long scout)
+    {
+        bool pull;
+        if (force >= 0) {
+            pull = force == 1;
+        } else if (round == 0) {
+            pull = false;
+        } else if (pull_prev && (double)count > (double)n / beta) {
+            pull = true;
+        } else {
+            const uint32_t out_wl = pull_prev ? 1u : (uint32_t)scout;
+            pull = (double)out_wl > (double)total_wl / alpha;
+        }
+        if (!pull) total_wl -= (uint32_t)scout;
+        return pull;
+    }
+};
+
+__device__ __forceinline__ void write_trace(DevTrace *trace, uint32_t L, bool pull, unsigned long long count,
+                                            unsigned long long scout, long long t_start)
+{
+    if (L < kTraceRows) {
+        DevTrace t;
+        t.round = L;
+        t.variant = pull ? SG_ASYNC_PULL_DD : SG_ASYNC_PUSH_DD;
+        t.in_nodes = (uint32_t)count;
+        t.pad = 0;
+        t.in_edges = scout;
+        t.ticks = (unsigned long long)(wall_clock64() - t_start);
+        trace[L] = t;
+    }
+}
+
+__device__ __forceinline__ void write_final(DevTrace *trace, DevStats *stats, uint32_t L, uint32_t pushes,
+                                            uint32_t pulls, unsigned long long ptrace, bool ok, long long t_start)
+{
+    if (L < kTraceRows) {
+        DevTrace t;
+        t.round = L;
+        t.variant = -1;
+        t.in_nodes = 0;
+        t.pad = 0;
+        t.in_edges = 0;
+        t.ticks = (unsigned long long)(wall_clock64() - t_start);
+        trace[L] = t;
+    }
+    DevStats s;
+    s.rounds = L;
+    s.push_rounds = pushes;
+    s.pull_rounds = pulls;
+    s.reached = 0;  // exact reach statistics come from sg_engine_get_stats
+    s.reached_edges = 0;
+    s.trace = ptrace;
+    s.max_depth = L ? L - 1 : 0;
+    s.status = ok ? SG_OK : SG_ERR_TIMEOUT;
+    s.iterations = L;
+    s.pad = 0;
+    s.residual = 0.0;
+    *stats = s;
+}
+
+// Map the engine's variant option onto a forced direction (-1 = auto).
+inline int forced_direction(const sg_engine_options &o)
+{
+    if (o.alpha == 0.0 || o.force_variant == SG_ASYNC_PUSH_DD || o.force_variant == SG_SYNC_PUSH_DD) return 0;
+    if (o.force_variant == SG_ASYNC_PULL_DD || o.force_variant == SG_SYNC_PULL_DD) return 1;
+    if (o.force_variant == SG_VARIANT_AUTO) return -1;
+    return -2;  // not a data-driven traversal variant
+}
+
+}  // namespace sg

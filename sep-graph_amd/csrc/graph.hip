@@ -55,12 +55,6 @@ __global__ void k_reach_stats(const uint32_t *value, const uint32_t *row, uint32
     }
 }
 
-// Census for a persistent grid: blocks that land on each XCD.
-__global__ void k_xcd_census(uint32_t *words)
-{
-    if (threadIdx.x == 0) atomicAdd(&words[bar_count(xcc_id())], 1u);
-}
-
 int dev_alloc(sg_graph *g, void **p, uint64_t bytes)
 {
     if (g->nallocs >= 10) SG_FAIL(SG_ERR_STATE, "sg_graph_create: too many allocations");
@@ -186,26 +180,30 @@ extern "C" int sg_graph_create(uint32_t n, uint64_t m, const uint32_t *row_start
             }
         }
     }
-    if (!rc) {
-        // static chunk list of high-degree vertices (host side, from the CSR)
+    // static chunk lists of high-degree vertices (host side): out-edges from
+    // the CSR, in-edges from the CSC (the same list when undirected)
+    auto chunk_list = [&](const uint32_t *offs, uint32_t threshold, const uint2 **dptr, uint32_t *count) -> int {
         std::vector<uint2> huge;
         for (uint32_t u = 0; u < n; u++) {
-            const uint32_t deg = row_start[u + 1] - row_start[u];
-            if (deg <= kHugeDegree) continue;
-            for (uint32_t c = 0; c < deg; c += kHugeChunk) huge.push_back(make_uint2(u, row_start[u] + c));
+            const uint32_t deg = offs[u + 1] - offs[u];
+            if (deg <= threshold) continue;
+            for (uint32_t c = 0; c < deg; c += kHugeChunk) huge.push_back(make_uint2(u, offs[u] + c));
         }
         void *p = nullptr;
-        rc = dev_alloc(g, &p, (uint64_t)huge.size() * sizeof(uint2));
-        if (!rc && !huge.empty()) {
+        int r = dev_alloc(g, &p, (uint64_t)huge.size() * sizeof(uint2));
+        if (!r && !huge.empty()) {
             hipError_t e = hipMemcpy(p, huge.data(), huge.size() * sizeof(uint2), hipMemcpyHostToDevice);
             if (e != hipSuccess) {
-                sg::set_error(std::string("sg_graph_create: huge list upload: ") + hipGetErrorString(e));
-                rc = SG_ERR_HIP;
+                sg::set_error(std::string("sg_graph_create: chunk list upload: ") + hipGetErrorString(e));
+                r = SG_ERR_HIP;
             }
         }
-        d.huge = (const uint2 *)p;
-        d.nhuge = (uint32_t)huge.size();
-    }
+        *dptr = (const uint2 *)p;
+        *count = (uint32_t)huge.size();
+        return r;
+    };
+    if (!rc) rc = chunk_list(row_start, kHugeDegree, &d.huge, &d.nhuge);
+    if (!rc) rc = chunk_list(undirected ? row_start : cs, kPullHubDegree, &d.hugein, &d.nhugein);
     if (rc) {
         sg_graph_destroy(g);
         return rc;
@@ -272,7 +270,7 @@ extern "C" int sg_engine_destroy(sg_engine *e)
 {
     if (!e) return SG_OK;
     if (e->stream) (void)hipStreamSynchronize(e->stream);
-    void *ptrs[] = {e->value, e->bitmaps, e->queues, e->ctr, e->abort_flag, e->stats, e->trace};
+    void *ptrs[] = {e->value, e->bitmaps, e->queues, e->stamp, e->ctr, e->abort_flag, e->stats, e->trace, e->exits};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     for (int i = 0; i < e->ngrids; i++) (void)hipFree(e->grids[i].words);
@@ -308,30 +306,47 @@ extern "C" int sg_engine_set_options(sg_engine *e, const sg_engine_options *o)
 
 namespace sg {
 
-static int census(sg_engine *e, sg_engine::PGrid &pg)
+// Probe one grid size: true when every block was resident at once; fills the
+// per-XCD block counts the barrier needs.
+static int probe_grid(sg_engine *e, sg_engine::PGrid &pg, bool *resident)
 {
     SG_HIP(hipMemsetAsync(pg.words, 0, kBarWords * 4, e->stream));
-    hipLaunchKernelGGL(k_xcd_census, dim3(pg.blocks), dim3(pg.bs), 0, e->stream, pg.words);
-    SG_HIP(hipGetLastError());
-    uint32_t counts[kMaxXcd];
+    int rc = pg.probe(pg.blocks, pg.words);
+    if (rc) return rc;
+    uint32_t counts[kMaxXcd + 2];
     SG_HIP(hipMemcpyAsync(counts, pg.words + bar_count(0), sizeof(counts), hipMemcpyDeviceToHost, e->stream));
     SG_HIP(hipStreamSynchronize(e->stream));
     uint32_t nx = 0, total = 0;
     for (int x = 0; x < kMaxXcd; x++) {
         nx += counts[x] > 0;
         total += counts[x];
+        pg.cnt[x] = counts[x];
     }
-    if (total != (uint32_t)pg.blocks) SG_FAIL(SG_ERR_HIP, "persistent grid: XCD census incomplete");
+    *resident = counts[kMaxXcd + 1] == 0 && total == (uint32_t)pg.blocks;
     pg.nxcd = nx;
+    // the probe's words share none with the barrier's counters, which stay zero
     return SG_OK;
 }
 
-int engine_persistent_grid(sg_engine *e, const void *fn, int bs, sg_engine::PGrid **out)
+static int size_grid(sg_engine *e, sg_engine::PGrid &pg, int per_cu)
+{
+    for (int pc = per_cu; pc >= 1; pc--) {
+        pg.blocks = pc * e->g->num_cus;
+        bool resident = false;
+        int rc = probe_grid(e, pg, &resident);
+        if (rc) return rc;
+        if (resident) return SG_OK;
+    }
+    SG_FAIL(SG_ERR_HIP, "persistent grid: one block per CU is not co-resident");
+}
+
+int engine_persistent_grid(sg_engine *e, const void *fn, int bs, const std::function<int(int, uint32_t *)> &probe,
+                           sg_engine::PGrid **out)
 {
     if (e->needs_bar_reset) {
-        // an aborted run leaves counters mid-way: zero and re-measure every grid
+        // an aborted run leaves counters mid-way: zero and re-probe every grid
         for (int i = 0; i < e->ngrids; i++) {
-            int rc = census(e, e->grids[i]);
+            int rc = size_grid(e, e->grids[i], e->grids[i].blocks / e->g->num_cus);
             if (rc) return rc;
         }
         SG_HIP(hipMemsetAsync(e->abort_flag, 0, 4, e->stream));
@@ -350,13 +365,13 @@ int engine_persistent_grid(sg_engine *e, const void *fn, int bs, sg_engine::PGri
     sg_engine::PGrid pg;
     pg.fn = fn;
     pg.bs = bs;
-    pg.blocks = per_cu * e->g->num_cus;
+    pg.probe = probe;
     hipError_t err = hipMalloc((void **)&pg.words, kBarWords * 4);
     if (err != hipSuccess) {
         (void)hipGetLastError();
         SG_FAIL(SG_ERR_NOMEM, "persistent grid: hipMalloc failed");
     }
-    int rc = census(e, pg);
+    int rc = size_grid(e, pg, per_cu);
     if (rc) {
         (void)hipFree(pg.words);
         return rc;
@@ -364,6 +379,24 @@ int engine_persistent_grid(sg_engine *e, const void *fn, int bs, sg_engine::PGri
     e->grids[e->ngrids] = pg;
     *out = &e->grids[e->ngrids++];
     return SG_OK;
+}
+
+int engine_alloc(void **p, uint64_t bytes)
+{
+    hipError_t err = hipMalloc(p, bytes ? bytes : 4);
+    if (err != hipSuccess) {
+        (void)hipGetLastError();
+        *p = nullptr;
+        sg::set_error(std::string("engine state: hipMalloc failed: ") + hipGetErrorString(err));
+        return SG_ERR_NOMEM;
+    }
+    return SG_OK;
+}
+
+int engine_ensure_stamp(sg_engine *e)
+{
+    if (e->stamp) return SG_OK;
+    return engine_alloc((void **)&e->stamp, (uint64_t)e->g->dev.n * 4);
 }
 
 int engine_ensure_bfs_state(sg_engine *e)
@@ -392,6 +425,8 @@ int engine_ensure_bfs_state(sg_engine *e)
     SG_HIP(hipMemsetAsync(e->abort_flag, 0, 4, e->stream));
     if ((rc = alloc((void **)&e->stats, sizeof(DevStats)))) return rc;
     if ((rc = alloc((void **)&e->trace, sizeof(DevTrace) * kTraceRows))) return rc;
+    if ((rc = alloc((void **)&e->exits, (uint64_t)kMaxBlocks * 16))) return rc;
+    SG_HIP(hipMemsetAsync(e->exits, 0, (size_t)kMaxBlocks * 16, e->stream));
     SG_HIP(hipMemsetAsync(e->stats, 0, sizeof(DevStats), e->stream));
     e->needs_bar_reset = false;
     int khz = 0;
@@ -462,7 +497,8 @@ extern "C" int sg_engine_get_stats(sg_engine *e, sg_stats *out)
         unsigned long long *acc = (unsigned long long *)((char *)e->ctr + kCtrBytes - 64);
         SG_HIP(hipMemsetAsync(acc, 0, 16, e->stream));
         const DevGraph &d = e->g->dev;
-        const uint32_t blocks = std::min<uint32_t>((d.n + 255) / 256, 4096);
+        // few blocks: each adds into the same two words once
+        const uint32_t blocks = std::min<uint32_t>((d.n + 255) / 256, (uint32_t)e->g->num_cus);
         hipLaunchKernelGGL(k_reach_stats, dim3(blocks), dim3(256), 0, e->stream, (const uint32_t *)e->value, d.row,
                            d.n, acc);
         SG_HIP(hipGetLastError());
@@ -501,5 +537,27 @@ extern "C" int sg_engine_get_trace(sg_engine *e, sg_round_trace *out, int max, i
         out[i].time_us = (double)t[i].ticks * us_per_tick;
     }
     *n_out = (int)rows;
+    return SG_OK;
+}
+
+extern "C" int sg_engine_debug_grid(sg_engine *e, int index, uint32_t *info, uint32_t *words, int nwords)
+{
+    if (!e || !info || index < 0) SG_FAIL(SG_ERR_INVALID, "sg_engine_debug_grid: bad argument");
+    if (index >= e->ngrids) SG_FAIL(SG_ERR_RANGE, "sg_engine_debug_grid: no such grid");
+    SG_HIP(hipStreamSynchronize(e->stream));
+    const sg_engine::PGrid &pg = e->grids[index];
+    info[0] = (uint32_t)pg.blocks;
+    info[1] = (uint32_t)pg.bs;
+    info[2] = pg.nxcd;
+    for (int x = 0; x < kMaxXcd; x++) info[3 + x] = pg.cnt[x];
+    if (words && nwords > 0) {
+        // barrier words, then the per-block exit records (4 words per block)
+        const int k = std::min(nwords, kBarWords);
+        SG_HIP(hipMemcpy(words, pg.words, (size_t)k * 4, hipMemcpyDeviceToHost));
+        if (nwords > kBarWords && e->exits) {
+            const int r = std::min(nwords - kBarWords, (int)kMaxBlocks * 4);
+            SG_HIP(hipMemcpy(words + kBarWords, e->exits, (size_t)r * 4, hipMemcpyDeviceToHost));
+        }
+    }
     return SG_OK;
 }

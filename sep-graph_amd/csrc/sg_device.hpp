@@ -98,11 +98,15 @@ constexpr int kBarTop = kBarLine;
 __host__ __device__ constexpr int bar_arrive(int x) { return kBarLine * (2 + x); }
 __host__ __device__ constexpr int bar_release(int x) { return kBarLine * (2 + kMaxXcd + x); }
 __host__ __device__ constexpr int bar_count(int x) { return kBarLine * (2 + 2 * kMaxXcd) + x; }
+// residency probe: blocks registered / blocks whose wait for the others expired
+constexpr int kBarProbeTotal = kBarLine * (2 + 2 * kMaxXcd) + kMaxXcd;
+constexpr int kBarProbeFail = kBarProbeTotal + 1;
 
 struct GridBarrier {
-    uint32_t *words;      // kBarWords, zero-initialised; counts at bar_count(x)
-    uint32_t nxcd;        // XCDs that hold at least one block of the grid
-    uint32_t spin_limit;  // max polls before giving up (sets *abort)
+    uint32_t *words;          // kBarWords, zero-initialised counters
+    uint32_t nxcd;            // XCDs that hold at least one block of the grid
+    uint32_t spin_limit;      // max polls before giving up (sets *abort)
+    uint32_t cnt[kMaxXcd];    // blocks of the grid on each XCD (passed by value)
 };
 
 __device__ __forceinline__ uint32_t xcc_id() { return __builtin_amdgcn_s_getreg((3 << 11) | 20) & 15; }
@@ -122,26 +126,65 @@ __device__ __forceinline__ uint32_t atomic_add_relaxed(uint32_t *p, uint32_t v)
     return __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
+// Loads and stores of flags that another block changes while this one waits.
+// An agent-scope atomic load (sc1 only) can keep hitting a cached copy across
+// polls on gfx950 -- observed as barrier waits that never saw a release -- so
+// polling uses system scope (sc0 sc1), which no cache level may satisfy.
+__device__ __forceinline__ uint32_t poll_load(const uint32_t *p)
+{
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+
+__device__ __forceinline__ void flag_store(uint32_t *p, uint32_t v)
+{
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+
 // Bounded wait until *p != v (want_ne) or *p == v (!want_ne).
 __device__ __forceinline__ bool spin_wait(const uint32_t *p, uint32_t v, bool want_ne, uint32_t limit,
                                           uint32_t *abort)
 {
     uint32_t polls = 0;
     for (;;) {
-        const uint32_t x = atomic_load_relaxed(p);
+        const uint32_t x = poll_load(p);
         if (want_ne ? x != v : x == v) return true;
-        if ((++polls & 63) == 0 && atomic_load_relaxed(abort)) return false;
+        if ((++polls & 63) == 0 && poll_load(abort)) return false;
         if (polls > limit) {
-            atomic_store_relaxed(abort, 1u);
+            flag_store(abort, 1u);
             return false;
         }
         __builtin_amdgcn_s_sleep(1);
     }
 }
 
+// Residency probe, run by a persistent kernel launched with its real grid and
+// resources before first use: every block counts itself on its XCD and then
+// waits (bounded) until all blocks of the grid have done so. A block whose
+// wait expires records a failure: the grid is then larger than what can be
+// co-resident and the host shrinks it. Every block exits either way. All of
+// a block's waves stay alive until thread 0 is done (the closing
+// __syncthreads), so a probe block occupies exactly the wave slots, registers
+// and LDS of a working block.
+__device__ inline void residency_probe(uint32_t *words, uint32_t limit)
+{
+    if (threadIdx.x == 0) {
+        atomicAdd(&words[bar_count(xcc_id())], 1u);
+        atomicAdd(&words[kBarProbeTotal], 1u);
+        uint32_t polls = 0;
+        while (poll_load(&words[kBarProbeTotal]) < gridDim.x) {
+            if (++polls > limit) {
+                atomicAdd(&words[kBarProbeFail], 1u);
+                break;
+            }
+            __builtin_amdgcn_s_sleep(2);
+        }
+    }
+    __syncthreads();
+}
+
 // Generation the barrier words are at when a kernel starts (thread 0 keeps it
 // in a register and advances it per barrier, so no barrier re-reads it).
-__device__ __forceinline__ uint32_t grid_gen(const GridBarrier &gb) { return atomic_load_relaxed(gb.words + kBarGen); }
+__device__ __forceinline__ uint32_t grid_gen(const GridBarrier &gb) { return poll_load(gb.words + kBarGen); }
 
 // Returns false if the wait timed out or another block aborted; the caller
 // must then leave its loop. `gen` is this block's generation (grid_gen at
@@ -158,24 +201,24 @@ __device__ __forceinline__ bool grid_sync(const GridBarrier &gb, uint32_t xcc, u
         uint32_t *w = gb.words;
         const uint32_t gen0 = gen;
         gen = gen0 + 1;
-        const uint32_t cnt = w[bar_count(xcc)];
+        const uint32_t cnt = gb.cnt[xcc];
         uint32_t ok = 1;
         const uint32_t a = atomic_add_relaxed(w + bar_arrive(xcc), 1u);
         if (a == cnt - 1) {
-            atomic_store_relaxed(w + bar_arrive(xcc), 0u);
+            flag_store(w + bar_arrive(xcc), 0u);
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
             const uint32_t t = atomic_add_relaxed(w + kBarTop, 1u);
             if (t == gb.nxcd - 1) {
                 // the reset must complete before the generation flips, or a
                 // leader's next-barrier arrival could be overwritten by it
-                atomic_store_relaxed(w + kBarTop, 0u);
+                flag_store(w + kBarTop, 0u);
                 __builtin_amdgcn_s_waitcnt(0);
-                atomic_store_relaxed(w + kBarGen, gen0 + 1);
+                flag_store(w + kBarGen, gen0 + 1);
             }
             ok = spin_wait(w + kBarGen, gen0, true, gb.spin_limit, abort);
             __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
             __builtin_amdgcn_s_waitcnt(0);
-            atomic_store_relaxed(w + bar_release(xcc), gen0 + 1);
+            flag_store(w + bar_release(xcc), gen0 + 1);
         } else {
             ok = spin_wait(w + bar_release(xcc), gen0 + 1, false, gb.spin_limit, abort);
             __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");

@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <functional>
 #include <string>
 
 #include "../../include/sepgraph.h"
@@ -45,12 +46,19 @@ struct DevGraph {
     // (vertex, first edge), so a push round can spread its edges over the grid.
     const uint2 *huge;
     uint32_t nhuge;           // number of chunk descriptors
+    // Split of the in-edges of every vertex with in-degree > kPullHubDegree
+    // over the CSC (vertex, first in-edge), for pull rounds that visit all
+    // in-edges (SSSP), so no wave walks a long in-edge list alone.
+    const uint2 *hugein;
+    uint32_t nhugein;
 };
 
 // A vertex whose out-degree exceeds this is expanded through the static chunk
 // list instead of by a single wave.
 constexpr uint32_t kHugeDegree = 1024;
 constexpr uint32_t kHugeChunk = 256;
+// In-degree above which a pull round splits a vertex's in-edges into chunks.
+constexpr uint32_t kPullHubDegree = 64;
 
 }  // namespace sg
 
@@ -85,6 +93,8 @@ struct DevStats {
 
 // Rows of the per-round device trace kept per engine.
 constexpr uint32_t kTraceRows = 4096;
+// Upper bound on persistent-grid blocks covered by the per-block exit records.
+constexpr uint32_t kMaxBlocks = 8192;
 
 struct DevTrace {
     uint32_t round;
@@ -110,10 +120,12 @@ struct sg_engine {
     uint64_t *bitmaps = nullptr;  // visited + 2 frontier bitmaps (3 * words)
     uint32_t *queues = nullptr;   // 3 frontier queues of qcap entries
     uint32_t qcap = 0;
+    uint32_t *stamp = nullptr;    // per-vertex round stamp (SSSP frontier membership)
     void *ctr = nullptr;          // kCtrBytes of per-round counters
     uint32_t *abort_flag = nullptr;
     sg::DevStats *stats = nullptr;
     sg::DevTrace *trace = nullptr;    // kTraceRows rows
+    uint32_t *exits = nullptr;        // per-block exit records (diagnostics), kMaxBlocks x 4
     uint32_t trace_rows = 0;          // rows written by the last run (host copy)
     double wall_khz = 100000.0;       // device wall clock rate
     // Persistent-grid configurations (one per kernel and block size): grid
@@ -123,7 +135,10 @@ struct sg_engine {
         int bs = 0;
         int blocks = 0;
         uint32_t nxcd = 0;
+        uint32_t cnt[sg::kMaxXcd] = {};   // blocks per XCD from the probe
         uint32_t *words = nullptr;
+        // launches the kernel in residency-probe mode with a given grid
+        std::function<int(int blocks, uint32_t *words)> probe;
     };
     PGrid grids[8];
     int ngrids = 0;
@@ -135,7 +150,13 @@ struct sg_engine {
 
 namespace sg {
 int engine_ensure_bfs_state(sg_engine *e);
-// Grid for a persistent kernel: blocks = resident blocks per CU x CUs, with a
-// barrier calibrated for it. Barriers are re-zeroed after an aborted run.
-int engine_persistent_grid(sg_engine *e, const void *fn, int bs, sg_engine::PGrid **out);
+int engine_ensure_stamp(sg_engine *e);
+// Device allocation owned by the engine (freed with it); reports SG_ERR_NOMEM.
+int engine_alloc(void **p, uint64_t bytes);
+// Grid for a persistent kernel: the largest blocks-per-CU x CUs (starting
+// from the occupancy API's figure) whose blocks all turn out to be resident
+// when the kernel is launched in probe mode, with the barrier's per-XCD block
+// counts from that probe. Barriers are re-zeroed after an aborted run.
+int engine_persistent_grid(sg_engine *e, const void *fn, int bs,
+                           const std::function<int(int, uint32_t *)> &probe, sg_engine::PGrid **out);
 }  // namespace sg

@@ -1,0 +1,473 @@
+// Hybrid (direction-switching) single-source shortest paths as one
+// persistent kernel per query.
+//
+// Reference path: samples/hybrid_sssp/hybrid_sssp.cu:151-242 drives the same
+// Engine round loop as BFS with the SSSP app: a push round relaxes the
+// out-edges of queued vertices with atomicMin(buffer[dst], buffer[src] + w)
+// and queues every dst it lowers (AccumulateBuffer, hybrid_sssp.cu:98-118);
+// a pull round schedules every vertex and lowers it from in-neighbours that
+// are in the frontier bitmap (CombineValueBuffer, hybrid_sssp.cu:72-95;
+// pull_functor.h:138-188). Distances are uint32, UINT32_MAX when unreached.
+//
+// Here, as for BFS, the round loop, the alpha/beta direction rule and the
+// frontier counters live on the device. Differences from the reference:
+//   * a relaxation first compares against a plain load of dist[v] and only
+//     issues atomicMin when it can lower it (atomics run at ~27 G/s on
+//     MI355X, loads several times faster);
+//   * stamp[v] = r + 1 marks v as a member of round r's frontier; a vertex
+//     lowered several times in a round is queued once per racing winner at
+//     most, and membership tests (hub chunks, pull) read the stamp;
+//   * a pull round owns each vertex with one lane, so the minimum over its
+//     frontier in-neighbours is a segmented wave reduction, not atomics.
+#include "frontier.hpp"
+
+using namespace sg;
+
+namespace {
+
+constexpr uint32_t kInf = 0xFFFFFFFFu;
+
+struct SsspArgs {
+    DevGraph g;
+    uint32_t *dist;
+    uint32_t *stamp;  // frontier r  <=>  stamp == r + 1
+    uint64_t *fbit;   // 3 * W words: rotating frontier bitmaps written by pull rounds
+    uint32_t *queue;  // 3 * qcap entries
+    uint32_t qcap;
+    FrontierCtr *ctr;
+    GridBarrier bar;
+    uint32_t *abort;
+    DevStats *stats;
+    DevTrace *trace;
+    uint32_t source;
+    int32_t force;
+    double alpha, beta;
+    uint32_t total_workload0;
+    uint32_t max_rounds;
+    uint32_t probe;   // launch is a residency probe of the grid
+    uint32_t *exits;  // per-block exit records (diagnostics)
+};
+
+struct Round {
+    const SsspArgs *a;
+    uint32_t L;
+    bool fbit_valid;
+    const uint64_t *fcur;
+    uint64_t *fnext;
+    Stager S;
+    unsigned long long scout, count;
+    bool huge;
+};
+
+__device__ __forceinline__ uint32_t sat_add(uint32_t d, uint32_t w) { return d > kInf - w ? kInf : d + w; }
+
+__device__ __forceinline__ bool in_frontier(const Round &R, uint32_t u)
+{
+    return R.fbit_valid ? test_bit(R.fcur, u) : R.a->stamp[u] == R.L + 1;
+}
+
+__device__ __forceinline__ uint32_t edge_w(const uint32_t *w, uint32_t e) { return w ? w[e] : 1u; }
+
+// Relax u->v with tentative distance nd for `act` lanes; wave-convergent.
+__device__ __forceinline__ void relax(Round &R, bool act, uint32_t v, uint32_t nd)
+{
+    const SsspArgs &a = *R.a;
+    bool queued = false;
+    if (act && nd < a.dist[v]) {
+        const uint32_t old = atomicMin(&a.dist[v], nd);
+        if (nd < old && a.stamp[v] != R.L + 2) {
+            a.stamp[v] = R.L + 2;
+            const uint32_t dv = a.g.row[v + 1] - a.g.row[v];
+            R.scout += dv;
+            R.count++;
+            R.huge |= dv > kHugeDegree;
+            queued = true;
+        }
+    }
+    stage(R.S, queued, v);
+}
+
+// Relax the out-edges of up to 64 queued vertices (one per lane) with
+// wave-level prefix-sum edge scheduling; hubs are left to the chunk list.
+__device__ __forceinline__ void push_expand(Round &R, bool valid, uint32_t u)
+{
+    const DevGraph &g = R.a->g;
+    uint32_t b = 0, d = 0, du = kInf;
+    if (valid) {
+        b = g.row[u];
+        d = g.row[u + 1] - b;
+        if (d > kHugeDegree) d = 0;
+        du = R.a->dist[u];
+        if (du == kInf) d = 0;
+    }
+    const uint32_t incl = wave_incl_scan(d);
+    const uint32_t total = shfl(incl, kWave - 1);
+    const uint32_t off = b - (incl - d);
+    const int ln = lane();
+    for (uint32_t jb = 0; jb < total; jb += kWave) {
+        const uint32_t j = jb + ln;
+        const bool act = j < total;
+        const int k = owner_lane(incl, j);
+        const uint32_t e = shfl(off, k) + j;
+        const uint32_t duk = shfl(du, k);
+        uint32_t v = 0, nd = kInf;
+        if (act) {
+            v = g.col[e];
+            nd = sat_add(duk, edge_w(g.w, e));
+        }
+        relax(R, act, v, nd);
+    }
+}
+
+// Pull step for the 64 vertices of word w: each lane's vertex takes the
+// minimum of dist[u] + w over in-neighbours u in the frontier.
+__device__ __forceinline__ void pull_word(Round &R, uint32_t w)
+{
+    const SsspArgs &a = *R.a;
+    const DevGraph &g = a.g;
+    const int ln = lane();
+    const uint32_t v = w * 64 + ln;
+    uint32_t b = 0, e = 0, best = kInf;
+    if (v < g.n) {
+        b = g.cstart[v];
+        e = g.cstart[v + 1];
+        best = a.dist[v];
+        if (e - b > kPullHubDegree) b = e;  // long in-edge lists go through the chunk list
+    }
+    const uint32_t dv0 = best;
+    for (;;) {
+        const uint32_t r = e - b;
+        const uint32_t incl = wave_incl_scan(r);
+        const uint32_t total = shfl(incl, kWave - 1);
+        if (total == 0) break;
+        const uint32_t excl = incl - r;
+        const uint32_t j = (uint32_t)ln;
+        const int k = owner_lane(incl, j);
+        const uint32_t bk = shfl(b, k), xk = shfl(excl, k);
+        uint32_t cand = kInf;
+        if (j < total) {
+            const uint32_t ed = bk + (j - xk);
+            const uint32_t u = g.csrc[ed];
+            if (in_frontier(R, u)) {
+                const uint32_t du = a.dist[u];
+                if (du != kInf) cand = sat_add(du, edge_w(g.cw, ed));
+            }
+        }
+        // segmented minimum over lanes with the same owner (owners are contiguous)
+        uint32_t m = cand;
+#pragma unroll
+        for (int d = 1; d < kWave; d <<= 1) {
+            const uint32_t y = __shfl_up(m, d, kWave);
+            const int ko = __shfl_up(k, d, kWave);
+            if (ln >= d && ko == k) m = min(m, y);
+        }
+        const uint32_t lo = min(excl, 64u), hi = min(incl, 64u), cov = hi - lo;
+        const uint32_t seg = shfl(m, (int)(hi > 0 ? hi - 1 : 0));
+        if (cov) {
+            best = min(best, seg);
+            b += cov;
+        }
+    }
+    const bool improved = v < g.n && best < dv0;
+    if (improved) {
+        a.dist[v] = best;
+        a.stamp[v] = R.L + 2;
+        const uint32_t dv = g.row[v + 1] - g.row[v];
+        R.scout += dv;
+        R.huge |= dv > kHugeDegree;
+    }
+    const uint64_t fm = ballot(improved);
+    if (ln == 0 && fm) {
+        // hubs of this word may set bits concurrently from the chunk list
+        atomicOr((unsigned long long *)&R.fnext[w], (unsigned long long)fm);
+        R.count += __popcll(fm);
+    }
+}
+
+// Pull for one chunk of a hub's in-edges: the chunk's minimum candidate is
+// merged with atomicMin, as several waves share the hub.
+__device__ __forceinline__ void pull_hub_chunk(Round &R, uint32_t v, uint32_t beg)
+{
+    const SsspArgs &a = *R.a;
+    const DevGraph &g = a.g;
+    const int ln = lane();
+    const uint32_t end = min(g.cstart[v + 1], beg + kHugeChunk);
+    uint32_t m = kInf;
+    for (uint32_t e0 = beg; e0 < end; e0 += kWave) {
+        const uint32_t ed = e0 + ln;
+        if (ed < end) {
+            const uint32_t u = g.csrc[ed];
+            if (in_frontier(R, u)) {
+                const uint32_t du = a.dist[u];
+                if (du != kInf) m = min(m, sat_add(du, edge_w(g.cw, ed)));
+            }
+        }
+    }
+#pragma unroll
+    for (int d = kWave / 2; d > 0; d >>= 1) m = min(m, (uint32_t)__shfl_xor(m, d, kWave));
+    if (ln == 0 && m < a.dist[v]) {
+        const uint32_t old = atomicMin(&a.dist[v], m);
+        if (m < old) {
+            atomicOr((unsigned long long *)&R.fnext[v >> 6], 1ull << (v & 63));
+            if (a.stamp[v] != R.L + 2) {
+                a.stamp[v] = R.L + 2;
+                const uint32_t dv = g.row[v + 1] - g.row[v];
+                R.scout += dv;
+                R.count++;
+                R.huge |= dv > kHugeDegree;
+            }
+        }
+    }
+}
+
+template <int BS>
+__global__ __launch_bounds__(BS) void k_sssp(SsspArgs a)
+{
+    if (a.probe) {
+        residency_probe(a.bar.words, 1u << 20);
+        return;
+    }
+    constexpr int WPB = BS / kWave;
+    constexpr uint32_t kLdsQ = 8u * BS;
+    const DevGraph &g = a.g;
+    const uint32_t n = g.n, W = g.words;
+    const uint32_t gtid = blockIdx.x * BS + threadIdx.x;
+    const uint32_t nthreads = gridDim.x * BS;
+    const uint32_t gwave = blockIdx.x * WPB + threadIdx.x / kWave;
+    const uint32_t nwaves = gridDim.x * WPB;
+    const int ln = lane();
+    const uint32_t xcc = xcc_id();
+    uint32_t gen = threadIdx.x == 0 ? grid_gen(a.bar) : 0u;
+    const long long t_start = wall_clock64();
+    const uint32_t wpw = (W + nwaves - 1) / nwaves;
+    const uint32_t wbeg = min(W, gwave * wpw), wend = min(W, wbeg + wpw);
+
+    __shared__ unsigned long long s_red[2][WPB];
+    __shared__ unsigned long long s_lvl[5];
+    __shared__ uint32_t s_q[kLdsQ];
+    __shared__ uint32_t s_qn, s_flush[2];
+
+    const uint32_t src = a.source;
+    for (uint32_t i = gtid; i < n; i += nthreads) {
+        a.dist[i] = (i == src) ? 0u : kInf;
+        a.stamp[i] = (i == src) ? 1u : 0u;
+    }
+    // frontier bitmaps of rounds 0 and 1 start clear (round L clears L+2's)
+    for (uint32_t w = gtid; w < 2 * W; w += nthreads) a.fbit[w] = 0;
+    if (blockIdx.x == 0) {
+        clear_ctr(&a.ctr[0]);
+        clear_ctr(&a.ctr[1]);
+        clear_ctr(&a.ctr[2]);
+        block_sync_global();
+        if (threadIdx.x == 0) {
+            const uint32_t deg = g.row[src + 1] - g.row[src];
+            a.queue[0] = src;
+            a.ctr[0].qlen = 1;
+            a.ctr[0].has_huge = deg > kHugeDegree;
+            a.ctr[0].stripe[0].scout = deg;
+            a.ctr[0].stripe[0].count = 1;
+        }
+    }
+    bool ok = grid_sync(a.bar, xcc, gen, a.abort);
+
+    DirectionPolicy pol{a.alpha, a.beta, a.total_workload0, n, a.force};
+    unsigned long long ptrace = 0;
+    uint32_t pushes = 0, pulls = 0;
+    uint32_t L = 0;
+    uint32_t reason = 3;  // 1: empty frontier, 2: round cap, 3: barrier failed
+    unsigned long long last_count = ~0ull;
+    while (ok) {
+        read_counts(&a.ctr[L % 3], s_lvl);
+        if (threadIdx.x == 0) s_qn = 0;
+        __syncthreads();
+        const RoundCounts rc = unpack_counts(s_lvl, a.qcap);
+        __syncthreads();
+        last_count = rc.count;
+        if (rc.count == 0) {
+            reason = 1;
+            break;
+        }
+        if (a.max_rounds && L >= a.max_rounds) {
+            reason = 2;
+            break;
+        }
+
+        const bool pull = pol.next(L, rc.count, rc.scout);
+        if (pull) {
+            pulls++;
+            if (L < 64) ptrace |= 1ull << L;
+        } else {
+            pushes++;
+        }
+        if (gtid == 0) write_trace(a.trace, L, pull, rc.count, rc.scout, t_start);
+        if (blockIdx.x == 0) clear_ctr(&a.ctr[(L + 2) % 3]);
+
+        Round R;
+        R.a = &a;
+        R.L = L;
+        R.fbit_valid = pol.pull_prev;
+        R.fcur = a.fbit + (size_t)(L % 3) * W;
+        R.fnext = a.fbit + (size_t)((L + 1) % 3) * W;
+        // the bitmap for round L+2 is free (read in round L-1): clear it
+        uint64_t *fclr = a.fbit + (size_t)((L + 2) % 3) * W;
+        for (uint32_t w = gtid; w < W; w += nthreads) fclr[w] = 0;
+        R.S = Stager{a.queue + (size_t)((L + 1) % 3) * a.qcap, a.qcap, &a.ctr[(L + 1) % 3], s_q, &s_qn, kLdsQ};
+        R.scout = 0;
+        R.count = 0;
+        R.huge = false;
+        const uint32_t *qin = a.queue + (size_t)(L % 3) * a.qcap;
+
+        if (!pull) {
+            const uint32_t nhuge = rc.has_huge ? g.nhuge : 0u;
+            for (uint32_t base = 0; base < nhuge; base += nwaves * kWave) {
+                const uint32_t c = base + (uint32_t)ln * nwaves + gwave;
+                uint2 dsc = make_uint2(0u, 0u);
+                bool mem = false;
+                if (c < nhuge) {
+                    dsc = g.huge[c];
+                    mem = in_frontier(R, dsc.x);
+                }
+                uint64_t mm = ballot(mem);
+                while (mm) {
+                    const int t = first_lane(mm);
+                    mm &= mm - 1;
+                    const uint32_t u = shfl(dsc.x, t), beg = shfl(dsc.y, t);
+                    const uint32_t end = min(g.row[u + 1], beg + kHugeChunk);
+                    const uint32_t du = a.dist[u];
+                    for (uint32_t e0 = beg; e0 < end; e0 += kWave) {
+                        const uint32_t e = e0 + ln;
+                        const bool act = e < end && du != kInf;
+                        uint32_t v = 0, nd = kInf;
+                        if (act) {
+                            v = g.col[e];
+                            nd = sat_add(du, edge_w(g.w, e));
+                        }
+                        relax(R, act, v, nd);
+                    }
+                }
+            }
+            if (pol.pull_prev) {
+                for (uint32_t wb = wbeg; wb < wend; wb += kWave) {
+                    const uint32_t nw = min((uint32_t)kWave, wend - wb);
+                    const uint64_t fl = (uint32_t)ln < nw ? R.fcur[wb + ln] : 0ull;
+                    uint64_t todo = ballot(fl != 0);
+                    while (todo) {
+                        const int t = first_lane(todo);
+                        todo &= todo - 1;
+                        const uint64_t fw = shfl(fl, t);
+                        push_expand(R, (fw >> ln) & 1, (wb + t) * 64 + ln);
+                    }
+                }
+            } else if (!rc.overflow) {
+                const uint32_t per = (rc.qlen + nwaves - 1) / nwaves;
+                const uint32_t qb = min(rc.qlen, gwave * per), qe = min(rc.qlen, qb + per);
+                for (uint32_t i0 = qb; i0 < qe; i0 += kWave) {
+                    const uint32_t i = i0 + ln;
+                    const bool valid = i < qe;
+                    push_expand(R, valid, valid ? qin[i] : 0u);
+                }
+            } else {
+                for (uint32_t w = gwave; w < W; w += nwaves) {
+                    const uint32_t v = w * 64 + ln;
+                    push_expand(R, v < n && a.stamp[v] == L + 1, v);
+                }
+            }
+            flush_stage(R.S, s_flush);
+        } else {
+            // Pull: (1) hubs' in-edges through the static chunk list, spread
+            // over all waves; (2) every other vertex with in-edges, a word at a
+            // time from this wave's range.
+            for (uint32_t c = gwave; c < g.nhugein; c += nwaves) {
+                const uint2 dsc = g.hugein[c];
+                pull_hub_chunk(R, dsc.x, dsc.y);
+            }
+            // words interleaved over waves: RMAT's heavy vertices cluster at
+            // low ids, so contiguous ranges would leave a few waves with most
+            // of the in-edges
+            for (uint32_t w = gwave; w < W; w += nwaves) {
+                if (g.inert && g.inert[w] == ~0ull) continue;
+                pull_word(R, w);
+            }
+        }
+
+        publish_counts<WPB>(R.S.nxt, R.scout, R.count, R.huge, s_red);
+        pol.pull_prev = pull;
+        L++;
+        ok = grid_sync(a.bar, xcc, gen, a.abort);
+    }
+
+    if (gtid == 0) write_final(a.trace, a.stats, L, pushes, pulls, ptrace, ok, t_start);
+    if (threadIdx.x == 0 && blockIdx.x < kMaxBlocks) {
+        uint4 rec = make_uint4(L, reason, (uint32_t)min(last_count, 0xFFFFFFFFull), xcc);
+        *(uint4 *)&a.exits[blockIdx.x * 4] = rec;
+    }
+}
+
+template <int BS>
+int launch_sssp(sg_engine *e, SsspArgs &args)
+{
+    sg_engine::PGrid *pg = nullptr;
+    auto probe = [e, args](int blocks, uint32_t *words) -> int {
+        SsspArgs p = args;
+        p.probe = 1;
+        p.bar.words = words;
+        hipLaunchKernelGGL(k_sssp<BS>, dim3(blocks), dim3(BS), 0, e->stream, p);
+        SG_HIP(hipGetLastError());
+        return SG_OK;
+    };
+    int rc = engine_persistent_grid(e, (const void *)k_sssp<BS>, BS, probe, &pg);
+    if (rc) return rc;
+    args.bar.words = pg->words;
+    args.bar.nxcd = pg->nxcd;
+    for (int x = 0; x < kMaxXcd; x++) args.bar.cnt[x] = pg->cnt[x];
+    SG_HIP(hipEventRecord(e->ev0, e->stream));
+    hipLaunchKernelGGL(k_sssp<BS>, dim3(pg->blocks), dim3(BS), 0, e->stream, args);
+    SG_HIP(hipGetLastError());
+    SG_HIP(hipEventRecord(e->ev1, e->stream));
+    return SG_OK;
+}
+
+}  // namespace
+
+extern "C" int sg_sssp(sg_engine *e, uint32_t source)
+{
+    if (!e) SG_FAIL(SG_ERR_INVALID, "sg_sssp: null engine");
+    const DevGraph &d = e->g->dev;
+    if (source >= d.n) SG_FAIL(SG_ERR_RANGE, "sg_sssp: source >= n");
+    const sg_engine_options &o = e->opt;
+    const int force = forced_direction(o);
+    if (force < -1) SG_FAIL(SG_ERR_UNSUPPORTED, "sg_sssp: only data-driven push/pull variants apply to SSSP");
+    int rc = engine_ensure_bfs_state(e);
+    if (rc) return rc;
+    if ((rc = engine_ensure_stamp(e))) return rc;
+    SsspArgs a{};
+    a.g = d;
+    a.dist = (uint32_t *)e->value;
+    a.stamp = e->stamp;
+    a.fbit = e->bitmaps;  // three rotating frontier bitmaps
+    a.queue = e->queues;
+    a.qcap = e->qcap;
+    a.ctr = (FrontierCtr *)e->ctr;
+    a.bar.spin_limit = 1u << 22;
+    a.abort = e->abort_flag;
+    a.stats = e->stats;
+    a.trace = e->trace;
+    a.exits = e->exits;
+    a.source = source;
+    a.force = force;
+    a.alpha = o.alpha == 0.0 ? 1.0 : o.alpha;
+    a.beta = o.beta;
+    a.total_workload0 = (uint32_t)((double)d.m * o.edge_factor);
+    a.max_rounds = (uint32_t)o.max_iteration;
+    e->last_algo = 1;
+    const int bs = o.block_size ? o.block_size : 1024;
+    switch (bs) {
+    case 256: rc = launch_sssp<256>(e, a); break;
+    case 512: rc = launch_sssp<512>(e, a); break;
+    case 1024: rc = launch_sssp<1024>(e, a); break;
+    default: SG_FAIL(SG_ERR_UNSUPPORTED, "sg_sssp: block_size must be 256, 512 or 1024");
+    }
+    if (rc) return rc;
+    e->timed = true;
+    return SG_OK;
+}

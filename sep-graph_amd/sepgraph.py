@@ -135,11 +135,13 @@ def lib() -> C.CDLL:
             "sg_engine_set_stream": ([_P, _P], C.c_int),
             "sg_engine_set_options": ([_P, C.POINTER(EngineOptions)], C.c_int),
             "sg_bfs": ([_P, C.c_uint32], C.c_int),
+            "sg_sssp": ([_P, C.c_uint32], C.c_int),
             "sg_engine_sync": ([_P], C.c_int),
             "sg_engine_value_ptr": ([_P, C.POINTER(_P), C.POINTER(C.c_uint64)], C.c_int),
             "sg_engine_gather_value": ([_P, _P, C.c_uint64], C.c_int),
             "sg_engine_get_stats": ([_P, C.POINTER(Stats)], C.c_int),
             "sg_engine_get_trace": ([_P, C.POINTER(RoundTrace), C.c_int, C.POINTER(C.c_int)], C.c_int),
+            "sg_engine_debug_grid": ([_P, C.c_int, _P, _P, C.c_int], C.c_int),
         }
         for name, (args, res) in sig.items():
             f = getattr(L, name)
@@ -322,6 +324,9 @@ class Engine:
     def bfs(self, source: int):
         _check(lib().sg_bfs(self.h, int(source)))
 
+    def sssp(self, source: int):
+        _check(lib().sg_sssp(self.h, int(source)))
+
     def sync(self):
         _check(lib().sg_engine_sync(self.h))
 
@@ -352,6 +357,12 @@ class Engine:
              "in_edges": r.in_edges, "time_us": r.time_us}
             for r in rows[: nrow.value]
         ]
+
+    def debug_grid(self, index: int = 0, nwords: int = 1600):
+        info = np.zeros(19, np.uint32)
+        words = np.zeros(nwords, np.uint32)
+        _check(lib().sg_engine_debug_grid(self.h, index, _ptr(info), _ptr(words), nwords))
+        return info, words
 
     def close(self):
         if self.h:
