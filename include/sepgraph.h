@@ -207,6 +207,24 @@ int sg_bfs(sg_engine *e, uint32_t source);
  * Asynchronous. */
 int sg_sssp(sg_engine *e, uint32_t source);
 
+/* PageRank schedules. SG_PR_PULL_TD is the reference's SYNC_PULL_TD
+ * (topology-driven pull, RunSyncPullTD, include/framework/algo_variants.cuh:753-823);
+ * SG_PR_PUSH_DD is its ASYNC_PUSH_DD residual push (hybrid_pr.cu:66-110). */
+enum { SG_PR_AUTO = 0, SG_PR_PULL_TD = 1, SG_PR_PUSH_DD = 2 };
+
+/* PageRank with damping `damping`, uniform teleport and uniform redistribution
+ * of dangling mass, ranks summing to 1 (FP32 per vertex, the engine's value
+ * array). SG_PR_PULL_TD iterates x' = (1-d)/n + d (P^T x + dangling(x)/n)
+ * from x = 1/n and stops after the first iteration whose L1 change (reduced in
+ * FP64 on the device) is below tol, or after max_iter iterations.
+ * SG_PR_PUSH_DD pushes residuals until the outstanding residual mass is below
+ * tol * (1 - d) and normalises. Per-vertex sums are accumulated in FP64.
+ * sg_stats.iterations / .residual report the run. Asynchronous.
+ * The reference's PR app solves the unnormalised system without dangling
+ * redistribution (samples/hybrid_pr/hybrid_pr.cu:26-125); for uniform
+ * teleport its solution normalised to sum 1 is this ranking vector. */
+int sg_pagerank(sg_engine *e, double damping, double tol, int32_t max_iter, int32_t mode);
+
 /* Synchronise the engine's stream and report the device-side status. */
 int sg_engine_sync(sg_engine *e);
 /* Device pointer and size in bytes of the last run's per-vertex result. */

@@ -57,7 +57,7 @@ __global__ void k_reach_stats(const uint32_t *value, const uint32_t *row, uint32
 
 int dev_alloc(sg_graph *g, void **p, uint64_t bytes)
 {
-    if (g->nallocs >= 10) SG_FAIL(SG_ERR_STATE, "sg_graph_create: too many allocations");
+    if (g->nallocs >= 16) SG_FAIL(SG_ERR_STATE, "sg_graph_create: too many allocations");
     hipError_t e = hipMalloc(p, bytes ? bytes : 4);
     if (e != hipSuccess) {
         (void)hipGetLastError();
@@ -182,28 +182,37 @@ extern "C" int sg_graph_create(uint32_t n, uint64_t m, const uint32_t *row_start
     }
     // static chunk lists of high-degree vertices (host side): out-edges from
     // the CSR, in-edges from the CSC (the same list when undirected)
-    auto chunk_list = [&](const uint32_t *offs, uint32_t threshold, const uint2 **dptr, uint32_t *count) -> int {
-        std::vector<uint2> huge;
-        for (uint32_t u = 0; u < n; u++) {
-            const uint32_t deg = offs[u + 1] - offs[u];
-            if (deg <= threshold) continue;
-            for (uint32_t c = 0; c < deg; c += kHugeChunk) huge.push_back(make_uint2(u, offs[u] + c));
-        }
+    auto upload2 = [&](const std::vector<uint2> &h, const uint2 **dptr, uint32_t *count) -> int {
         void *p = nullptr;
-        int r = dev_alloc(g, &p, (uint64_t)huge.size() * sizeof(uint2));
-        if (!r && !huge.empty()) {
-            hipError_t e = hipMemcpy(p, huge.data(), huge.size() * sizeof(uint2), hipMemcpyHostToDevice);
+        int r = dev_alloc(g, &p, (uint64_t)h.size() * sizeof(uint2));
+        if (!r && !h.empty()) {
+            hipError_t e = hipMemcpy(p, h.data(), h.size() * sizeof(uint2), hipMemcpyHostToDevice);
             if (e != hipSuccess) {
                 sg::set_error(std::string("sg_graph_create: chunk list upload: ") + hipGetErrorString(e));
                 r = SG_ERR_HIP;
             }
         }
         *dptr = (const uint2 *)p;
-        *count = (uint32_t)huge.size();
+        *count = (uint32_t)h.size();
         return r;
     };
-    if (!rc) rc = chunk_list(row_start, kHugeDegree, &d.huge, &d.nhuge);
-    if (!rc) rc = chunk_list(undirected ? row_start : cs, kPullHubDegree, &d.hugein, &d.nhugein);
+    auto chunk_list = [&](const uint32_t *offs, uint32_t threshold, std::vector<uint2> &huge,
+                          std::vector<uint2> *hubs) {
+        for (uint32_t u = 0; u < n; u++) {
+            const uint32_t deg = offs[u + 1] - offs[u];
+            if (deg <= threshold) continue;
+            if (hubs) hubs->push_back(make_uint2((uint32_t)huge.size(), (deg + kHugeChunk - 1) / kHugeChunk));
+            for (uint32_t c = 0; c < deg; c += kHugeChunk) huge.push_back(make_uint2(u, offs[u] + c));
+        }
+    };
+    {
+        std::vector<uint2> out_chunks, in_chunks, in_hubs;
+        chunk_list(row_start, kHugeDegree, out_chunks, nullptr);
+        chunk_list(undirected ? row_start : cs, kPullHubDegree, in_chunks, &in_hubs);
+        if (!rc) rc = upload2(out_chunks, &d.huge, &d.nhuge);
+        if (!rc) rc = upload2(in_chunks, &d.hugein, &d.nhugein);
+        if (!rc) rc = upload2(in_hubs, &d.hubin, &d.nhubin);
+    }
     if (rc) {
         sg_graph_destroy(g);
         return rc;
@@ -270,7 +279,8 @@ extern "C" int sg_engine_destroy(sg_engine *e)
 {
     if (!e) return SG_OK;
     if (e->stream) (void)hipStreamSynchronize(e->stream);
-    void *ptrs[] = {e->value, e->bitmaps, e->queues, e->stamp, e->ctr, e->abort_flag, e->stats, e->trace, e->exits};
+    void *ptrs[] = {e->value, e->bitmaps, e->queues, e->stamp, e->ctr,
+                    e->abort_flag, e->stats, e->trace, e->exits, e->aux};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     for (int i = 0; i < e->ngrids; i++) (void)hipFree(e->grids[i].words);
@@ -397,6 +407,21 @@ int engine_ensure_stamp(sg_engine *e)
 {
     if (e->stamp) return SG_OK;
     return engine_alloc((void **)&e->stamp, (uint64_t)e->g->dev.n * 4);
+}
+
+int engine_ensure_aux(sg_engine *e, uint64_t bytes)
+{
+    if (e->aux && e->aux_bytes >= bytes) return SG_OK;
+    if (e->aux) {
+        SG_HIP(hipStreamSynchronize(e->stream));
+        (void)hipFree(e->aux);
+        e->aux = nullptr;
+        e->aux_bytes = 0;
+    }
+    int rc = engine_alloc(&e->aux, bytes);
+    if (rc) return rc;
+    e->aux_bytes = bytes;
+    return SG_OK;
 }
 
 int engine_ensure_bfs_state(sg_engine *e)

@@ -1,0 +1,685 @@
+// PageRank as persistent kernels: a topology-driven pull (power iteration)
+// and a data-driven residual propagation that switches between push and pull
+// per round.
+//
+// Reference path: samples/hybrid_pr/hybrid_pr.cu:26-125 (the PageRank app:
+// residual push, AccumulateBuffer with float atomicAdd) driven by
+// RunSyncPullTD (include/framework/algo_variants.cuh:753-823, topology-driven
+// pull over the CSC) or RunAsyncPushDD (algo_variants.cuh:133-203, worklist of
+// vertices whose residual crossed the threshold, hybrid_pr.cu:66-110).
+//
+// The ranks computed here are the normalised PageRank with uniform teleport
+// and uniform redistribution of dangling mass,
+//     x = (1-d)/n + d (P^T x + (sum of x over dangling vertices)/n),
+// stored as FP32. Per-vertex sums, the dangling mass and the L1 residual are
+// accumulated in FP64. In the pull mode every reduction runs in a fixed order
+// for a given grid, so repeated runs give identical bits.
+//
+// Pull TD: one lane per vertex sums x[u]/outdeg(u) (kept as an FP32 array
+// `contrib`, ping-ponged between iterations) over its in-edges; the wave walks
+// the concatenated in-edge lists 64 edges per step and reduces each lane's
+// segment with a segmented shuffle scan. Vertices with in-degree above
+// kPullHubDegree are summed per 256-edge chunk into FP64 slots and finished
+// after a grid barrier, one wave per hub, in chunk order.
+//
+// Residual DD: solves the unnormalised system y = (1-d)/n + d P^T y by
+// residual propagation (dangling mass leaks) and normalises y at the end; for
+// uniform teleport this is the same ranking vector, x = y / |y|_1. A round
+// activates every vertex whose residual reached theta = tol (1-d) / n (the
+// rounds' worklist holds vertices whose residual crossed theta), moves the
+// residual into y and spreads d r / outdeg along out-edges: with FP64 atomics
+// when the active out-degree is small (push), or by a gather over in-edges
+// from active sources when it exceeds m / alpha (pull).
+#include "frontier.hpp"
+
+using namespace sg;
+
+namespace {
+
+// ------------------------------------------------------------------ helpers
+
+// Deterministic block sum of two doubles; the result is valid in thread 0.
+// s_red holds one double2 per wave.
+template <int WPB>
+__device__ __forceinline__ double2 block_sum2(double a, double b, double2 *s_red)
+{
+    a = wave_sum_f64(a);
+    b = wave_sum_f64(b);
+    const int wid = threadIdx.x / kWave;
+    if (lane() == 0) s_red[wid] = make_double2(a, b);
+    __syncthreads();
+    double2 t = make_double2(0.0, 0.0);
+    if (threadIdx.x == 0) {
+        for (int i = 0; i < WPB; i++) {
+            t.x += s_red[i].x;
+            t.y += s_red[i].y;
+        }
+    }
+    __syncthreads();
+    return t;
+}
+
+// Sum of the per-block partials part[0 .. nblocks) in a fixed order; every
+// block computes the same value. Result returned to all threads.
+template <int BS>
+__device__ __forceinline__ double2 grid_sum2(const double2 *part, uint32_t nblocks, double2 *s_red, double2 *s_out)
+{
+    double a = 0.0, b = 0.0;
+    for (uint32_t i = threadIdx.x; i < nblocks; i += BS) {
+        const double2 q = part[i];
+        a += q.x;
+        b += q.y;
+    }
+    const double2 t = block_sum2<BS / kWave>(a, b, s_red);
+    if (threadIdx.x == 0) *s_out = t;
+    __syncthreads();
+    const double2 r = *s_out;
+    __syncthreads();
+    return r;
+}
+
+// Segmented inclusive sum across lanes whose segment owner k is the same
+// (owners are non-decreasing along the wave).
+__device__ __forceinline__ double seg_sum(double s, int k)
+{
+    const int ln = lane();
+#pragma unroll
+    for (int d = 1; d < kWave; d <<= 1) {
+        const double y = __shfl_up(s, d, kWave);
+        const int ko = __shfl_up(k, d, kWave);
+        if (ln >= d && ko == k) s += y;
+    }
+    return s;
+}
+
+// Sum over the in-edges [b, e) of each lane's vertex of val(edge) (FP64),
+// the wave walking the concatenated lists 64 edges per step.
+template <typename F>
+__device__ __forceinline__ double wave_gather_sum(uint32_t b, uint32_t e, F val)
+{
+    const int ln = lane();
+    double acc = 0.0;
+    for (;;) {
+        const uint32_t r = e - b;
+        const uint32_t incl = wave_incl_scan(r);
+        const uint32_t total = shfl(incl, kWave - 1);
+        if (total == 0) break;
+        const uint32_t excl = incl - r;
+        const uint32_t j = (uint32_t)ln;
+        const int k = owner_lane(incl, j);
+        const uint32_t bk = shfl(b, k), xk = shfl(excl, k);
+        double cand = 0.0;
+        if (j < total) cand = val(bk + (j - xk));
+        const double s = seg_sum(cand, k);
+        const uint32_t lo = min(excl, 64u), hi = min(incl, 64u), cov = hi - lo;
+        const double seg = __shfl(s, (int)(hi > 0 ? hi - 1 : 0), kWave);
+        if (cov) {
+            acc += seg;
+            b += cov;
+        }
+    }
+    return acc;
+}
+
+// ------------------------------------------------------------ pull TD kernel
+
+struct PrArgs {
+    DevGraph g;
+    float *x;          // ranks [n]
+    float *contrib;    // 2 x n: x[u] / outdeg(u) of the current / next iteration
+    double *hubpart;   // [nhugein] chunk sums
+    double2 *part;     // 2 x kMaxBlocks (residual, dangling) partials
+    GridBarrier bar;
+    uint32_t *abort;
+    DevStats *stats;
+    DevTrace *trace;
+    double damping, tol;
+    uint32_t max_iter;
+    uint32_t probe;
+};
+
+template <int BS>
+__global__ __launch_bounds__(BS) void k_pr_pull(PrArgs a)
+{
+    if (a.probe) {
+        residency_probe(a.bar.words, 1u << 20);
+        return;
+    }
+    constexpr int WPB = BS / kWave;
+    const DevGraph &g = a.g;
+    const uint32_t n = g.n, W = g.words;
+    const uint32_t gtid = blockIdx.x * BS + threadIdx.x;
+    const uint32_t nthreads = gridDim.x * BS;
+    const uint32_t gwave = blockIdx.x * WPB + threadIdx.x / kWave;
+    const uint32_t nwaves = gridDim.x * WPB;
+    const int ln = lane();
+    const uint32_t xcc = xcc_id();
+    uint32_t gen = threadIdx.x == 0 ? grid_gen(a.bar) : 0u;
+    const long long t_start = wall_clock64();
+    const double d = a.damping;
+    const double inv_n = 1.0 / (double)n;
+
+    __shared__ double2 s_red[WPB];
+    __shared__ double2 s_glob;
+
+    // x = 1/n; contributions of iteration 0; dangling mass
+    double dang = 0.0;
+    for (uint32_t v = gtid; v < n; v += nthreads) {
+        a.x[v] = (float)inv_n;
+        const uint32_t dout = g.row[v + 1] - g.row[v];
+        a.contrib[v] = dout ? (float)(inv_n / dout) : 0.0f;
+        if (!dout) dang += inv_n;
+    }
+    {
+        const double2 t = block_sum2<WPB>(0.0, dang, s_red);
+        if (threadIdx.x == 0) a.part[blockIdx.x] = t;
+    }
+    bool ok = grid_sync(a.bar, xcc, gen, a.abort);
+
+    uint32_t it = 0;
+    double res = INFINITY;
+    while (ok) {
+        const double2 gs = grid_sum2<BS>(a.part + (size_t)(it & 1) * kMaxBlocks, gridDim.x, s_red, &s_glob);
+        if (it > 0) res = gs.x;
+        if ((it > 0 && res < a.tol) || it >= a.max_iter) break;
+        const double base = (1.0 - d) * inv_n + d * gs.y * inv_n;
+        const float *cc = a.contrib + (size_t)(it & 1) * n;
+        float *cn = a.contrib + (size_t)((it + 1) & 1) * n;
+        if (gtid == 0 && it < kTraceRows) {
+            DevTrace t;
+            t.round = it;
+            t.variant = SG_SYNC_PULL_TD;
+            t.in_nodes = n;
+            t.pad = 0;
+            t.in_edges = g.m;
+            t.ticks = (unsigned long long)(wall_clock64() - t_start);
+            a.trace[it] = t;
+        }
+        double r_acc = 0.0, d_acc = 0.0;
+        auto finish = [&](uint32_t v, double s) {
+            const double xn = base + d * s;
+            r_acc += fabs(xn - (double)a.x[v]);
+            a.x[v] = (float)xn;
+            const uint32_t dout = g.row[v + 1] - g.row[v];
+            if (dout) cn[v] = (float)(xn / dout);
+            else d_acc += xn;
+        };
+        // (A1) chunk sums of the in-hubs
+        for (uint32_t c = gwave; c < g.nhugein; c += nwaves) {
+            const uint2 dsc = g.hugein[c];
+            const uint32_t end = min(g.cstart[dsc.x + 1], dsc.y + kHugeChunk);
+            double s = 0.0;
+            for (uint32_t e0 = dsc.y; e0 < end; e0 += kWave) {
+                const uint32_t e = e0 + ln;
+                if (e < end) s += (double)cc[g.csrc[e]];
+            }
+            s = wave_sum_f64(s);
+            if (ln == 0) a.hubpart[c] = s;
+        }
+        // (A2) every other vertex, a 64-vertex word per wave at a time
+        for (uint32_t w = gwave; w < W; w += nwaves) {
+            const uint32_t v = w * 64 + ln;
+            uint32_t b = 0, e = 0;
+            bool own = false;
+            if (v < n) {
+                b = g.cstart[v];
+                e = g.cstart[v + 1];
+                own = e - b <= kPullHubDegree;
+                if (!own) b = e;
+            }
+            const double s = wave_gather_sum(b, e, [&](uint32_t ed) { return (double)cc[g.csrc[ed]]; });
+            if (own) finish(v, s);
+        }
+        ok = grid_sync(a.bar, xcc, gen, a.abort);
+        if (!ok) break;
+        // (B) in-hubs: chunk sums in chunk order, one wave per hub
+        for (uint32_t h = gwave; h < g.nhubin; h += nwaves) {
+            const uint2 hb = g.hubin[h];
+            double s = 0.0;
+            for (uint32_t i = ln; i < hb.y; i += kWave) s += a.hubpart[hb.x + i];
+            s = wave_sum_f64(s);
+            if (ln == 0) finish(g.hugein[hb.x].x, s);
+        }
+        {
+            const double2 t = block_sum2<WPB>(r_acc, d_acc, s_red);
+            if (threadIdx.x == 0) a.part[(size_t)((it + 1) & 1) * kMaxBlocks + blockIdx.x] = t;
+        }
+        it++;
+        ok = grid_sync(a.bar, xcc, gen, a.abort);
+    }
+
+    if (gtid == 0) {
+        DevStats s{};
+        s.rounds = it;
+        s.pull_rounds = it;
+        s.iterations = it;
+        s.residual = res;
+        s.status = ok ? SG_OK : SG_ERR_TIMEOUT;
+        *a.stats = s;
+        if (it < kTraceRows) {
+            DevTrace t{};
+            t.round = it;
+            t.variant = -1;
+            t.ticks = (unsigned long long)(wall_clock64() - t_start);
+            a.trace[it] = t;
+        }
+    }
+}
+
+// ------------------------------------------------------- residual DD kernel
+
+struct PrDdArgs {
+    DevGraph g;
+    float *x;          // normalised ranks [n] (output)
+    double *y;         // unnormalised ranks [n]
+    double *r;         // residuals [n]
+    double *p;         // d r / outdeg of this round's active vertices [n]
+    uint64_t *abit;    // 2 x W: active vertices of rounds k (k % 2)
+    uint32_t *queue;   // 2 x qcap: vertices whose residual crossed theta
+    uint32_t qcap;
+    FrontierCtr *ctr;  // [0,1]: activation totals; [2,3,4]: queue appends
+    double2 *part;     // kMaxBlocks partials for the final sums
+    GridBarrier bar;
+    uint32_t *abort;
+    DevStats *stats;
+    DevTrace *trace;
+    double damping, theta, alpha;
+    double total_wl;
+    int32_t force;     // -1 auto, 0 push, 1 pull
+    uint32_t max_rounds;
+    uint32_t probe;
+};
+
+struct DdRound {
+    const PrDdArgs *a;
+    double theta;
+    Stager S;
+    unsigned long long count;
+};
+
+// Add `add` to r[v] (atomically when `atomic`), queueing v if this addition
+// took its residual across theta. Wave-convergent.
+__device__ __forceinline__ void dd_accumulate(DdRound &R, bool act, uint32_t v, double add, bool atomic)
+{
+    bool crossed = false;
+    if (act && add > 0.0) {
+        double old;
+        if (atomic) {
+            old = atomicAdd(&R.a->r[v], add);
+        } else {
+            old = R.a->r[v];
+            R.a->r[v] = old + add;
+        }
+        crossed = old < R.theta && old + add >= R.theta;
+        R.count += crossed;
+    }
+    stage(R.S, crossed, v);
+}
+
+// Push the active vertices of one word (one lane per vertex): wave-level
+// prefix-sum scheduling of their out-edges; hubs go through the chunk list.
+__device__ __forceinline__ void dd_push_word(DdRound &R, uint64_t word, uint32_t w)
+{
+    const PrDdArgs &a = *R.a;
+    const DevGraph &g = a.g;
+    const int ln = lane();
+    const uint32_t u = w * 64 + ln;
+    uint32_t b = 0, dg = 0;
+    double pu = 0.0;
+    if ((word >> ln) & 1) {
+        b = g.row[u];
+        dg = g.row[u + 1] - b;
+        if (dg > kHugeDegree) dg = 0;
+        if (dg) pu = a.p[u];
+    }
+    const uint32_t incl = wave_incl_scan(dg);
+    const uint32_t total = shfl(incl, kWave - 1);
+    const uint32_t off = b - (incl - dg);
+    for (uint32_t jb = 0; jb < total; jb += kWave) {
+        const uint32_t j = jb + ln;
+        const bool act = j < total;
+        const int k = owner_lane(incl, j);
+        const uint32_t e = shfl(off, k) + j;
+        const double pk = __shfl(pu, k, kWave);
+        const uint32_t v = act ? g.col[e] : 0u;
+        dd_accumulate(R, act, v, pk, true);
+    }
+}
+
+template <int BS>
+__global__ __launch_bounds__(BS) void k_pr_dd(PrDdArgs a)
+{
+    if (a.probe) {
+        residency_probe(a.bar.words, 1u << 20);
+        return;
+    }
+    constexpr int WPB = BS / kWave;
+    constexpr uint32_t kLdsQ = 4u * BS;
+    const DevGraph &g = a.g;
+    const uint32_t n = g.n, W = g.words;
+    const uint32_t gtid = blockIdx.x * BS + threadIdx.x;
+    const uint32_t nthreads = gridDim.x * BS;
+    const uint32_t gwave = blockIdx.x * WPB + threadIdx.x / kWave;
+    const uint32_t nwaves = gridDim.x * WPB;
+    const int ln = lane();
+    const uint32_t xcc = xcc_id();
+    uint32_t gen = threadIdx.x == 0 ? grid_gen(a.bar) : 0u;
+    const long long t_start = wall_clock64();
+    const double d = a.damping, theta = a.theta;
+    FrontierCtr *act_ctr = a.ctr;       // 2 slots
+    FrontierCtr *q_ctr = a.ctr + 2;     // 3 slots
+
+    __shared__ double2 s_red2[WPB];
+    __shared__ double2 s_glob;
+    __shared__ unsigned long long s_red[2][WPB];
+    __shared__ unsigned long long s_lvl[5];
+    __shared__ uint32_t s_q[kLdsQ];
+    __shared__ uint32_t s_qn, s_flush[2];
+
+    const double r0 = (1.0 - d) / (double)n;
+    for (uint32_t v = gtid; v < n; v += nthreads) {
+        a.y[v] = 0.0;
+        a.r[v] = r0;
+    }
+    for (uint32_t w = gtid; w < 2 * W; w += nthreads) a.abit[w] = 0;
+    if (blockIdx.x == 0) {
+        for (int s = 0; s < 5; s++) clear_ctr(&a.ctr[s]);
+    }
+    bool ok = grid_sync(a.bar, xcc, gen, a.abort);
+
+    uint32_t k = 0, pushes = 0, pulls = 0;
+    unsigned long long ptrace = 0;
+    while (ok) {
+        if (a.max_rounds && k >= a.max_rounds) break;
+        // ---- activation: move residual >= theta into y, set p and the bit
+        read_counts(&q_ctr[k % 3], s_lvl);
+        __syncthreads();
+        const RoundCounts qc = unpack_counts(s_lvl, a.qcap);
+        __syncthreads();
+        uint64_t *ab = a.abit + (size_t)(k & 1) * W;
+        unsigned long long scout = 0, cnt = 0;
+        auto activate = [&](uint32_t u, double ru) {
+            a.y[u] += ru;
+            const uint32_t dout = g.row[u + 1] - g.row[u];
+            if (dout) a.p[u] = d * ru / dout;
+            scout += dout;
+            cnt++;
+        };
+        if (k == 0 || qc.overflow) {
+            for (uint32_t w = gwave; w < W; w += nwaves) {
+                const uint32_t v = w * 64 + ln;
+                bool act = false;
+                if (v < n) {
+                    const double rv = a.r[v];
+                    act = rv >= theta;
+                    if (act) {
+                        a.r[v] = 0.0;
+                        activate(v, rv);
+                    }
+                }
+                const uint64_t m = ballot(act);
+                if (ln == 0) ab[w] = m;
+            }
+        } else {
+            const uint32_t *qin = a.queue + (size_t)(k & 1) * a.qcap;
+            for (uint32_t i = gtid; i < qc.qlen; i += nthreads) {
+                const uint32_t u = qin[i];
+                // claim the residual: a vertex queued twice is activated once
+                const double ru = atomicExch(&a.r[u], 0.0);
+                if (ru >= theta) {
+                    atomicOr((unsigned long long *)&ab[u >> 6], 1ull << (u & 63));
+                    activate(u, ru);
+                } else if (ru != 0.0) {
+                    atomicAdd(&a.r[u], ru);
+                }
+            }
+        }
+        publish_counts<WPB>(&act_ctr[k & 1], scout, cnt, false, s_red);
+        ok = grid_sync(a.bar, xcc, gen, a.abort);
+        if (!ok) break;
+
+        // ---- propagation
+        read_counts(&act_ctr[k & 1], s_lvl);
+        if (threadIdx.x == 0) s_qn = 0;
+        __syncthreads();
+        const RoundCounts ac = unpack_counts(s_lvl, a.qcap);
+        __syncthreads();
+        if (ac.count == 0) break;  // every residual is below theta
+        bool pull;
+        if (a.force >= 0) pull = a.force == 1;
+        else pull = (double)ac.scout > a.total_wl / a.alpha;
+        if (pull) {
+            pulls++;
+            if (k < 64) ptrace |= 1ull << k;
+        } else {
+            pushes++;
+        }
+        if (gtid == 0) write_trace(a.trace, k, pull, ac.count, ac.scout, t_start);
+        if (blockIdx.x == 0) {
+            clear_ctr(&act_ctr[(k + 1) & 1]);
+            clear_ctr(&q_ctr[(k + 2) % 3]);
+        }
+        uint64_t *ab_next = a.abit + (size_t)((k + 1) & 1) * W;
+        for (uint32_t w = gtid; w < W; w += nthreads) ab_next[w] = 0;
+
+        DdRound R;
+        R.a = &a;
+        R.theta = theta;
+        R.S = Stager{a.queue + (size_t)((k + 1) & 1) * a.qcap, a.qcap, &q_ctr[(k + 1) % 3], s_q, &s_qn, kLdsQ};
+        R.count = 0;
+        if (!pull) {
+            for (uint32_t c0 = 0; c0 < g.nhuge; c0 += nwaves * kWave) {
+                const uint32_t c = c0 + (uint32_t)ln * nwaves + gwave;
+                uint2 dsc = make_uint2(0u, 0u);
+                bool mem = false;
+                if (c < g.nhuge) {
+                    dsc = g.huge[c];
+                    mem = test_bit(ab, dsc.x);
+                }
+                uint64_t mm = ballot(mem);
+                while (mm) {
+                    const int t = first_lane(mm);
+                    mm &= mm - 1;
+                    const uint32_t u = shfl(dsc.x, t), beg = shfl(dsc.y, t);
+                    const uint32_t end = min(g.row[u + 1], beg + kHugeChunk);
+                    const double pu = a.p[u];
+                    for (uint32_t e0 = beg; e0 < end; e0 += kWave) {
+                        const uint32_t e = e0 + ln;
+                        const bool act = e < end;
+                        dd_accumulate(R, act, act ? g.col[e] : 0u, pu, true);
+                    }
+                }
+            }
+            for (uint32_t w = gwave; w < W; w += nwaves) {
+                const uint64_t word = ab[w];
+                if (word) dd_push_word(R, word, w);
+            }
+        } else {
+            for (uint32_t c = gwave; c < g.nhugein; c += nwaves) {
+                const uint2 dsc = g.hugein[c];
+                const uint32_t end = min(g.cstart[dsc.x + 1], dsc.y + kHugeChunk);
+                double s = 0.0;
+                for (uint32_t e0 = dsc.y; e0 < end; e0 += kWave) {
+                    const uint32_t e = e0 + ln;
+                    if (e < end) {
+                        const uint32_t u = g.csrc[e];
+                        if (test_bit(ab, u)) s += a.p[u];
+                    }
+                }
+                s = wave_sum_f64(s);
+                dd_accumulate(R, ln == 0, dsc.x, s, true);
+            }
+            for (uint32_t w = gwave; w < W; w += nwaves) {
+                const uint32_t v = w * 64 + ln;
+                uint32_t b = 0, e = 0;
+                bool own = false;
+                if (v < n) {
+                    b = g.cstart[v];
+                    e = g.cstart[v + 1];
+                    own = e - b <= kPullHubDegree;
+                    if (!own) b = e;
+                }
+                const double s = wave_gather_sum(b, e, [&](uint32_t ed) {
+                    const uint32_t u = g.csrc[ed];
+                    return test_bit(ab, u) ? a.p[u] : 0.0;
+                });
+                dd_accumulate(R, own, v, s, false);
+            }
+        }
+        flush_stage(R.S, s_flush);
+        publish_counts<WPB>(&q_ctr[(k + 1) % 3], 0ull, R.count, false, s_red);
+        k++;
+        ok = grid_sync(a.bar, xcc, gen, a.abort);
+    }
+
+    // ---- absorb what residual is left, then normalise
+    double ysum = 0.0, rsum = 0.0;
+    if (ok) {
+        for (uint32_t v = gtid; v < n; v += nthreads) {
+            const double rv = a.r[v];
+            const double yv = a.y[v] + rv;
+            a.y[v] = yv;
+            ysum += yv;
+            rsum += rv;
+        }
+        const double2 t = block_sum2<WPB>(ysum, rsum, s_red2);
+        if (threadIdx.x == 0) a.part[blockIdx.x] = t;
+        ok = grid_sync(a.bar, xcc, gen, a.abort);
+    }
+    double2 tot = make_double2(1.0, 0.0);
+    if (ok) {
+        tot = grid_sum2<BS>(a.part, gridDim.x, s_red2, &s_glob);
+        const double inv = tot.x > 0.0 ? 1.0 / tot.x : 0.0;
+        for (uint32_t v = gtid; v < n; v += nthreads) a.x[v] = (float)(a.y[v] * inv);
+    }
+    if (gtid == 0) {
+        DevStats s{};
+        s.rounds = k;
+        s.push_rounds = pushes;
+        s.pull_rounds = pulls;
+        s.trace = ptrace;
+        s.iterations = k;
+        s.residual = tot.y;  // residual mass absorbed at the end (unnormalised)
+        s.status = ok ? SG_OK : SG_ERR_TIMEOUT;
+        *a.stats = s;
+        if (k < kTraceRows) {
+            DevTrace t{};
+            t.round = k;
+            t.variant = -1;
+            t.ticks = (unsigned long long)(wall_clock64() - t_start);
+            a.trace[k] = t;
+        }
+    }
+}
+
+template <typename Args, void (*K)(Args)>
+int launch_persistent(sg_engine *e, Args &args, int bs)
+{
+    sg_engine::PGrid *pg = nullptr;
+    auto probe = [e, args, bs](int blocks, uint32_t *words) -> int {
+        Args p = args;
+        p.probe = 1;
+        p.bar.words = words;
+        void *params[] = {&p};
+        SG_HIP(hipLaunchKernel((const void *)K, dim3(blocks), dim3(bs), params, 0, e->stream));
+        return SG_OK;
+    };
+    int rc = engine_persistent_grid(e, (const void *)K, bs, probe, &pg);
+    if (rc) return rc;
+    if (pg->blocks > (int)kMaxBlocks) SG_FAIL(SG_ERR_STATE, "sg_pagerank: grid exceeds the partial-sum area");
+    args.bar.words = pg->words;
+    args.bar.nxcd = pg->nxcd;
+    for (int x = 0; x < kMaxXcd; x++) args.bar.cnt[x] = pg->cnt[x];
+    SG_HIP(hipEventRecord(e->ev0, e->stream));
+    void *params[] = {&args};
+    SG_HIP(hipLaunchKernel((const void *)K, dim3(pg->blocks), dim3(bs), params, 0, e->stream));
+    SG_HIP(hipEventRecord(e->ev1, e->stream));
+    return SG_OK;
+}
+
+}  // namespace
+
+extern "C" int sg_pagerank(sg_engine *e, double damping, double tol, int32_t max_iter, int32_t mode)
+{
+    if (!e) SG_FAIL(SG_ERR_INVALID, "sg_pagerank: null engine");
+    if (!(damping >= 0.0 && damping < 1.0)) SG_FAIL(SG_ERR_RANGE, "sg_pagerank: damping must be in [0, 1)");
+    if (!(tol > 0.0)) SG_FAIL(SG_ERR_RANGE, "sg_pagerank: tol must be > 0");
+    if (max_iter < 0) SG_FAIL(SG_ERR_RANGE, "sg_pagerank: max_iter < 0");
+    const sg_engine_options &o = e->opt;
+    if (mode == SG_PR_AUTO) {
+        const int fv = o.force_variant;
+        mode = (fv == SG_ASYNC_PUSH_DD || fv == SG_ASYNC_PULL_DD || fv == SG_SYNC_PUSH_DD || fv == SG_SYNC_PULL_DD)
+                   ? SG_PR_PUSH_DD
+                   : SG_PR_PULL_TD;
+    }
+    if (mode != SG_PR_PULL_TD && mode != SG_PR_PUSH_DD) SG_FAIL(SG_ERR_INVALID, "sg_pagerank: unknown mode");
+    const DevGraph &d = e->g->dev;
+    int rc = engine_ensure_bfs_state(e);
+    if (rc) return rc;
+    const uint64_t n = d.n;
+    const int bs = o.block_size ? o.block_size : 1024;
+    if (bs != 256 && bs != 512 && bs != 1024) SG_FAIL(SG_ERR_UNSUPPORTED, "sg_pagerank: block_size must be 256, 512 or 1024");
+    e->last_algo = 2;
+    const uint64_t part_bytes = 2ull * kMaxBlocks * sizeof(double2);
+    if (mode == SG_PR_PULL_TD) {
+        // contrib (2n floats) | hubpart | partials
+        const uint64_t off_hub = (2 * n * 4 + 15) / 16 * 16;
+        const uint64_t off_part = off_hub + ((uint64_t)d.nhugein * 8 + 15) / 16 * 16;
+        if ((rc = engine_ensure_aux(e, off_part + part_bytes))) return rc;
+        char *base = (char *)e->aux;
+        PrArgs a{};
+        a.g = d;
+        a.x = (float *)e->value;
+        a.contrib = (float *)base;
+        a.hubpart = (double *)(base + off_hub);
+        a.part = (double2 *)(base + off_part);
+        a.bar.spin_limit = 1u << 22;
+        a.abort = e->abort_flag;
+        a.stats = e->stats;
+        a.trace = e->trace;
+        a.damping = damping;
+        a.tol = tol;
+        a.max_iter = max_iter ? (uint32_t)max_iter : 1000u;
+        switch (bs) {
+        case 256: rc = launch_persistent<PrArgs, k_pr_pull<256>>(e, a, 256); break;
+        case 512: rc = launch_persistent<PrArgs, k_pr_pull<512>>(e, a, 512); break;
+        default: rc = launch_persistent<PrArgs, k_pr_pull<1024>>(e, a, 1024); break;
+        }
+    } else {
+        const int force = forced_direction(o);
+        if (force < -1) SG_FAIL(SG_ERR_UNSUPPORTED, "sg_pagerank: data-driven mode takes push/pull DD variants only");
+        // y | r | p (n doubles each) | partials
+        const uint64_t off_part = 3 * n * 8;
+        if ((rc = engine_ensure_aux(e, off_part + part_bytes))) return rc;
+        char *base = (char *)e->aux;
+        PrDdArgs a{};
+        a.g = d;
+        a.x = (float *)e->value;
+        a.y = (double *)base;
+        a.r = (double *)(base + n * 8);
+        a.p = (double *)(base + 2 * n * 8);
+        a.part = (double2 *)(base + off_part);
+        a.abit = e->bitmaps;
+        a.queue = e->queues;
+        a.qcap = e->qcap;
+        a.ctr = (FrontierCtr *)e->ctr;
+        a.bar.spin_limit = 1u << 22;
+        a.abort = e->abort_flag;
+        a.stats = e->stats;
+        a.trace = e->trace;
+        a.damping = damping;
+        a.theta = tol * (1.0 - damping) / (double)n;
+        a.alpha = o.alpha == 0.0 ? 1.0 : o.alpha;
+        a.total_wl = (double)d.m * o.edge_factor;
+        a.force = force;
+        a.max_rounds = max_iter ? (uint32_t)max_iter : 100000u;
+        switch (bs) {
+        case 256: rc = launch_persistent<PrDdArgs, k_pr_dd<256>>(e, a, 256); break;
+        case 512: rc = launch_persistent<PrDdArgs, k_pr_dd<512>>(e, a, 512); break;
+        default: rc = launch_persistent<PrDdArgs, k_pr_dd<1024>>(e, a, 1024); break;
+        }
+    }
+    if (rc) return rc;
+    e->timed = true;
+    return SG_OK;
+}

@@ -51,6 +51,10 @@ struct DevGraph {
     // in-edges (SSSP), so no wave walks a long in-edge list alone.
     const uint2 *hugein;
     uint32_t nhugein;
+    // One entry per such in-hub: (index of its first descriptor in hugein,
+    // number of descriptors); its chunks are contiguous there.
+    const uint2 *hubin;
+    uint32_t nhubin;
 };
 
 // A vertex whose out-degree exceeds this is expanded through the static chunk
@@ -67,7 +71,7 @@ struct sg_graph {
     bool undirected = false;
     bool weighted = false;
     uint64_t device_bytes = 0;
-    void *allocs[10] = {};
+    void *allocs[16] = {};
     int nallocs = 0;
     int device = 0;
     int num_cus = 0;
@@ -125,6 +129,8 @@ struct sg_engine {
     uint32_t *abort_flag = nullptr;
     sg::DevStats *stats = nullptr;
     sg::DevTrace *trace = nullptr;    // kTraceRows rows
+    void *aux = nullptr;              // per-algorithm working arrays (PageRank, BC), grown on demand
+    uint64_t aux_bytes = 0;
     uint32_t *exits = nullptr;        // per-block exit records (diagnostics), kMaxBlocks x 4
     uint32_t trace_rows = 0;          // rows written by the last run (host copy)
     double wall_khz = 100000.0;       // device wall clock rate
@@ -151,6 +157,9 @@ struct sg_engine {
 namespace sg {
 int engine_ensure_bfs_state(sg_engine *e);
 int engine_ensure_stamp(sg_engine *e);
+// Grow the engine's auxiliary arena to at least `bytes` (synchronises the
+// stream before replacing a smaller one; contents are not preserved).
+int engine_ensure_aux(sg_engine *e, uint64_t bytes);
 // Device allocation owned by the engine (freed with it); reports SG_ERR_NOMEM.
 int engine_alloc(void **p, uint64_t bytes);
 // Grid for a persistent kernel: the largest blocks-per-CU x CUs (starting

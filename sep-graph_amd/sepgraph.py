@@ -32,6 +32,7 @@ ERRORS = {
     -8: "SG_ERR_UNSUPPORTED",
 }
 CSR_SYMMETRIZE, CSR_DROP_SELF_LOOPS, CSR_DEDUP = 1, 2, 4
+PR_AUTO, PR_PULL_TD, PR_PUSH_DD = 0, 1, 2
 
 VARIANTS = {
     "auto": -1,
@@ -136,6 +137,7 @@ def lib() -> C.CDLL:
             "sg_engine_set_options": ([_P, C.POINTER(EngineOptions)], C.c_int),
             "sg_bfs": ([_P, C.c_uint32], C.c_int),
             "sg_sssp": ([_P, C.c_uint32], C.c_int),
+            "sg_pagerank": ([_P, C.c_double, C.c_double, C.c_int32, C.c_int32], C.c_int),
             "sg_engine_sync": ([_P], C.c_int),
             "sg_engine_value_ptr": ([_P, C.POINTER(_P), C.POINTER(C.c_uint64)], C.c_int),
             "sg_engine_gather_value": ([_P, _P, C.c_uint64], C.c_int),
@@ -326,6 +328,10 @@ class Engine:
 
     def sssp(self, source: int):
         _check(lib().sg_sssp(self.h, int(source)))
+
+    def pagerank(self, damping: float = 0.85, tol: float = 1e-6, max_iter: int = 0, mode: int = 0):
+        """Normalised PageRank into the value array (FP32); mode PR_PULL_TD or PR_PUSH_DD."""
+        _check(lib().sg_pagerank(self.h, float(damping), float(tol), int(max_iter), int(mode)))
 
     def sync(self):
         _check(lib().sg_engine_sync(self.h))

@@ -1,0 +1,179 @@
+"""PageRank on the GPU through the C ABI against the FP64 oracle
+(power iteration with uniform teleport and uniform dangling redistribution),
+for both schedules: topology-driven pull (the reference's RunSyncPullTD,
+include/framework/algo_variants.cuh:753-823) and data-driven residual
+propagation (RunAsyncPushDD with the PageRank app, samples/hybrid_pr/hybrid_pr.cu:66-110).
+
+Tolerances: ranks are FP32, so the bar is relative L1 error <= 1e-6 against
+the FP64 oracle. The pull schedule stops on the same rule as the oracle
+(first iteration whose L1 change is < tol) and is compared with the oracle run
+for the same number of iterations. The data-driven schedule converges by a
+different route and is compared with an oracle converged to 1e-12."""
+import numpy as np
+import pytest
+
+import graphs as G
+
+pytestmark = pytest.mark.gpu
+D = 0.85
+
+
+def rel_l1(x, ref):
+    return float(np.abs(x.astype(np.float64) - ref).sum() / np.abs(ref).sum())
+
+
+def power_step(row, col, x, d=D):
+    """One FP64 power-iteration step F(x) (numpy, any size)."""
+    n = len(row) - 1
+    x = x.astype(np.float64)
+    deg = np.diff(row.astype(np.int64))
+    contrib = np.where(deg > 0, x / np.maximum(deg, 1), 0.0)
+    src = np.repeat(np.arange(n), deg)
+    y = np.bincount(col, weights=contrib[src], minlength=n)
+    dang = x[deg == 0].sum()
+    return (1 - d) / n + d * (y + dang / n)
+
+
+def run_pr(sg, hg, mode, tol=1e-6, max_iter=0, **opts):
+    g = sg.Graph(hg)
+    e = sg.Engine(g, **opts)
+    e.pagerank(D, tol, max_iter, mode)
+    x = e.gather(np.float32)
+    st = e.stats()
+    return x, st, e
+
+
+def check_td(orc, hg, x, st, tol=1e-6):
+    ref, it, res = orc.pagerank(hg.row, hg.col, D, tol, 100000)
+    # same stopping rule: the iteration counts agree up to a tie at the threshold
+    assert abs(st["iterations"] - it) <= 1, (st["iterations"], it, res)
+    same, _, _ = orc.pagerank(hg.row, hg.col, D, 0.0, st["iterations"])
+    assert rel_l1(x, same) <= 1e-6
+    if st["iterations"] == it:
+        assert rel_l1(x, ref) <= 1e-6
+    assert st["residual"] < tol
+    assert abs(float(x.astype(np.float64).sum()) - 1.0) < 1e-5
+
+
+SMALL = {
+    "path": lambda: G.path(40),
+    "cycle_dir": lambda: G.cycle(50, undirected=False),
+    "star": lambda: G.star(30),
+    "star_out": lambda: G.star(30, undirected=False),  # leaves are dangling
+    "complete": lambda: G.complete(12),
+    "dir_random": lambda: G.random_graph(600, 4000, 3),
+    "sparse_dir": lambda: G.random_graph(3000, 2500, 5),  # many dangling and isolated vertices
+    "und_random": lambda: G.random_graph(500, 1500, 4, undirected=True),
+    "grid": lambda: G.grid(19, 21),
+}
+
+
+@pytest.mark.parametrize("name", sorted(SMALL))
+def test_pr_pull_td_small(sg, orc, gpu, name):
+    row, col = SMALL[name]()
+    hg = sg.HostGraph(len(row) - 1, row, col, None, undirected=False)
+    x, st, _ = run_pr(sg, hg, sg.PR_PULL_TD)
+    check_td(orc, hg, x, st)
+
+
+@pytest.mark.parametrize("name", sorted(SMALL))
+@pytest.mark.parametrize("variant", ["auto", "async_push_dd", "async_pull_dd"])
+def test_pr_dd_small(sg, orc, gpu, name, variant):
+    row, col = SMALL[name]()
+    hg = sg.HostGraph(len(row) - 1, row, col, None, undirected=False)
+    x, st, _ = run_pr(sg, hg, sg.PR_PUSH_DD, tol=1e-9, force_variant=variant)
+    ref, _, _ = orc.pagerank(hg.row, hg.col, D, 1e-13, 100000)
+    assert rel_l1(x, ref) <= 1e-6, (name, variant, st)
+    if variant == "async_push_dd":
+        assert st["pull_rounds"] == 0
+    if variant == "async_pull_dd":
+        assert st["push_rounds"] == 0
+
+
+def test_pr_closed_forms(sg, gpu):
+    # directed cycle and complete graph: uniform 1/n
+    for row, col in (G.cycle(64, undirected=False), G.complete(9)):
+        n = len(row) - 1
+        hg = sg.HostGraph(n, row, col, None)
+        for mode in (sg.PR_PULL_TD, sg.PR_PUSH_DD):
+            x, _, _ = run_pr(sg, hg, mode, tol=1e-9)
+            np.testing.assert_allclose(x, 1.0 / n, rtol=1e-6)
+    # star K_{1,k}: y = (1-d)/n + d k x, x = (1-d)/n + d y / k
+    k = 20
+    row, col = G.star(k)
+    n = k + 1
+    A = np.array([[1.0, -D * k], [-D / k, 1.0]])
+    yc, xl = np.linalg.solve(A, [(1 - D) / n, (1 - D) / n])
+    hg = sg.HostGraph(n, row, col, None)
+    for mode in (sg.PR_PULL_TD, sg.PR_PUSH_DD):
+        x, _, _ = run_pr(sg, hg, mode, tol=1e-10)
+        np.testing.assert_allclose(x[0], yc, rtol=1e-6)
+        np.testing.assert_allclose(x[1:], xl, rtol=1e-6)
+    # edgeless graph: every vertex dangling, ranks 1/n
+    n = 1000
+    hg = sg.HostGraph(n, np.zeros(n + 1, np.uint32), np.zeros(0, np.uint32), None)
+    for mode in (sg.PR_PULL_TD, sg.PR_PUSH_DD):
+        x, _, _ = run_pr(sg, hg, mode)
+        np.testing.assert_allclose(x, 1.0 / n, rtol=1e-6)
+
+
+def test_pr_max_iter_and_errors(sg, orc, gpu):
+    row, col = G.random_graph(2000, 12000, 9)
+    hg = sg.HostGraph(len(row) - 1, row, col, None)
+    x, st, e = run_pr(sg, hg, sg.PR_PULL_TD, tol=1e-12, max_iter=7)
+    assert st["iterations"] == 7
+    ref, _, _ = orc.pagerank(row, col, D, 0.0, 7)
+    assert rel_l1(x, ref) <= 1e-6
+    for bad in ((1.0, 1e-6), (-0.1, 1e-6), (0.85, 0.0)):
+        with pytest.raises(sg.SepGraphError):
+            e.pagerank(bad[0], bad[1], 0, sg.PR_PULL_TD)
+    with pytest.raises(sg.SepGraphError):
+        e.pagerank(D, 1e-6, 0, 7)
+
+
+@pytest.mark.parametrize("bs", [256, 512, 1024])
+def test_pr_rmat16_block_sizes(sg, orc, gpu, bs):
+    hg = sg.rmat_graph(16, 16, symmetrize=False, seed=1)
+    x, st, e = run_pr(sg, hg, sg.PR_PULL_TD, block_size=bs)
+    check_td(orc, hg, x, st)
+    # repeated runs give identical bits
+    e.pagerank(D, 1e-6, 0, sg.PR_PULL_TD)
+    assert np.array_equal(e.gather(np.float32), x)
+    e.pagerank(D, 1e-8, 0, sg.PR_PUSH_DD)
+    xd = e.gather(np.float32)
+    ref, _, _ = orc.pagerank(hg.row, hg.col, D, 1e-13, 100000)
+    assert rel_l1(xd, ref) <= 1e-6
+
+
+def test_pr_rmat20_directed(sg, orc, gpu):
+    hg = sg.rmat_graph(20, 16, symmetrize=False, seed=1)
+    x, st, e = run_pr(sg, hg, sg.PR_PULL_TD)
+    check_td(orc, hg, x, st)
+    e.pagerank(D, 1e-8, 0, sg.PR_PUSH_DD)
+    xd = e.gather(np.float32)
+    sd = e.stats()
+    ref, _, _ = orc.pagerank(hg.row, hg.col, D, 1e-13, 100000)
+    assert rel_l1(xd, ref) <= 1e-6, sd
+    assert sd["push_rounds"] + sd["pull_rounds"] == sd["rounds"]
+
+
+def test_pr_rmat23_directed_full_size(sg, gpu):
+    """PageRank on directed RMAT-23 (n = 8,388,608): both schedules. At this
+    size the check is the fixed-point equation itself: one FP64 power step
+    from the GPU's ranks moves them by less than the stopping tolerance
+    (plus FP32 storage rounding), ranks sum to 1 and are >= (1-d)/n, and the
+    two schedules agree."""
+    hg = sg.rmat_graph(23, 16, symmetrize=False, seed=1)
+    n = hg.n
+    x, st, e = run_pr(sg, hg, sg.PR_PULL_TD)
+    assert st["residual"] < 1e-6 and 5 <= st["iterations"] <= 200, st
+    xd64 = x.astype(np.float64)
+    assert abs(xd64.sum() - 1.0) < 1e-5
+    assert xd64.min() >= (1 - D) / n * (1 - 1e-6)
+    step = power_step(hg.row, hg.col, x)
+    assert np.abs(step - xd64).sum() < 1e-6 + 2 ** -23
+    e.pagerank(D, 1e-8, 0, sg.PR_PUSH_DD)
+    xd = e.gather(np.float32)
+    assert np.abs(power_step(hg.row, hg.col, xd) - xd.astype(np.float64)).sum() < 1e-6
+    # the pull result is within its stopping error of the fixed point
+    assert rel_l1(x, xd.astype(np.float64)) < 1e-5
