@@ -225,6 +225,90 @@ enum { SG_PR_AUTO = 0, SG_PR_PULL_TD = 1, SG_PR_PUSH_DD = 2 };
  * teleport its solution normalised to sum 1 is this ranking vector. */
 int sg_pagerank(sg_engine *e, double damping, double tol, int32_t max_iter, int32_t mode);
 
+/* Betweenness centrality from k sources (host array), `batch` sources per
+ * launch (1..64; 0 means 64): value[v] = 0.5 * sum over sources s of the
+ * Brandes dependency delta_s(v), FP32 (accumulated in FP64; path counts
+ * sigma in FP64). With k = 1 this is the reference's single-source BC
+ * (samples/hybrid_bc/hybrid_bc.cu:303-376, BetweennessCentralityHost,
+ * hybrid_bc_host.cpp:16-92). Graphs may be directed (forward over
+ * out-edges, dependencies over successors). Asynchronous; sg_stats.rounds
+ * counts forward levels over all batches, .max_depth the deepest level,
+ * .iterations the batches. */
+int sg_bc(sg_engine *e, const uint32_t *sources, int32_t k, int32_t batch);
+
+/* Single-source BC dependencies for k sources, `batch` per launch: the value
+ * array becomes k rows of n FP32, row i = 0.5 * delta_{sources[i]}(v), i.e.
+ * the reference's single-source BC result for that source
+ * (samples/hybrid_bc/hybrid_bc.cu:303-376). Serves BC-dependency queries. */
+int sg_bc_dependency(sg_engine *e, const uint32_t *sources, int32_t k, int32_t batch);
+
+/* BFS from k sources, `batch` (1..64, 0 = 64) per launch sharing one sweep
+ * of the adjacency per level (bit-packed frontiers, bit s = source s). The
+ * value array becomes k rows of n int32 levels, row i equal to sg_bfs from
+ * sources[i] (-1 when unreached). No reference counterpart: the reference
+ * runs one source per Engine::Start (framework.cuh:315-375); this serves
+ * batched BFS queries. */
+int sg_bfs_multi(sg_engine *e, const uint32_t *sources, int32_t k, int32_t batch);
+
+/* SSSP from k sources, `batch` (1..64, 0 = 64) per launch: distance rows
+ * [vertex][64] relaxed for every query of the batch at once (data-driven
+ * push rounds). The value array becomes k rows of n uint32 distances, row i
+ * equal to sg_sssp from sources[i] (UINT32_MAX when unreached). No reference
+ * counterpart (one source per run there); serves batched SSSP queries. */
+int sg_sssp_multi(sg_engine *e, const uint32_t *sources, int32_t k, int32_t batch);
+
+/* ---------------------------------------------------------------- serving
+ * No reference counterpart: the reference answers one query per process
+ * (samples/hybrid_bfs/main.cpp and its siblings). A server keeps a graph resident and answers
+ * BFS / SSSP / BC-dependency queries with continuous batching: a worker
+ * forms each batch (up to 64 sources of one kind, the kind whose oldest
+ * query has waited longest) when the GPU frees up, from whatever is queued.
+ * Admission is by device memory: each query holds one result row of n
+ * 32-bit values (int32 levels, uint32 distances, FP32 0.5*delta) from the
+ * start of its batch until released; rows = memory_budget / (4 n). Results
+ * equal those of sg_bfs / sg_sssp / sg_bc_dependency for the same source.
+ * Thread-safe: any thread may submit, wait, read and release. */
+enum { SG_QUERY_BFS = 0, SG_QUERY_SSSP = 1, SG_QUERY_BC_DEP = 2, SG_QUERY_KINDS = 3 };
+
+typedef struct sg_server sg_server;
+
+typedef struct sg_server_config {
+    uint64_t memory_budget;  /* device bytes for result rows */
+    int32_t max_rows;        /* cap on result rows (0: the budget alone) */
+    int32_t max_batch[3];    /* per kind: sources per launch, 1..64 (0: 64) */
+    int32_t reserved[8];
+} sg_server_config;
+
+typedef struct sg_server_stats {
+    uint64_t submitted, completed, failed, batches, queued;
+    uint64_t device_bytes;   /* graph + working arenas + result rows */
+    double mean_batch;       /* queries per launch */
+    double busy_ms;          /* wall time with a batch on the GPU */
+    uint32_t rows, free_rows;
+    uint32_t reserved[8];
+} sg_server_stats;
+
+void sg_server_config_default(sg_server_config *c);
+/* opt may be NULL (defaults); the server owns an engine on its own stream. */
+int sg_server_create(sg_graph *g, const sg_server_config *cfg, const sg_engine_options *opt, sg_server **out);
+int sg_server_destroy(sg_server *s);
+int sg_server_submit(sg_server *s, int32_t kind, uint32_t source, uint64_t *ticket);
+/* timeout_ms < 0 waits without limit; SG_ERR_TIMEOUT if not finished in time */
+int sg_server_wait(sg_server *s, uint64_t ticket, int32_t timeout_ms);
+/* device pointer to the finished query's row (valid until released), its
+ * size, time queued before its batch started and arrival-to-done time */
+int sg_server_result(sg_server *s, uint64_t ticket, void **dev_ptr, uint64_t *bytes, double *queue_ms,
+                     double *total_ms);
+int sg_server_copy_result(sg_server *s, uint64_t ticket, void *host_out, uint64_t bytes);
+int sg_server_release(sg_server *s, uint64_t ticket);
+int sg_server_get_stats(sg_server *s, sg_server_stats *out);
+/* Open-loop load: query i (kind, source) is submitted arrival_s[i] seconds
+ * after the call starts (non-decreasing), results are released on
+ * completion, and latency_ms[i] = completion - scheduled arrival. Returns
+ * when all have finished. */
+int sg_server_replay(sg_server *s, const int32_t *kinds, const uint32_t *sources, const double *arrival_s,
+                     int64_t count, double *latency_ms);
+
 /* Synchronise the engine's stream and report the device-side status. */
 int sg_engine_sync(sg_engine *e);
 /* Device pointer and size in bytes of the last run's per-vertex result. */

@@ -405,6 +405,7 @@ extern "C" int sg_bfs(sg_engine *e, uint32_t source)
     if (force < -1) SG_FAIL(SG_ERR_UNSUPPORTED, "sg_bfs: only data-driven push/pull variants apply to BFS");
     int rc = engine_ensure_bfs_state(e);
     if (rc) return rc;
+    if ((rc = engine_ensure_value(e, (uint64_t)d.n * 4))) return rc;
     BfsArgs a{};
     a.g = d;
     a.level = (int32_t *)e->value;

@@ -280,7 +280,7 @@ extern "C" int sg_engine_destroy(sg_engine *e)
     if (!e) return SG_OK;
     if (e->stream) (void)hipStreamSynchronize(e->stream);
     void *ptrs[] = {e->value, e->bitmaps, e->queues, e->stamp, e->ctr,
-                    e->abort_flag, e->stats, e->trace, e->exits, e->aux};
+                    e->abort_flag, e->stats, e->trace, e->exits, e->aux, e->srcbuf};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     for (int i = 0; i < e->ngrids; i++) (void)hipFree(e->grids[i].words);
@@ -409,6 +409,38 @@ int engine_ensure_stamp(sg_engine *e)
     return engine_alloc((void **)&e->stamp, (uint64_t)e->g->dev.n * 4);
 }
 
+int engine_ensure_srcbuf(sg_engine *e, uint64_t entries)
+{
+    if (e->srcbuf && e->srcbuf_cap >= entries) return SG_OK;
+    if (e->srcbuf) {
+        SG_HIP(hipStreamSynchronize(e->stream));
+        (void)hipFree(e->srcbuf);
+        e->srcbuf = nullptr;
+        e->srcbuf_cap = 0;
+    }
+    int rc = engine_alloc((void **)&e->srcbuf, entries * 4);
+    if (rc) return rc;
+    e->srcbuf_cap = entries;
+    return SG_OK;
+}
+
+int engine_ensure_value(sg_engine *e, uint64_t bytes)
+{
+    if (!e->value || e->value_cap < bytes) {
+        if (e->value) {
+            SG_HIP(hipStreamSynchronize(e->stream));
+            (void)hipFree(e->value);
+            e->value = nullptr;
+            e->value_cap = 0;
+        }
+        int rc = engine_alloc(&e->value, bytes);
+        if (rc) return rc;
+        e->value_cap = bytes;
+    }
+    e->value_bytes = bytes;
+    return SG_OK;
+}
+
 int engine_ensure_aux(sg_engine *e, uint64_t bytes)
 {
     if (e->aux && e->aux_bytes >= bytes) return SG_OK;
@@ -438,8 +470,7 @@ int engine_ensure_bfs_state(sg_engine *e)
         return SG_OK;
     };
     int rc;
-    e->value_bytes = (uint64_t)d.n * 4;
-    if ((rc = alloc(&e->value, e->value_bytes))) return rc;
+    if ((rc = engine_ensure_value(e, (uint64_t)d.n * 4))) return rc;
     if ((rc = alloc((void **)&e->bitmaps, (uint64_t)d.words * 8 * 3))) return rc;
     // Frontier queues may hold a vertex more than once (non-atomic discovery
     // across XCDs); past qcap a round falls back to scanning the level array.

@@ -616,6 +616,7 @@ extern "C" int sg_pagerank(sg_engine *e, double damping, double tol, int32_t max
     const DevGraph &d = e->g->dev;
     int rc = engine_ensure_bfs_state(e);
     if (rc) return rc;
+    if ((rc = engine_ensure_value(e, (uint64_t)d.n * 4))) return rc;
     const uint64_t n = d.n;
     const int bs = o.block_size ? o.block_size : 1024;
     if (bs != 256 && bs != 512 && bs != 1024) SG_FAIL(SG_ERR_UNSUPPORTED, "sg_pagerank: block_size must be 256, 512 or 1024");

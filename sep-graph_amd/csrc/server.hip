@@ -1,0 +1,394 @@
+// Batched query serving over a resident graph (C ABI in include/sepgraph.h).
+//
+// The reference has no serving layer: each sample answers one query per
+// process (samples/hybrid_*/main.cpp). Here one server keeps a graph and an
+// engine resident and answers BFS, SSSP and BC-dependency queries:
+//   * admission by device memory: every query needs one result row of n
+//     32-bit values; the server owns a pool of rows sized by the caller's
+//     memory budget, and a query waits in its kind's FIFO until a row is free
+//     (rows come back when the client releases a result);
+//   * continuous batching: a worker thread forms the next batch the moment
+//     the GPU is free, from whatever is queued at that time -- the kind whose
+//     oldest request has waited longest, up to that kind's batch limit (<= 64)
+//     -- so requests that arrive while a batch runs join the next one;
+//   * one launch per batch: the batched persistent kernels (bit-packed
+//     multi-source BFS, multi-source SSSP, multi-source Brandes) write each
+//     query's result straight into its row of the pool.
+// Batches are serialised on the engine's stream: a persistent kernel assumes
+// all of its blocks are co-resident, which two concurrent launches would not
+// be. Host work (queueing, result hand-off) overlaps the running batch.
+#include "sg_internal.hpp"
+
+#include <algorithm>
+#include <chrono>
+#include <condition_variable>
+#include <cstring>
+#include <memory>
+#include <deque>
+#include <mutex>
+#include <thread>
+#include <unordered_map>
+#include <vector>
+
+using namespace sg;
+
+namespace {
+
+double now_s()
+{
+    return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+int kind_to_ms(int kind)
+{
+    return kind == SG_QUERY_BFS ? kMsBfs : kind == SG_QUERY_SSSP ? kMsSssp : kMsBcDep;
+}
+
+enum { kQueued = 0, kRunning = 1, kDone = 2, kFailed = 3 };
+
+}  // namespace
+
+struct sg_server {
+    sg_graph *g = nullptr;
+    sg_engine *e = nullptr;
+    sg_server_config cfg{};
+    uint64_t row_bytes = 0;
+    uint32_t nrows = 0;
+    void *pool = nullptr;
+    uint32_t *d_src = nullptr, *d_rows = nullptr;  // 64 entries each
+    uint32_t *h_src = nullptr, *h_rows = nullptr;  // pinned staging
+
+    struct Req {
+        int kind;
+        uint32_t source;
+        int32_t row;
+        int state;
+        int status;
+        bool auto_release;
+        double t_arrival, t_start, t_done;
+        double *latency_out;  // replay: where to store t_done - t_arrival (ms)
+    };
+    std::mutex mu;
+    std::condition_variable cv_work, cv_done;
+    std::unordered_map<uint64_t, Req> reqs;
+    std::deque<uint64_t> queue[SG_QUERY_KINDS];
+    std::vector<uint32_t> free_rows;
+    uint64_t next_ticket = 1;
+    bool stop = false;
+    std::thread worker;
+    // counters
+    uint64_t submitted = 0, completed = 0, failed = 0, batches = 0, batched_queries = 0;
+    double busy_s = 0.0;
+
+    bool has_work() const
+    {
+        if (free_rows.empty()) return false;
+        for (int k = 0; k < SG_QUERY_KINDS; k++)
+            if (!queue[k].empty()) return true;
+        return false;
+    }
+
+    void finish_locked(uint64_t t, Req &r, bool ok, double t_done)
+    {
+        r.state = ok ? kDone : kFailed;
+        r.status = ok ? SG_OK : SG_ERR_HIP;
+        r.t_done = t_done;
+        if (ok) completed++;
+        else failed++;
+        if (r.latency_out) *r.latency_out = ok ? (t_done - r.t_arrival) * 1e3 : -1.0;
+        if (r.auto_release) {
+            if (r.row >= 0) free_rows.push_back((uint32_t)r.row);
+            reqs.erase(t);
+        }
+    }
+
+    void run()
+    {
+        std::vector<uint64_t> tickets;
+        for (;;) {
+            std::unique_lock<std::mutex> lk(mu);
+            cv_work.wait(lk, [&] { return stop || has_work(); });
+            if (stop) return;
+            int kind = -1;
+            double oldest = 0.0;
+            for (int k = 0; k < SG_QUERY_KINDS; k++) {
+                if (queue[k].empty()) continue;
+                const double t = reqs[queue[k].front()].t_arrival;
+                if (kind < 0 || t < oldest) {
+                    kind = k;
+                    oldest = t;
+                }
+            }
+            const uint32_t limit = (uint32_t)std::max(1, std::min(64, cfg.max_batch[kind] ? cfg.max_batch[kind] : 64));
+            const uint32_t b = (uint32_t)std::min<size_t>({queue[kind].size(), free_rows.size(), (size_t)limit});
+            tickets.clear();
+            const double t0 = now_s();
+            for (uint32_t i = 0; i < b; i++) {
+                const uint64_t t = queue[kind].front();
+                queue[kind].pop_front();
+                Req &r = reqs[t];
+                r.row = (int32_t)free_rows.back();
+                free_rows.pop_back();
+                r.state = kRunning;
+                r.t_start = t0;
+                h_src[i] = r.source;
+                h_rows[i] = (uint32_t)r.row;
+                tickets.push_back(t);
+            }
+            lk.unlock();
+
+            bool ok = hipMemcpyAsync(d_src, h_src, b * 4, hipMemcpyHostToDevice, e->stream) == hipSuccess &&
+                      hipMemcpyAsync(d_rows, h_rows, b * 4, hipMemcpyHostToDevice, e->stream) == hipSuccess &&
+                      hipMemsetAsync(e->stats, 0, sizeof(DevStats), e->stream) == hipSuccess;
+            ok = ok && ms_launch(e, kind_to_ms(kind), d_src, b, pool, d_rows, true, true) == SG_OK;
+            ok = ok && sg_engine_sync(e) == SG_OK;
+            const double t1 = now_s();
+
+            lk.lock();
+            batches++;
+            batched_queries += b;
+            busy_s += t1 - t0;
+            for (uint64_t t : tickets) finish_locked(t, reqs[t], ok, t1);
+            lk.unlock();
+            cv_done.notify_all();
+            cv_work.notify_all();
+        }
+    }
+};
+
+extern "C" void sg_server_config_default(sg_server_config *c)
+{
+    if (!c) return;
+    std::memset(c, 0, sizeof(*c));
+    c->memory_budget = 4ull << 30;
+    c->max_rows = 4096;
+    for (int k = 0; k < SG_QUERY_KINDS; k++) c->max_batch[k] = 64;
+}
+
+extern "C" int sg_server_create(sg_graph *g, const sg_server_config *cfg, const sg_engine_options *opt,
+                                sg_server **out)
+{
+    if (!g || !out) SG_FAIL(SG_ERR_INVALID, "sg_server_create: null argument");
+    *out = nullptr;
+    sg_server_config c;
+    if (cfg) c = *cfg;
+    else sg_server_config_default(&c);
+    for (int k = 0; k < SG_QUERY_KINDS; k++)
+        if (c.max_batch[k] < 0 || c.max_batch[k] > 64) SG_FAIL(SG_ERR_RANGE, "sg_server_create: max_batch in [0, 64]");
+    const uint64_t n = g->dev.n;
+    const uint64_t row_bytes = n * 4;
+    uint64_t rows = c.memory_budget / row_bytes;
+    if (c.max_rows) rows = std::min<uint64_t>(rows, (uint64_t)c.max_rows);
+    if (rows < 1) SG_FAIL(SG_ERR_CAPACITY, "sg_server_create: memory budget below one result row");
+    std::unique_ptr<sg_server> s(new sg_server());
+    s->g = g;
+    s->cfg = c;
+    s->row_bytes = row_bytes;
+    s->nrows = (uint32_t)std::min<uint64_t>(rows, 0xFFFFFFFFull);
+    int rc = sg_engine_create(g, &s->e);
+    if (rc) return rc;
+    if (opt && (rc = sg_engine_set_options(s->e, opt))) {
+        sg_engine_destroy(s->e);
+        return rc;
+    }
+    auto fail = [&](int code) {
+        sg_server *p = s.release();
+        sg_server_destroy(p);
+        return code;
+    };
+    // working arenas of every kind (the largest wins) and the result pool
+    for (int k = 0; k < SG_QUERY_KINDS; k++)
+        if ((rc = ms_prepare(s->e, kind_to_ms(k)))) return fail(rc);
+    if ((rc = engine_alloc(&s->pool, (uint64_t)s->nrows * row_bytes))) return fail(rc);
+    if ((rc = engine_alloc((void **)&s->d_src, 256))) return fail(rc);
+    if ((rc = engine_alloc((void **)&s->d_rows, 256))) return fail(rc);
+    if (hipHostMalloc((void **)&s->h_src, 256, hipHostMallocDefault) != hipSuccess ||
+        hipHostMalloc((void **)&s->h_rows, 256, hipHostMallocDefault) != hipSuccess) {
+        (void)hipGetLastError();
+        sg::set_error("sg_server_create: pinned staging allocation failed");
+        return fail(SG_ERR_NOMEM);
+    }
+    s->free_rows.reserve(s->nrows);
+    for (uint32_t r = s->nrows; r-- > 0;) s->free_rows.push_back(r);
+    sg_server *p = s.release();
+    p->worker = std::thread([p] { p->run(); });
+    *out = p;
+    return SG_OK;
+}
+
+extern "C" int sg_server_destroy(sg_server *s)
+{
+    if (!s) return SG_OK;
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        s->stop = true;
+    }
+    s->cv_work.notify_all();
+    if (s->worker.joinable()) s->worker.join();
+    if (s->e) {
+        (void)hipStreamSynchronize(s->e->stream);
+        sg_engine_destroy(s->e);
+    }
+    if (s->pool) (void)hipFree(s->pool);
+    if (s->d_src) (void)hipFree(s->d_src);
+    if (s->d_rows) (void)hipFree(s->d_rows);
+    if (s->h_src) (void)hipHostFree(s->h_src);
+    if (s->h_rows) (void)hipHostFree(s->h_rows);
+    delete s;
+    return SG_OK;
+}
+
+namespace {
+
+int submit_locked(sg_server *s, int32_t kind, uint32_t source, double t_arrival, bool auto_release, double *lat,
+                  uint64_t *ticket)
+{
+    const uint64_t t = s->next_ticket++;
+    sg_server::Req r{};
+    r.kind = kind;
+    r.source = source;
+    r.row = -1;
+    r.state = kQueued;
+    r.status = SG_OK;
+    r.auto_release = auto_release;
+    r.t_arrival = t_arrival;
+    r.latency_out = lat;
+    s->reqs.emplace(t, r);
+    s->queue[kind].push_back(t);
+    s->submitted++;
+    if (ticket) *ticket = t;
+    return SG_OK;
+}
+
+}  // namespace
+
+extern "C" int sg_server_submit(sg_server *s, int32_t kind, uint32_t source, uint64_t *ticket)
+{
+    if (!s || !ticket) SG_FAIL(SG_ERR_INVALID, "sg_server_submit: null argument");
+    if (kind < 0 || kind >= SG_QUERY_KINDS) SG_FAIL(SG_ERR_INVALID, "sg_server_submit: unknown query kind");
+    if (source >= s->g->dev.n) SG_FAIL(SG_ERR_RANGE, "sg_server_submit: source >= n");
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        submit_locked(s, kind, source, now_s(), false, nullptr, ticket);
+    }
+    s->cv_work.notify_one();
+    return SG_OK;
+}
+
+extern "C" int sg_server_wait(sg_server *s, uint64_t ticket, int32_t timeout_ms)
+{
+    if (!s) SG_FAIL(SG_ERR_INVALID, "sg_server_wait: null server");
+    std::unique_lock<std::mutex> lk(s->mu);
+    auto it = s->reqs.find(ticket);
+    if (it == s->reqs.end()) SG_FAIL(SG_ERR_INVALID, "sg_server_wait: unknown ticket");
+    auto done = [&] {
+        auto jt = s->reqs.find(ticket);
+        return jt == s->reqs.end() || jt->second.state >= kDone;
+    };
+    if (timeout_ms < 0) s->cv_done.wait(lk, done);
+    else if (!s->cv_done.wait_for(lk, std::chrono::milliseconds(timeout_ms), done))
+        SG_FAIL(SG_ERR_TIMEOUT, "sg_server_wait: not finished");
+    auto jt = s->reqs.find(ticket);
+    if (jt != s->reqs.end() && jt->second.state == kFailed) SG_FAIL(jt->second.status, "sg_server_wait: batch failed");
+    return SG_OK;
+}
+
+extern "C" int sg_server_result(sg_server *s, uint64_t ticket, void **dev_ptr, uint64_t *bytes, double *queue_ms,
+                                double *total_ms)
+{
+    if (!s) SG_FAIL(SG_ERR_INVALID, "sg_server_result: null server");
+    std::lock_guard<std::mutex> lk(s->mu);
+    auto it = s->reqs.find(ticket);
+    if (it == s->reqs.end()) SG_FAIL(SG_ERR_INVALID, "sg_server_result: unknown ticket");
+    const sg_server::Req &r = it->second;
+    if (r.state != kDone) SG_FAIL(SG_ERR_STATE, "sg_server_result: query not finished");
+    if (dev_ptr) *dev_ptr = (char *)s->pool + (size_t)r.row * s->row_bytes;
+    if (bytes) *bytes = s->row_bytes;
+    if (queue_ms) *queue_ms = (r.t_start - r.t_arrival) * 1e3;
+    if (total_ms) *total_ms = (r.t_done - r.t_arrival) * 1e3;
+    return SG_OK;
+}
+
+extern "C" int sg_server_copy_result(sg_server *s, uint64_t ticket, void *host_out, uint64_t bytes)
+{
+    void *p = nullptr;
+    uint64_t nb = 0;
+    int rc = sg_server_result(s, ticket, &p, &nb, nullptr, nullptr);
+    if (rc) return rc;
+    if (!host_out || bytes < nb) SG_FAIL(SG_ERR_CAPACITY, "sg_server_copy_result: host buffer too small");
+    SG_HIP(hipMemcpy(host_out, p, nb, hipMemcpyDeviceToHost));
+    return SG_OK;
+}
+
+extern "C" int sg_server_release(sg_server *s, uint64_t ticket)
+{
+    if (!s) SG_FAIL(SG_ERR_INVALID, "sg_server_release: null server");
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        auto it = s->reqs.find(ticket);
+        if (it == s->reqs.end()) SG_FAIL(SG_ERR_INVALID, "sg_server_release: unknown ticket");
+        if (it->second.state < kDone) SG_FAIL(SG_ERR_STATE, "sg_server_release: query still queued or running");
+        if (it->second.row >= 0) s->free_rows.push_back((uint32_t)it->second.row);
+        s->reqs.erase(it);
+    }
+    s->cv_work.notify_one();
+    return SG_OK;
+}
+
+extern "C" int sg_server_get_stats(sg_server *s, sg_server_stats *out)
+{
+    if (!s || !out) SG_FAIL(SG_ERR_INVALID, "sg_server_get_stats: null argument");
+    std::lock_guard<std::mutex> lk(s->mu);
+    std::memset(out, 0, sizeof(*out));
+    out->submitted = s->submitted;
+    out->completed = s->completed;
+    out->failed = s->failed;
+    out->batches = s->batches;
+    out->mean_batch = s->batches ? (double)s->batched_queries / (double)s->batches : 0.0;
+    out->busy_ms = s->busy_s * 1e3;
+    out->rows = s->nrows;
+    out->free_rows = (uint32_t)s->free_rows.size();
+    uint64_t q = 0;
+    for (int k = 0; k < SG_QUERY_KINDS; k++) q += s->queue[k].size();
+    out->queued = q;
+    out->device_bytes = s->g->device_bytes + s->e->aux_bytes + (uint64_t)s->nrows * s->row_bytes + s->e->value_cap;
+    return SG_OK;
+}
+
+extern "C" int sg_server_replay(sg_server *s, const int32_t *kinds, const uint32_t *sources, const double *arrival_s,
+                                int64_t count, double *latency_ms)
+{
+    if (!s || !kinds || !sources || !arrival_s || !latency_ms) SG_FAIL(SG_ERR_INVALID, "sg_server_replay: null argument");
+    for (int64_t i = 0; i < count; i++) {
+        if (kinds[i] < 0 || kinds[i] >= SG_QUERY_KINDS) SG_FAIL(SG_ERR_INVALID, "sg_server_replay: unknown query kind");
+        if (sources[i] >= s->g->dev.n) SG_FAIL(SG_ERR_RANGE, "sg_server_replay: source >= n");
+        if (i && arrival_s[i] < arrival_s[i - 1]) SG_FAIL(SG_ERR_INVALID, "sg_server_replay: arrivals must not decrease");
+        latency_ms[i] = -1.0;
+    }
+    uint64_t base_completed, base_failed;
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        base_completed = s->completed;
+        base_failed = s->failed;
+    }
+    // open loop: each query is submitted at its arrival time (measured from
+    // the start of the replay); latency counts from that scheduled time
+    const double t0 = now_s();
+    for (int64_t i = 0; i < count; i++) {
+        const double due = t0 + arrival_s[i];
+        for (;;) {
+            const double t = now_s();
+            if (t >= due) break;
+            if (due - t > 2e-4) std::this_thread::sleep_for(std::chrono::microseconds((int64_t)((due - t) * 1e6) - 100));
+        }
+        {
+            std::lock_guard<std::mutex> lk(s->mu);
+            submit_locked(s, kinds[i], sources[i], due, true, &latency_ms[i], nullptr);
+        }
+        s->cv_work.notify_one();
+    }
+    std::unique_lock<std::mutex> lk(s->mu);
+    s->cv_done.wait(lk, [&] { return s->completed + s->failed - base_completed - base_failed >= (uint64_t)count; });
+    if (s->failed != base_failed) SG_FAIL(SG_ERR_HIP, "sg_server_replay: a batch failed");
+    return SG_OK;
+}

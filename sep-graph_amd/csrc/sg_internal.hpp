@@ -119,8 +119,9 @@ struct sg_engine {
     int last_algo = -1;
 
     // device allocations
-    void *value = nullptr;        // per-vertex result (levels / distances / ranks)
-    uint64_t value_bytes = 0;
+    void *value = nullptr;        // result of the last run (levels / distances / ranks / BC)
+    uint64_t value_bytes = 0;     // bytes of that result
+    uint64_t value_cap = 0;       // allocated bytes
     uint64_t *bitmaps = nullptr;  // visited + 2 frontier bitmaps (3 * words)
     uint32_t *queues = nullptr;   // 3 frontier queues of qcap entries
     uint32_t qcap = 0;
@@ -131,6 +132,8 @@ struct sg_engine {
     sg::DevTrace *trace = nullptr;    // kTraceRows rows
     void *aux = nullptr;              // per-algorithm working arrays (PageRank, BC), grown on demand
     uint64_t aux_bytes = 0;
+    uint32_t *srcbuf = nullptr;       // device copy of a batched call's sources
+    uint64_t srcbuf_cap = 0;          // entries
     uint32_t *exits = nullptr;        // per-block exit records (diagnostics), kMaxBlocks x 4
     uint32_t trace_rows = 0;          // rows written by the last run (host copy)
     double wall_khz = 100000.0;       // device wall clock rate
@@ -157,9 +160,25 @@ struct sg_engine {
 namespace sg {
 int engine_ensure_bfs_state(sg_engine *e);
 int engine_ensure_stamp(sg_engine *e);
+// Make the value array hold `bytes` (grown when needed, synchronising the
+// stream first) and record that the next result has that size.
+int engine_ensure_value(sg_engine *e, uint64_t bytes);
 // Grow the engine's auxiliary arena to at least `bytes` (synchronises the
 // stream before replacing a smaller one; contents are not preserved).
 int engine_ensure_aux(sg_engine *e, uint64_t bytes);
+int engine_ensure_srcbuf(sg_engine *e, uint64_t entries);
+
+// Batched (bit-packed, up to 64 sources per launch) query kinds.
+enum MsKind { kMsBfs = 0, kMsSssp = 1, kMsBcDep = 2, kMsBcSum = 3 };
+// Size the engine's arena for a kind (may synchronise and reallocate).
+int ms_prepare(sg_engine *e, int kind);
+// One batch of nsrc <= 64 device-resident sources, asynchronously on the
+// engine's stream. Results: row dev_orow[i] (or i when null) of `out`, n
+// elements per row (int32 levels, uint32 distances, FP32 dependencies); for
+// kMsBcSum, `out` is the n-float BC sum written by the batch with `last`.
+int ms_launch(sg_engine *e, int kind, const uint32_t *dev_src, uint32_t nsrc, void *out, const uint32_t *dev_orow,
+              bool first, bool last);
+
 // Device allocation owned by the engine (freed with it); reports SG_ERR_NOMEM.
 int engine_alloc(void **p, uint64_t bytes);
 // Grid for a persistent kernel: the largest blocks-per-CU x CUs (starting

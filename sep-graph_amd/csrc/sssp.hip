@@ -439,6 +439,7 @@ extern "C" int sg_sssp(sg_engine *e, uint32_t source)
     if (force < -1) SG_FAIL(SG_ERR_UNSUPPORTED, "sg_sssp: only data-driven push/pull variants apply to SSSP");
     int rc = engine_ensure_bfs_state(e);
     if (rc) return rc;
+    if ((rc = engine_ensure_value(e, (uint64_t)d.n * 4))) return rc;
     if ((rc = engine_ensure_stamp(e))) return rc;
     SsspArgs a{};
     a.g = d;

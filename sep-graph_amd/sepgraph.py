@@ -138,6 +138,10 @@ def lib() -> C.CDLL:
             "sg_bfs": ([_P, C.c_uint32], C.c_int),
             "sg_sssp": ([_P, C.c_uint32], C.c_int),
             "sg_pagerank": ([_P, C.c_double, C.c_double, C.c_int32, C.c_int32], C.c_int),
+            "sg_bc": ([_P, _P, C.c_int32, C.c_int32], C.c_int),
+            "sg_bc_dependency": ([_P, _P, C.c_int32, C.c_int32], C.c_int),
+            "sg_bfs_multi": ([_P, _P, C.c_int32, C.c_int32], C.c_int),
+            "sg_sssp_multi": ([_P, _P, C.c_int32, C.c_int32], C.c_int),
             "sg_engine_sync": ([_P], C.c_int),
             "sg_engine_value_ptr": ([_P, C.POINTER(_P), C.POINTER(C.c_uint64)], C.c_int),
             "sg_engine_gather_value": ([_P, _P, C.c_uint64], C.c_int),
@@ -333,6 +337,26 @@ class Engine:
         """Normalised PageRank into the value array (FP32); mode PR_PULL_TD or PR_PUSH_DD."""
         _check(lib().sg_pagerank(self.h, float(damping), float(tol), int(max_iter), int(mode)))
 
+    def bc(self, sources, batch: int = 64):
+        """0.5 * sum of Brandes dependencies over `sources` into the value array (FP32)."""
+        src = _u32(sources)
+        _check(lib().sg_bc(self.h, _ptr(src), len(src), int(batch)))
+
+    def bc_dependency(self, sources, batch: int = 64):
+        """Per-source 0.5 * delta_s rows (k x n FP32) into the value array."""
+        src = _u32(sources)
+        _check(lib().sg_bc_dependency(self.h, _ptr(src), len(src), int(batch)))
+
+    def bfs_multi(self, sources, batch: int = 64):
+        """BFS levels from every source (k x n int32) into the value array."""
+        src = _u32(sources)
+        _check(lib().sg_bfs_multi(self.h, _ptr(src), len(src), int(batch)))
+
+    def sssp_multi(self, sources, batch: int = 64):
+        """SSSP distances from every source (k x n uint32) into the value array."""
+        src = _u32(sources)
+        _check(lib().sg_sssp_multi(self.h, _ptr(src), len(src), int(batch)))
+
     def sync(self):
         _check(lib().sg_engine_sync(self.h))
 
@@ -373,6 +397,131 @@ class Engine:
     def close(self):
         if self.h:
             lib().sg_engine_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+# -------------------------------------------------------------------- serving
+QUERY_BFS, QUERY_SSSP, QUERY_BC_DEP = 0, 1, 2
+_QUERY_DTYPE = {QUERY_BFS: np.int32, QUERY_SSSP: np.uint32, QUERY_BC_DEP: np.float32}
+
+
+class ServerConfig(C.Structure):
+    _fields_ = [
+        ("memory_budget", C.c_uint64),
+        ("max_rows", C.c_int32),
+        ("max_batch", C.c_int32 * 3),
+        ("reserved", C.c_int32 * 8),
+    ]
+
+
+class ServerStats(C.Structure):
+    _fields_ = [
+        ("submitted", C.c_uint64),
+        ("completed", C.c_uint64),
+        ("failed", C.c_uint64),
+        ("batches", C.c_uint64),
+        ("queued", C.c_uint64),
+        ("device_bytes", C.c_uint64),
+        ("mean_batch", C.c_double),
+        ("busy_ms", C.c_double),
+        ("rows", C.c_uint32),
+        ("free_rows", C.c_uint32),
+        ("reserved", C.c_uint32 * 8),
+    ]
+
+    def as_dict(self) -> dict:
+        return {k: getattr(self, k) for k, _ in self._fields_ if k != "reserved"}
+
+
+_server_ready = False
+
+
+def _server_lib():
+    global _server_ready
+    L = lib()
+    if not _server_ready:
+        D = C.POINTER(C.c_double)
+        sig = {
+            "sg_server_config_default": ([C.POINTER(ServerConfig)], None),
+            "sg_server_create": ([_P, C.POINTER(ServerConfig), C.POINTER(EngineOptions), C.POINTER(_P)], C.c_int),
+            "sg_server_destroy": ([_P], C.c_int),
+            "sg_server_submit": ([_P, C.c_int32, C.c_uint32, C.POINTER(C.c_uint64)], C.c_int),
+            "sg_server_wait": ([_P, C.c_uint64, C.c_int32], C.c_int),
+            "sg_server_result": ([_P, C.c_uint64, C.POINTER(_P), C.POINTER(C.c_uint64), D, D], C.c_int),
+            "sg_server_copy_result": ([_P, C.c_uint64, _P, C.c_uint64], C.c_int),
+            "sg_server_release": ([_P, C.c_uint64], C.c_int),
+            "sg_server_get_stats": ([_P, C.POINTER(ServerStats)], C.c_int),
+            "sg_server_replay": ([_P, _P, _P, _P, C.c_int64, _P], C.c_int),
+        }
+        for name, (args, res) in sig.items():
+            f = getattr(L, name)
+            f.argtypes = args
+            f.restype = res
+        _server_ready = True
+    return L
+
+
+class Server:
+    """sg_server: batched BFS / SSSP / BC-dependency queries on a resident graph."""
+
+    def __init__(self, graph: Graph, memory_budget: int = 4 << 30, max_rows: int = 4096, max_batch=(64, 64, 64), **opts):
+        L = _server_lib()
+        self.graph = graph
+        self.n = graph.host.n
+        cfg = ServerConfig()
+        L.sg_server_config_default(C.byref(cfg))
+        cfg.memory_budget = memory_budget
+        cfg.max_rows = max_rows
+        for i, b in enumerate(max_batch):
+            cfg.max_batch[i] = b
+        o = default_options(**opts)
+        h = _P()
+        _check(L.sg_server_create(graph.h, C.byref(cfg), C.byref(o), C.byref(h)))
+        self.h = h
+
+    def submit(self, kind: int, source: int) -> int:
+        t = C.c_uint64()
+        _check(_server_lib().sg_server_submit(self.h, int(kind), int(source), C.byref(t)))
+        return t.value
+
+    def wait(self, ticket: int, timeout_ms: int = -1):
+        _check(_server_lib().sg_server_wait(self.h, ticket, timeout_ms))
+
+    def result(self, ticket: int, kind: int) -> np.ndarray:
+        out = np.empty(self.n, _QUERY_DTYPE[kind])
+        _check(_server_lib().sg_server_copy_result(self.h, ticket, _ptr(out), out.nbytes))
+        return out
+
+    def times(self, ticket: int):
+        q, t = C.c_double(), C.c_double()
+        _check(_server_lib().sg_server_result(self.h, ticket, None, None, C.byref(q), C.byref(t)))
+        return q.value, t.value
+
+    def release(self, ticket: int):
+        _check(_server_lib().sg_server_release(self.h, ticket))
+
+    def stats(self) -> dict:
+        s = ServerStats()
+        _check(_server_lib().sg_server_get_stats(self.h, C.byref(s)))
+        return s.as_dict()
+
+    def replay(self, kinds, sources, arrival_s) -> np.ndarray:
+        k = np.ascontiguousarray(kinds, np.int32)
+        s = _u32(sources)
+        a = np.ascontiguousarray(arrival_s, np.float64)
+        lat = np.empty(len(k), np.float64)
+        _check(_server_lib().sg_server_replay(self.h, _ptr(k), _ptr(s), _ptr(a), len(k), _ptr(lat)))
+        return lat
+
+    def close(self):
+        if self.h:
+            _server_lib().sg_server_destroy(self.h)
             self.h = None
 
     def __del__(self):

@@ -1,0 +1,109 @@
+"""Serving: continuous batching of BFS / SSSP / BC-dependency queries through
+the C ABI. Every result must equal the same query run alone through the
+single-query entry points (bit-exact for BFS levels and SSSP distances,
+1e-5 relative for BC dependencies, whose FP64 sums may associate differently
+in a different batch), whatever the batch it landed in; admission must hold
+queries back while no result row is free."""
+import threading
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def wgraph(sg):
+    return sg.rmat_graph(14, 16, symmetrize=True, seed=1, weights=(1, 64))
+
+
+def alone(sg, e, kind, s):
+    if kind == sg.QUERY_BFS:
+        e.bfs(int(s))
+        return e.gather(np.int32)
+    if kind == sg.QUERY_SSSP:
+        e.sssp(int(s))
+        return e.gather(np.uint32)
+    e.bc_dependency([int(s)], 1)
+    return e.gather(np.float32)
+
+
+def same(sg, kind, got, ref):
+    if kind == sg.QUERY_BC_DEP:
+        return bool(np.all(np.abs(got.astype(np.float64) - ref) <= 1e-5 * np.abs(ref) + 1e-6))
+    return np.array_equal(got, ref)
+
+
+def test_server_mix_equals_alone(sg, gpu, wgraph):
+    g = sg.Graph(wgraph)
+    srv = sg.Server(g, max_rows=512)
+    rng = np.random.default_rng(4)
+    srcs = sg.pick_sources(wgraph.row, 150, seed=5)
+    kinds = rng.choice([0, 1, 2], size=150, p=[0.6, 0.3, 0.1])
+    tickets = [None] * 150
+
+    def client(lo, hi):
+        for i in range(lo, hi):
+            tickets[i] = srv.submit(int(kinds[i]), int(srcs[i]))
+
+    th = [threading.Thread(target=client, args=(i * 50, i * 50 + 50)) for i in range(3)]
+    for t in th:
+        t.start()
+    for t in th:
+        t.join()
+    results = []
+    for i in range(150):
+        srv.wait(tickets[i], 60000)
+        results.append(srv.result(tickets[i], int(kinds[i])))
+        q, tot = srv.times(tickets[i])
+        assert 0 <= q <= tot
+        srv.release(tickets[i])
+    st = srv.stats()
+    assert st["completed"] == 150 and st["failed"] == 0
+    assert st["mean_batch"] > 1.0, st  # concurrent arrivals were batched
+    e = sg.Engine(g)
+    for i in range(150):
+        ref = alone(sg, e, int(kinds[i]), srcs[i])
+        assert same(sg, int(kinds[i]), results[i], ref), (i, int(kinds[i]), int(srcs[i]))
+    srv.close()
+
+
+def test_server_admission_by_rows(sg, gpu, wgraph):
+    g = sg.Graph(wgraph)
+    n = wgraph.n
+    srv = sg.Server(g, memory_budget=3 * n * 4, max_rows=0)
+    assert srv.stats()["rows"] == 3
+    srcs = sg.pick_sources(wgraph.row, 8, seed=6)
+    t = [srv.submit(sg.QUERY_BFS, int(s)) for s in srcs]
+    for i in range(3):
+        srv.wait(t[i], 60000)
+    # the remaining five wait for a free row
+    with pytest.raises(sg.SepGraphError):
+        srv.wait(t[3], 300)
+    assert srv.stats()["queued"] == 5
+    e = sg.Engine(g)
+    for i in range(8):
+        srv.wait(t[i], 60000)
+        assert np.array_equal(srv.result(t[i], sg.QUERY_BFS), alone(sg, e, sg.QUERY_BFS, srcs[i]))
+        srv.release(t[i])
+    with pytest.raises(sg.SepGraphError):
+        srv.submit(sg.QUERY_BFS, n)
+    with pytest.raises(sg.SepGraphError):
+        srv.submit(7, 0)
+    srv.close()
+
+
+def test_server_replay(sg, gpu, wgraph):
+    g = sg.Graph(wgraph)
+    srv = sg.Server(g, max_rows=1024)
+    rng = np.random.default_rng(8)
+    count = 2000
+    kinds = rng.choice([0, 1, 2], size=count, p=[0.6, 0.3, 0.1])
+    srcs = sg.pick_sources(wgraph.row, count, seed=9) if count <= wgraph.n else None
+    arr = np.cumsum(rng.exponential(1.0 / 20000.0, count))
+    lat = srv.replay(kinds, srcs, arr)
+    assert np.all(lat >= 0)
+    st = srv.stats()
+    assert st["completed"] == count and st["free_rows"] == st["rows"]
+    assert st["mean_batch"] > 1.0
+    srv.close()
