@@ -74,6 +74,66 @@ def cpu_baseline(hg, sources, gpu_levels, budget_s):
     }
 
 
+def serving_bench(sg, g, hg, e, args):
+    """Queries/s and latency of the batching server against plain one-at-a-time
+    execution, on the resident graph: a seeded mix of 60% BFS, 30% SSSP, 10%
+    BC-dependency queries with sources uniform over non-isolated vertices."""
+    rng = np.random.default_rng(4)
+    pool = sg.pick_sources(hg.row, 4096, seed=5)
+    # plain: the single-query entry points, one query at a time, synchronised
+    kinds = rng.choice([0, 1, 2], size=args.serve_plain, p=[0.6, 0.3, 0.1])
+    srcs = rng.choice(pool, size=args.serve_plain)
+    t0 = time.perf_counter()
+    for k, s in zip(kinds, srcs):
+        if k == 0:
+            e.bfs(int(s))
+        elif k == 1:
+            e.sssp(int(s))
+        else:
+            e.bc_dependency([int(s)], 1)
+        e.sync()
+    plain_qps = args.serve_plain / (time.perf_counter() - t0)
+    log(f"plain one-at-a-time: {plain_qps:.1f} queries/s")
+    mb = [int(x) for x in args.serve_batch.split(",")]
+    srv = sg.Server(g, memory_budget=8 << 30, max_rows=1024, max_batch=mb)
+    # warm the batched kernels (grid probes, arenas) outside the timed runs
+    srv.replay([0, 1, 2], pool[:3], [0.0, 0.0, 0.0])
+    runs = []
+    for rate in [float(r) for r in args.serve_rates.split(",")]:
+        kinds = rng.choice([0, 1, 2], size=args.serve_count, p=[0.6, 0.3, 0.1])
+        srcs = rng.choice(pool, size=args.serve_count)
+        arr = np.cumsum(rng.exponential(1.0 / rate, args.serve_count))
+        before = srv.stats()
+        t0 = time.perf_counter()
+        lat = srv.replay(kinds, srcs, arr)
+        wall = time.perf_counter() - t0
+        after = srv.stats()
+        nb = after["batches"] - before["batches"]
+        run = {
+            "offered_qps": rate,
+            "achieved_qps": round(args.serve_count / wall, 1),
+            "p50_ms": round(float(np.percentile(lat, 50)), 3),
+            "p99_ms": round(float(np.percentile(lat, 99)), 3),
+            "batches": int(nb),
+            "mean_batch": round(args.serve_count / max(nb, 1), 2),
+        }
+        log(f"serving at {rate:.0f} q/s offered: {run}")
+        runs.append(run)
+    st = srv.stats()
+    srv.close()
+    best = max(r["achieved_qps"] for r in runs)
+    return {
+        "mix": "60% BFS / 30% SSSP / 10% BC-dependency, RMAT-%d directed" % args.scale,
+        "plain_qps": round(plain_qps, 1),
+        "batched_qps": best,
+        "speedup_vs_plain": round(best / plain_qps, 2),
+        "runs": runs,
+        "queries_per_rate": args.serve_count,
+        "max_batch": mb,
+        "device_bytes": int(st["device_bytes"]),
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -83,6 +143,13 @@ def main():
     ap.add_argument("--sources", type=int, default=64)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--sssp-sources", type=int, default=16)
+    ap.add_argument("--no-serving", action="store_true")
+    ap.add_argument("--serve-rates", type=str, default="2000,20000",
+                    help="Poisson arrival rates (queries/s) for the serving run")
+    ap.add_argument("--serve-count", type=int, default=1500, help="queries per arrival rate")
+    ap.add_argument("--serve-plain", type=int, default=40, help="queries timed one at a time (plain baseline)")
+    ap.add_argument("--serve-batch", type=str, default="64,64,64", help="max sources per launch: bfs,sssp,bc")
     ap.add_argument("--copy-mb", type=int, default=0,
                     help="only run a plain device copy of this many MiB (counter calibration)")
     args = ap.parse_args()
@@ -160,6 +227,23 @@ def main():
     log(f"timed {nq * args.steps} BFS in {wall * 1e3:.1f} ms: {gteps:.1f} GTEPS, "
         f"kernel mean {durs_ms.mean() * 1e3:.1f} us, achieved {achieved:.0f} GB/s")
 
+    # SSSP on the same graph and sources (a subset: one query is ~20x a BFS)
+    ns = min(args.sssp_sources, nq)
+    sssp_m, sssp_ms = 0.0, 0.0
+    for s in sources[:ns]:
+        e.sssp(int(s))
+        st = e.stats()
+        if st["status"] != 0:
+            raise RuntimeError(f"SSSP status {st['status']} for source {s}")
+        sssp_m += st["reached_edges"]
+        sssp_ms += st["kernel_ms"]
+    sssp_gteps = sssp_m / (sssp_ms / 1e3) / 1e9 if sssp_ms else 0.0
+    log(f"SSSP: {ns} queries, mean {sssp_ms / max(ns, 1):.2f} ms, {sssp_gteps:.2f} GTEPS")
+
+    serving = None
+    if not args.no_serving:
+        serving = serving_bench(sg, g, hg, e, args)
+
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(hg, sources, keep, args.cpu_seconds)
@@ -200,6 +284,8 @@ def main():
             "mean_launch_us": round(float(durs_ms.mean()) * 1e3, 2),
         },
         "cpu_baseline": cpu,
+        "sssp": {"gteps": round(sssp_gteps, 3), "queries": ns, "mean_ms": round(sssp_ms / max(ns, 1), 3)},
+        "serving": serving,
     }
     print(json.dumps(out), flush=True)
 

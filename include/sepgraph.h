@@ -309,6 +309,36 @@ int sg_server_get_stats(sg_server *s, sg_server_stats *out);
 int sg_server_replay(sg_server *s, const int32_t *kinds, const uint32_t *sources, const double *arrival_s,
                      int64_t count, double *latency_ms);
 
+/* ---------------------------------------------------------------- autotuner
+ * No reference counterpart (the reference fixes alpha = 15, beta = 18 and
+ * --block_size for every graph, app_skeleton.h:50-51). Times the real kernels
+ * on k sample sources of the engine's graph and picks: alpha/beta and block
+ * size for sg_bfs, alpha and block size for sg_sssp, and per query kind the
+ * sources per launch with the highest queries/s whose launch time stays
+ * within latency_budget_ms (feed out->max_batch to sg_server_config). Leaves
+ * the engine on the tuned BFS options. Results never depend on these knobs. */
+typedef struct sg_tune_config {
+    double latency_budget_ms;  /* per batched launch (default 25) */
+    int32_t tune_bfs, tune_sssp, tune_batches;
+    int32_t reserved[8];
+} sg_tune_config;
+
+typedef struct sg_tune_result {
+    double bfs_alpha, bfs_beta;
+    int32_t bfs_block_size;
+    int32_t sssp_block_size;
+    double sssp_alpha;
+    double bfs_ms, bfs_default_ms;    /* mean per query: tuned / alpha 15, beta 18, 1024 threads */
+    double sssp_ms, sssp_default_ms;
+    int32_t max_batch[3];             /* per SG_QUERY_* kind */
+    int32_t pad;
+    double batch_ms[3], batch_qps[3];
+    int32_t reserved[8];
+} sg_tune_result;
+
+void sg_tune_config_default(sg_tune_config *c);
+int sg_autotune(sg_engine *e, const uint32_t *sources, int32_t k, const sg_tune_config *cfg, sg_tune_result *out);
+
 /* Synchronise the engine's stream and report the device-side status. */
 int sg_engine_sync(sg_engine *e);
 /* Device pointer and size in bytes of the last run's per-vertex result. */

@@ -1,0 +1,252 @@
+// Header-only C++ facade with the reference's driver-side names over the C ABI
+// of libsepgraph.so, so a hybrid_* sample driver keeps its structure:
+//
+//   typedef sepgraph::engine::Engine<level_t, level_t, sepgraph::graphs::NoWeight,
+//                                    sepgraph::apps::BFS, index_t> HybridEngine;
+//   HybridEngine engine(sepgraph::policy::AlgoType::TRAVERSAL_SCHEME);
+//   engine.LoadGraph(csr);                   // reference: LoadGraph() from --graphfile
+//   sepgraph::common::EngineOptions opt;
+//   opt.ForceVariant(sepgraph::common::AlgoVariant::ASYNC_PUSH_DD);
+//   engine.SetOptions(opt);
+//   engine.InitGraph(source);
+//   engine.Start();
+//   const std::vector<level_t> &levels = engine.GatherValue();
+//
+// Reference interfaces mirrored (names and argument meaning):
+//   sepgraph::common enums and AlgoVariant      include/framework/common.h:13-177
+//   sepgraph::common::EngineOptions             include/framework/common.h:203-321
+//   sepgraph::policy::AlgoType                  include/framework/hybrid_policy.h
+//   sepgraph::engine::Engine<TValue, TBuffer, TWeight, TApp, InitArgs...>
+//       SetOptions / LoadGraph / InitGraph / Start / PrintInfo / GatherValue /
+//       CSRGraph                                include/framework/framework.cuh:56-397
+// The apps are tags: the per-edge hooks of BFS, SSSP, PageRank and BC
+// (samples/hybrid_*/hybrid_*.cu) are compiled into the library's kernels, so
+// an app is selected, not re-implemented. Errors throw sepgraph::Error
+// carrying the SG_ERR_* code instead of exiting the process.
+#pragma once
+
+#include <cstdint>
+#include <cstdio>
+#include <stdexcept>
+#include <string>
+#include <tuple>
+#include <type_traits>
+#include <vector>
+
+#include "sepgraph.h"
+
+namespace sepgraph {
+
+class Error : public std::runtime_error {
+public:
+    Error(int code, const std::string &what) : std::runtime_error(what), code_(code) {}
+    int code() const { return code_; }
+
+private:
+    int code_;
+};
+
+inline void check(int rc)
+{
+    if (rc != SG_OK) throw Error(rc, std::string("sepgraph: ") + sg_last_error());
+}
+
+namespace common {
+
+enum class LoadBalancing { NONE, COARSE_GRAINED, FINE_GRAINED, HYBRID };
+enum class Priority { NONE, LOW_HIGH, SAMPLING };
+enum class Model { SYNC, ASYNC };
+enum class MsgPassing { PUSH, PULL };
+enum class Scheduling { TOPOLOGY_DRIVEN, DATA_DRIVEN };
+
+class AlgoVariant {
+public:
+    Model m_model = Model::ASYNC;
+    MsgPassing m_msg_passing = MsgPassing::PUSH;
+    Scheduling m_scheduling = Scheduling::DATA_DRIVEN;
+
+    static const AlgoVariant ASYNC_PUSH_TD, ASYNC_PUSH_DD, ASYNC_PULL_TD, ASYNC_PULL_DD;
+    static const AlgoVariant SYNC_PUSH_TD, SYNC_PUSH_DD, SYNC_PULL_TD, SYNC_PULL_DD;
+
+    AlgoVariant() = default;
+    AlgoVariant(Model model, MsgPassing msg_passing, Scheduling scheduling)
+        : m_model(model), m_msg_passing(msg_passing), m_scheduling(scheduling)
+    {
+    }
+    bool operator==(const AlgoVariant &o) const
+    {
+        return m_model == o.m_model && m_msg_passing == o.m_msg_passing && m_scheduling == o.m_scheduling;
+    }
+    MsgPassing GetMsgPassing() const { return m_msg_passing; }
+    Scheduling GetScheduling() const { return m_scheduling; }
+    // SG_ASYNC_PUSH_TD .. SG_SYNC_PULL_DD
+    int Code() const
+    {
+        return (m_model == Model::SYNC ? 4 : 0) + (m_msg_passing == MsgPassing::PULL ? 2 : 0) +
+               (m_scheduling == Scheduling::DATA_DRIVEN ? 1 : 0);
+    }
+    std::string ToString() const
+    {
+        std::string s = m_model == Model::SYNC ? "SYNC_" : "ASYNC_";
+        s += m_msg_passing == MsgPassing::PUSH ? "PUSH_" : "PULL_";
+        s += m_scheduling == Scheduling::TOPOLOGY_DRIVEN ? "TD" : "DD";
+        return s;
+    }
+};
+
+inline const AlgoVariant AlgoVariant::ASYNC_PUSH_TD{Model::ASYNC, MsgPassing::PUSH, Scheduling::TOPOLOGY_DRIVEN};
+inline const AlgoVariant AlgoVariant::ASYNC_PUSH_DD{Model::ASYNC, MsgPassing::PUSH, Scheduling::DATA_DRIVEN};
+inline const AlgoVariant AlgoVariant::ASYNC_PULL_TD{Model::ASYNC, MsgPassing::PULL, Scheduling::TOPOLOGY_DRIVEN};
+inline const AlgoVariant AlgoVariant::ASYNC_PULL_DD{Model::ASYNC, MsgPassing::PULL, Scheduling::DATA_DRIVEN};
+inline const AlgoVariant AlgoVariant::SYNC_PUSH_TD{Model::SYNC, MsgPassing::PUSH, Scheduling::TOPOLOGY_DRIVEN};
+inline const AlgoVariant AlgoVariant::SYNC_PUSH_DD{Model::SYNC, MsgPassing::PUSH, Scheduling::DATA_DRIVEN};
+inline const AlgoVariant AlgoVariant::SYNC_PULL_TD{Model::SYNC, MsgPassing::PULL, Scheduling::TOPOLOGY_DRIVEN};
+inline const AlgoVariant AlgoVariant::SYNC_PULL_DD{Model::SYNC, MsgPassing::PULL, Scheduling::DATA_DRIVEN};
+
+class EngineOptions {
+public:
+    EngineOptions() { sg_engine_options_default(&o_); }
+
+    void SetSampleBasedPriority(float cut_threshold = 0.2f)
+    {
+        o_.priority = SG_PRIO_SAMPLING;
+        o_.cut_threshold = cut_threshold;
+    }
+    void SetTwoLevelBasedPriority(float priority_threshold)
+    {
+        o_.priority = SG_PRIO_LOW_HIGH;
+        o_.priority_threshold = priority_threshold;
+    }
+    void ForceVariant(AlgoVariant v) { o_.force_variant = v.Code(); }
+    bool IsForceVariant() const { return o_.force_variant != SG_VARIANT_AUTO; }
+    void SetFused() { o_.fused = 1; }
+    bool IsFused() const { return o_.fused != 0; }
+    void SetLoadBalancing(MsgPassing msg_passing, LoadBalancing lb)
+    {
+        (msg_passing == MsgPassing::PUSH ? o_.lb_push : o_.lb_pull) = (int32_t)lb;
+    }
+    // direction-switch thresholds (the reference's --alpha / --beta flags)
+    void SetAlphaBeta(double alpha, double beta)
+    {
+        o_.alpha = alpha;
+        o_.beta = beta;
+    }
+    void SetBlockSize(int block_size) { o_.block_size = block_size; }
+    void SetMaxIteration(int max_iteration) { o_.max_iteration = max_iteration; }
+    const sg_engine_options &c_options() const { return o_; }
+
+private:
+    sg_engine_options o_{};
+};
+
+}  // namespace common
+
+namespace policy {
+enum class AlgoType { TRAVERSAL_SCHEME, ITERATIVE_SCHEME };
+}
+
+namespace graphs {
+struct NoWeight {
+};
+// A host CSR (the reference's host::CSRGraph, csr_graph.h:104-476): row_start
+// has n + 1 entries, edge_dst m; edge_weights empty or m entries.
+struct CSRGraph {
+    uint32_t nnodes = 0;
+    uint64_t nedges = 0;
+    std::vector<uint32_t> row_start, edge_dst, edge_weights;
+    bool undirected = false;
+};
+}  // namespace graphs
+
+namespace apps {
+struct BFS {};       // samples/hybrid_bfs: levels from a source, InitGraph(source)
+struct SSSP {};      // samples/hybrid_sssp: distances from a source, InitGraph(source)
+struct PageRank {};  // samples/hybrid_pr: InitGraph(tolerance) (damping 0.85)
+struct BC {};        // samples/hybrid_bc: single-source BC, InitGraph(source)
+}  // namespace apps
+
+namespace engine {
+
+template <typename TValue, typename TBuffer, typename TWeight, typename TApp, typename... InitArgs>
+class Engine {
+public:
+    explicit Engine(policy::AlgoType algo_type) : algo_type_(algo_type) {}
+    Engine(const Engine &) = delete;
+    Engine &operator=(const Engine &) = delete;
+    ~Engine()
+    {
+        if (e_) sg_engine_destroy(e_);
+        if (g_) sg_graph_destroy(g_);
+    }
+
+    void SetOptions(common::EngineOptions &options)
+    {
+        options_ = options;
+        if (e_) check(sg_engine_set_options(e_, &options_.c_options()));
+    }
+
+    void LoadGraph(const graphs::CSRGraph &csr)
+    {
+        csr_ = csr;
+        const uint32_t *w = (!std::is_same<TWeight, graphs::NoWeight>::value && !csr_.edge_weights.empty())
+                                ? csr_.edge_weights.data()
+                                : nullptr;
+        check(sg_graph_create(csr_.nnodes, csr_.nedges, csr_.row_start.data(), csr_.edge_dst.data(), w, nullptr,
+                              nullptr, nullptr, csr_.undirected ? 1 : 0, &g_));
+        check(sg_engine_create(g_, &e_));
+        check(sg_engine_set_options(e_, &options_.c_options()));
+    }
+
+    // The app's initial state (source vertex for BFS, SSSP and BC; the
+    // convergence tolerance for PageRank).
+    void InitGraph(InitArgs... args) { init_ = std::make_tuple(args...); }
+
+    void Start()
+    {
+        if (!e_) throw Error(SG_ERR_STATE, "sepgraph: Start before LoadGraph");
+        if constexpr (std::is_same<TApp, apps::BFS>::value) {
+            check(sg_bfs(e_, (uint32_t)std::get<0>(init_)));
+        } else if constexpr (std::is_same<TApp, apps::SSSP>::value) {
+            check(sg_sssp(e_, (uint32_t)std::get<0>(init_)));
+        } else if constexpr (std::is_same<TApp, apps::PageRank>::value) {
+            check(sg_pagerank(e_, 0.85, (double)std::get<0>(init_), 0,
+                              algo_type_ == policy::AlgoType::ITERATIVE_SCHEME ? SG_PR_PUSH_DD : SG_PR_PULL_TD));
+        } else {
+            const uint32_t s = (uint32_t)std::get<0>(init_);
+            check(sg_bc_dependency(e_, &s, 1, 1));
+        }
+        check(sg_engine_sync(e_));
+        check(sg_engine_get_stats(e_, &stats_));
+    }
+
+    void PrintInfo() const
+    {
+        std::printf("rounds %u (push %u, pull %u), time %.3f ms\n", stats_.rounds, stats_.push_rounds,
+                    stats_.pull_rounds, stats_.kernel_ms);
+    }
+
+    const std::vector<TValue> &GatherValue()
+    {
+        static_assert(sizeof(TValue) == 4, "values are 32-bit");
+        value_.resize(csr_.nnodes);
+        check(sg_engine_gather_value(e_, value_.data(), (uint64_t)value_.size() * sizeof(TValue)));
+        return value_;
+    }
+
+    const graphs::CSRGraph &CSRGraph() const { return csr_; }
+    const sg_stats &Stats() const { return stats_; }
+    sg_engine *Handle() const { return e_; }
+
+private:
+    policy::AlgoType algo_type_;
+    common::EngineOptions options_;
+    graphs::CSRGraph csr_;
+    sg_graph *g_ = nullptr;
+    sg_engine *e_ = nullptr;
+    std::tuple<InitArgs...> init_{};
+    std::vector<TValue> value_;
+    sg_stats stats_{};
+};
+
+}  // namespace engine
+}  // namespace sepgraph

@@ -1,0 +1,147 @@
+// Autotuner (C ABI sg_autotune): picks, for one resident graph, the
+// direction-switch thresholds and block size of the single-query traversals
+// and the sources-per-launch of each batched query kind, by timing the real
+// kernels on sample sources.
+//
+// The reference fixes alpha = 15, beta = 18 (app_skeleton.h:50-51) and the
+// block size (--block_size 256, traversal.h:73-82) for every graph; its
+// ITERATIVE scheme only predicts per-round times from sampled rounds
+// (hybrid_policy.h:130-169). Here the choice is made per graph by
+// measurement, with results bit-identical for every candidate (the direction
+// rule and block shape change the schedule, never the levels or distances).
+#include <algorithm>
+#include <chrono>
+#include <cstring>
+#include <vector>
+
+#include "../../include/sepgraph.h"
+
+namespace {
+
+struct Saved {
+    sg_engine_options o;
+};
+
+// Mean kernel time (ms) of `run` over the sample sources; < 0 on error.
+template <typename F>
+double mean_ms(sg_engine *e, const uint32_t *src, int k, F run)
+{
+    double tot = 0.0;
+    for (int i = 0; i < k; i++) {
+        if (run(src[i]) != SG_OK) return -1.0;
+        sg_stats st;
+        if (sg_engine_get_stats(e, &st) != SG_OK || st.status != SG_OK) return -1.0;
+        tot += st.kernel_ms;
+    }
+    return tot / k;
+}
+
+}  // namespace
+
+extern "C" void sg_tune_config_default(sg_tune_config *c)
+{
+    if (!c) return;
+    std::memset(c, 0, sizeof(*c));
+    c->latency_budget_ms = 25.0;
+    c->tune_bfs = 1;
+    c->tune_sssp = 1;
+    c->tune_batches = 1;
+}
+
+extern "C" int sg_autotune(sg_engine *e, const uint32_t *sources, int32_t k, const sg_tune_config *cfg,
+                           sg_tune_result *out)
+{
+    if (!e || !sources || !out || k < 1) return SG_ERR_INVALID;
+    sg_tune_config c;
+    if (cfg) c = *cfg;
+    else sg_tune_config_default(&c);
+    std::memset(out, 0, sizeof(*out));
+    sg_engine_options base;
+    sg_engine_options_default(&base);
+    const double alphas[] = {4, 8, 15, 30, 60};
+    const double betas[] = {9, 18, 36};
+    const int blocks[] = {256, 512, 1024};
+    int rc = SG_OK;
+
+    if (c.tune_bfs) {
+        double best = -1.0;
+        for (int bs : blocks)
+            for (double a : alphas)
+                for (double b : betas) {
+                    sg_engine_options o = base;
+                    o.alpha = a;
+                    o.beta = b;
+                    o.block_size = bs;
+                    if ((rc = sg_engine_set_options(e, &o))) return rc;
+                    const double t = mean_ms(e, sources, k, [&](uint32_t s) { return sg_bfs(e, s); });
+                    if (t < 0) return SG_ERR_HIP;
+                    if (best < 0 || t < best) {
+                        best = t;
+                        out->bfs_alpha = a;
+                        out->bfs_beta = b;
+                        out->bfs_block_size = bs;
+                    }
+                    if (a == 15 && b == 18 && bs == 1024) out->bfs_default_ms = t;
+                }
+        out->bfs_ms = best;
+    }
+    if (c.tune_sssp) {
+        double best = -1.0;
+        for (int bs : blocks)
+            for (double a : alphas) {
+                sg_engine_options o = base;
+                o.alpha = a;
+                o.block_size = bs;
+                if ((rc = sg_engine_set_options(e, &o))) return rc;
+                const double t = mean_ms(e, sources, std::min(k, 4), [&](uint32_t s) { return sg_sssp(e, s); });
+                if (t < 0) return SG_ERR_HIP;
+                if (best < 0 || t < best) {
+                    best = t;
+                    out->sssp_alpha = a;
+                    out->sssp_block_size = bs;
+                }
+                if (a == 15 && bs == 1024) out->sssp_default_ms = t;
+            }
+        out->sssp_ms = best;
+    }
+    if (c.tune_batches) {
+        // per kind: the batch size with the highest queries/s whose launch
+        // stays within the latency budget (the smallest batch if none does)
+        if ((rc = sg_engine_set_options(e, &base))) return rc;
+        const int sizes[] = {4, 8, 16, 32, 64};
+        std::vector<uint32_t> src(64);
+        for (int i = 0; i < 64; i++) src[i] = sources[i % k];
+        for (int kind = 0; kind < SG_QUERY_KINDS; kind++) {
+            double best_qps = -1.0;
+            for (int b : sizes) {
+                auto run = [&]() {
+                    return kind == SG_QUERY_BFS ? sg_bfs_multi(e, src.data(), b, b)
+                           : kind == SG_QUERY_SSSP ? sg_sssp_multi(e, src.data(), b, b)
+                                                   : sg_bc_dependency(e, src.data(), b, b);
+                };
+                if ((rc = run())) return rc;  // warm (grid probe, arena)
+                if ((rc = run())) return rc;
+                sg_stats st;
+                if ((rc = sg_engine_get_stats(e, &st))) return rc;
+                const double ms = st.kernel_ms;
+                const double qps = b / (ms / 1e3);
+                const bool fits = ms <= c.latency_budget_ms;
+                if ((fits && qps > best_qps) || best_qps < 0) {
+                    best_qps = qps;
+                    out->max_batch[kind] = b;
+                    out->batch_ms[kind] = ms;
+                    out->batch_qps[kind] = qps;
+                }
+                if (!fits) break;
+            }
+        }
+    }
+    // leave the engine on the tuned BFS configuration
+    sg_engine_options o = base;
+    if (c.tune_bfs) {
+        o.alpha = out->bfs_alpha;
+        o.beta = out->bfs_beta;
+        o.block_size = out->bfs_block_size;
+    }
+    return sg_engine_set_options(e, &o);
+}

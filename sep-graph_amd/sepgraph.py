@@ -529,3 +529,59 @@ class Server:
             self.close()
         except Exception:
             pass
+
+
+# ------------------------------------------------------------------ autotuner
+class TuneConfig(C.Structure):
+    _fields_ = [
+        ("latency_budget_ms", C.c_double),
+        ("tune_bfs", C.c_int32),
+        ("tune_sssp", C.c_int32),
+        ("tune_batches", C.c_int32),
+        ("reserved", C.c_int32 * 8),
+    ]
+
+
+class TuneResult(C.Structure):
+    _fields_ = [
+        ("bfs_alpha", C.c_double),
+        ("bfs_beta", C.c_double),
+        ("bfs_block_size", C.c_int32),
+        ("sssp_block_size", C.c_int32),
+        ("sssp_alpha", C.c_double),
+        ("bfs_ms", C.c_double),
+        ("bfs_default_ms", C.c_double),
+        ("sssp_ms", C.c_double),
+        ("sssp_default_ms", C.c_double),
+        ("max_batch", C.c_int32 * 3),
+        ("pad", C.c_int32),
+        ("batch_ms", C.c_double * 3),
+        ("batch_qps", C.c_double * 3),
+        ("reserved", C.c_int32 * 8),
+    ]
+
+    def as_dict(self) -> dict:
+        d = {}
+        for k, _ in self._fields_:
+            if k in ("reserved", "pad"):
+                continue
+            v = getattr(self, k)
+            d[k] = list(v) if hasattr(v, "__len__") else v
+        return d
+
+
+def autotune(engine: Engine, sources, latency_budget_ms: float = 25.0, bfs=True, sssp=True, batches=True) -> dict:
+    """sg_autotune: per-graph alpha/beta/block size and batch sizes by measurement."""
+    L = lib()
+    L.sg_tune_config_default.argtypes = [C.POINTER(TuneConfig)]
+    L.sg_tune_config_default.restype = None
+    L.sg_autotune.argtypes = [_P, _P, C.c_int32, C.POINTER(TuneConfig), C.POINTER(TuneResult)]
+    L.sg_autotune.restype = C.c_int
+    cfg = TuneConfig()
+    L.sg_tune_config_default(C.byref(cfg))
+    cfg.latency_budget_ms = latency_budget_ms
+    cfg.tune_bfs, cfg.tune_sssp, cfg.tune_batches = int(bfs), int(sssp), int(batches)
+    src = _u32(sources)
+    out = TuneResult()
+    _check(L.sg_autotune(engine.h, _ptr(src), len(src), C.byref(cfg), C.byref(out)))
+    return out.as_dict()
