@@ -368,7 +368,7 @@ int engine_persistent_grid(sg_engine *e, const void *fn, int bs, const std::func
             return SG_OK;
         }
     }
-    if (e->ngrids == 8) SG_FAIL(SG_ERR_STATE, "persistent grid: too many kernel configurations");
+    if (e->ngrids == sg_engine::kMaxGrids) SG_FAIL(SG_ERR_STATE, "persistent grid: too many kernel configurations");
     int per_cu = 0;
     SG_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, bs, 0));
     if (per_cu < 1) SG_FAIL(SG_ERR_HIP, "persistent grid: kernel cannot be resident");

@@ -149,7 +149,8 @@ struct sg_engine {
         // launches the kernel in residency-probe mode with a given grid
         std::function<int(int blocks, uint32_t *words)> probe;
     };
-    PGrid grids[8];
+    static constexpr int kMaxGrids = 32;
+    PGrid grids[kMaxGrids];
     int ngrids = 0;
     bool needs_bar_reset = false;
 
