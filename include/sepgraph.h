@@ -201,9 +201,12 @@ int sg_bfs(sg_engine *e, uint32_t source);
  * weight 1 on every edge (the reference's all-ones fallback,
  * include/groute/graphs/csr_graph.h:1840-1846). Direction per round follows
  * the same alpha/beta rule as BFS. With options.priority == SG_PRIO_LOW_HIGH
- * the two-level (near/far) schedule of SetTwoLevelBasedPriority is used, with
- * delta = options.priority_threshold, or the reference's heuristic
- * 32 * mean weight / mean degree (hybrid_sssp.cu:183-199) when that is 0.
+ * the two-level (near/far) schedule of SetTwoLevelBasedPriority is used
+ * (push rounds only): each round expands the frontier vertices whose distance
+ * lies in the lowest delta-wide bucket present and carries the others over,
+ * with delta = options.priority_threshold, or the reference's heuristic
+ * c * mean weight / mean degree (hybrid_sssp.cu:183-199, c = 64 lanes here
+ * for its 32-lane warp) when that is below 1. Results are identical either way.
  * Asynchronous. */
 int sg_sssp(sg_engine *e, uint32_t source);
 

@@ -34,7 +34,19 @@
 
 using namespace sg;
 
+#ifndef SG_PULL_STEPS
+#define SG_PULL_STEPS 1
+#endif
+// Minimum waves per SIMD the compiler must allow for k_bfs (0: no bound);
+// 8 caps registers at 64 so two 1024-thread blocks fit on a CU.
+#ifndef SG_BFS_WAVES
+#define SG_BFS_WAVES 0
+#endif
+
 namespace {
+
+// In-edges tested per lane per step of a pull walk (see pull_group).
+constexpr int kPullSteps = SG_PULL_STEPS;
 
 struct BfsArgs {
     DevGraph g;
@@ -148,23 +160,44 @@ __device__ __forceinline__ void pull_group(Round &R, const uint32_t (&wg)[G], co
     }
 #pragma unroll
     for (int i = 0; i < G; i++) {
+        // Each step tests up to kPullSteps x 64 in-edges of the concatenated
+        // remaining lists: all loads of a step are issued before any result
+        // is needed, so one memory latency covers kPullSteps x 64 tests.
         for (;;) {
             const uint32_t r = found[i] ? 0u : e[i] - b[i];
             const uint32_t incl = wave_incl_scan(r);
             const uint32_t total = shfl(incl, kWave - 1);
             if (total == 0) break;
             const uint32_t excl = incl - r;
-            const uint32_t j = (uint32_t)ln;
-            const int k = owner_lane(incl, j);
-            const uint32_t bk = shfl(b[i], k), xk = shfl(excl, k);
-            bool hit = false;
-            if (j < total) hit = in_frontier(R, g.csrc[bk + (j - xk)]);
-            const uint64_t H = ballot(hit);
-            const uint32_t lo = min(excl, 64u), hi = min(incl, 64u), cov = hi - lo;
-            if (cov) {
-                const uint64_t msk = (cov == 64 ? ~0ull : ((1ull << cov) - 1)) << lo;
-                if (H & msk) found[i] = true;
-                b[i] += cov;
+            uint32_t idx[kPullSteps];
+            bool valid[kPullSteps];
+#pragma unroll
+            for (int q = 0; q < kPullSteps; q++) {
+                const uint32_t j = (uint32_t)(q * kWave + ln);
+                valid[q] = j < total;
+                const int k = owner_lane(incl, valid[q] ? j : 0u);
+                idx[q] = shfl(b[i], k) + (j - shfl(excl, k));
+            }
+            uint32_t src[kPullSteps];
+#pragma unroll
+            for (int q = 0; q < kPullSteps; q++) src[q] = valid[q] ? g.csrc[idx[q]] : 0u;
+            uint64_t H[kPullSteps];
+#pragma unroll
+            for (int q = 0; q < kPullSteps; q++) H[q] = ballot(valid[q] && in_frontier(R, src[q]));
+            const uint32_t span = min(total, (uint32_t)(kPullSteps * kWave));
+            const uint32_t lo = min(excl, span), hi = min(incl, span);
+            if (hi > lo) {
+#pragma unroll
+                for (int q = 0; q < kPullSteps; q++) {
+                    const uint32_t s0 = (uint32_t)(q * kWave);
+                    const uint32_t a0 = max(lo, s0), a1 = min(hi, s0 + kWave);
+                    if (a1 > a0) {
+                        const uint32_t c = a1 - a0;
+                        const uint64_t msk = (c == 64 ? ~0ull : ((1ull << c) - 1)) << (a0 - s0);
+                        if (H[q] & msk) found[i] = true;
+                    }
+                }
+                b[i] += hi - lo;
             }
         }
     }
@@ -186,8 +219,14 @@ __device__ __forceinline__ void pull_group(Round &R, const uint32_t (&wg)[G], co
     }
 }
 
+#if SG_BFS_WAVES
+#define SG_BFS_BOUNDS(BS) __launch_bounds__(BS, SG_BFS_WAVES)
+#else
+#define SG_BFS_BOUNDS(BS) __launch_bounds__(BS)
+#endif
+
 template <int BS>
-__global__ __launch_bounds__(BS) void k_bfs(BfsArgs a)
+__global__ SG_BFS_BOUNDS(BS) void k_bfs(BfsArgs a)
 {
     if (a.probe) {
         residency_probe(a.bar.words, 1u << 20);

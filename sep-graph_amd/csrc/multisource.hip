@@ -709,49 +709,52 @@ struct MsRelax {
     uint64_t *Fn;
     uint32_t *ring_next;
     uint32_t *cnt_next;
+    uint32_t qbits;  // queries of the batch fit in 2^qbits lanes
     unsigned long long scout, count;
 };
 
-// Relax edge u->v of weight w for the queries in `mask` (lane q = query q);
-// du is this lane's dist[u][q]. Whole wave, uniform v/w/mask.
-__device__ __forceinline__ void ms_relax_edge(MsRelax &R, uint32_t v, uint32_t w, uint64_t mask, uint32_t du)
-{
-    const MsSsspArgs &a = *R.a;
-    const int ln = lane();
-    bool imp = false;
-    if ((mask >> ln) & 1) {
-        const uint32_t nd = du > 0xFFFFFFFFu - w ? 0xFFFFFFFFu : du + w;
-        uint32_t *p = &a.dist[(size_t)v * 64 + ln];
-        if (nd < *p) imp = nd < atomicMin(p, nd);
-    }
-    const uint64_t im = ballot(imp);
-    if (im && ln == first_lane(im)) {
-        const unsigned long long old = atomicOr((unsigned long long *)&R.Fn[v], (unsigned long long)im);
-        if (old == 0) {
-            const uint32_t pos = atomicAdd(R.cnt_next, 1u);
-            if (pos < a.g.n) R.ring_next[pos] = v;
-            else a.ctl[1] = 1;
-            R.scout += a.g.row[v + 1] - a.g.row[v];
-            R.count++;
-        }
-    }
-}
-
-// Relax the out-edges [b, e) of u for the queries in `mask`.
+// Relax the out-edges [b, e) of u for the queries in `mask`. The wave is
+// split into 64 / 2^qbits groups of 2^qbits lanes (lane q of a group = query
+// q) and each group relaxes one edge per step, so a batch of B queries keeps
+// about 64 / B edges in flight per wave. A group's leader publishes the
+// queries whose distance it lowered into the next round's mask of the edge's
+// head, and queues the head when it is the first to do so.
 __device__ __forceinline__ void ms_relax_range(MsRelax &R, uint32_t u, uint32_t b, uint32_t e, uint64_t mask)
 {
-    const DevGraph &g = R.a->g;
+    const MsSsspArgs &a = *R.a;
+    const DevGraph &g = a.g;
     const int ln = lane();
-    const uint32_t du = R.a->dist[(size_t)u * 64 + ln];
-    for (uint32_t e0 = b; e0 < e; e0 += kWave) {
-        const uint32_t ed = e0 + ln;
-        uint32_t v = 0, w = 0;
+    const uint32_t qb = R.qbits;
+    const uint32_t q = (uint32_t)ln & ((1u << qb) - 1), grp = (uint32_t)ln >> qb, ngrp = 64u >> qb;
+    const uint64_t gmask = qb == 6 ? ~0ull : ((1ull << (1u << qb)) - 1);
+    const bool mine = (mask >> q) & 1;
+    const uint32_t du = mine ? a.dist[(size_t)u * 64 + q] : 0xFFFFFFFFu;
+    for (uint32_t e0 = b; e0 < e; e0 += ngrp) {
+        const uint32_t ed = e0 + grp;
+        bool imp = false;
+        uint32_t v = 0;
         if (ed < e) {
             v = g.col[ed];
-            w = g.w ? g.w[ed] : 1u;
+            if (mine) {
+                const uint32_t w = g.w ? g.w[ed] : 1u;
+                const uint32_t nd = du > 0xFFFFFFFFu - w ? 0xFFFFFFFFu : du + w;
+                uint32_t *p = &a.dist[(size_t)v * 64 + q];
+                if (nd < *p) imp = nd < atomicMin(p, nd);
+            }
         }
-        const uint32_t cnt = min(e - e0, (uint32_t)kWave);
-        for (uint32_t j = 0; j < cnt; j++) ms_relax_edge(R, shfl(v, (int)j), shfl(w, (int)j), mask, du);
+        const uint64_t im = ballot(imp);
+        if (!im) continue;
+        const uint64_t gm = (im >> (grp << qb)) & gmask;
+        if (q == 0 && gm) {
+            const unsigned long long old = atomicOr((unsigned long long *)&R.Fn[v], (unsigned long long)gm);
+            if (old == 0) {
+                const uint32_t pos = atomicAdd(R.cnt_next, 1u);
+                if (pos < g.n) R.ring_next[pos] = v;
+                else a.ctl[1] = 1;
+                R.scout += g.row[v + 1] - g.row[v];
+                R.count++;
+            }
+        }
     }
 }
 
@@ -773,6 +776,8 @@ __global__ __launch_bounds__(BS) void k_mssssp(MsSsspArgs a)
     const uint32_t xcc = xcc_id();
     uint32_t gen = threadIdx.x == 0 ? grid_gen(a.bar) : 0u;
     const long long t_start = wall_clock64();
+    uint32_t qbits = 0;
+    while ((1u << qbits) < a.nsrc) qbits++;
 
     __shared__ unsigned long long s_red[2][WPB];
     __shared__ unsigned long long s_lvl[5];
@@ -839,7 +844,8 @@ __global__ __launch_bounds__(BS) void k_mssssp(MsSsspArgs a)
         const uint32_t *old = a.ring + (size_t)((r + 2) % 3) * n;
         for (uint32_t i = gtid; i < prev_len; i += nthreads) Ff[old[i]] = 0;
 
-        MsRelax R{&a, a.F + (size_t)((r + 1) % 3) * n, a.ring + (size_t)((r + 1) % 3) * n, &a.ctl[2 + (r + 1) % 3], 0, 0};
+        MsRelax R{&a, a.F + (size_t)((r + 1) % 3) * n, a.ring + (size_t)((r + 1) % 3) * n, &a.ctl[2 + (r + 1) % 3],
+                  qbits, 0, 0};
         // hubs through the static chunk list
         for (uint32_t c = gwave; c < g.nhuge; c += nwaves) {
             const uint2 dsc = g.huge[c];

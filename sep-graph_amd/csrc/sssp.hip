@@ -46,6 +46,7 @@ struct SsspArgs {
     uint32_t max_rounds;
     uint32_t probe;   // launch is a residency probe of the grid
     uint32_t *exits;  // per-block exit records (diagnostics)
+    uint32_t delta;   // two-level priority bucket width (0: off)
 };
 
 struct Round {
@@ -57,7 +58,13 @@ struct Round {
     Stager S;
     unsigned long long scout, count;
     bool huge;
+    uint32_t thr;   // entries with dist >= thr are deferred (priority mode)
+    uint32_t minv;  // smallest distance this lane queued for the next round
 };
+
+// The smallest distance queued for a round is kept complemented in the
+// counter slot's first pad word, so that zeroed slots read as "none".
+__device__ __forceinline__ uint32_t &slot_cmin(FrontierCtr *c) { return c->pad[0]; }
 
 __device__ __forceinline__ uint32_t sat_add(uint32_t d, uint32_t w) { return d > kInf - w ? kInf : d + w; }
 
@@ -75,16 +82,39 @@ __device__ __forceinline__ void relax(Round &R, bool act, uint32_t v, uint32_t n
     bool queued = false;
     if (act && nd < a.dist[v]) {
         const uint32_t old = atomicMin(&a.dist[v], nd);
-        if (nd < old && a.stamp[v] != R.L + 2) {
-            a.stamp[v] = R.L + 2;
-            const uint32_t dv = a.g.row[v + 1] - a.g.row[v];
+        if (nd < old) {
+            R.minv = min(R.minv, nd);
+            if (a.stamp[v] != R.L + 2) {
+                a.stamp[v] = R.L + 2;
+                const uint32_t dv = a.g.row[v + 1] - a.g.row[v];
+                R.scout += dv;
+                R.count++;
+                R.huge |= dv > kHugeDegree;
+                queued = true;
+            }
+        }
+    }
+    stage(R.S, queued, v);
+}
+
+// Priority mode: carry a frontier vertex whose distance is not below the
+// threshold into the next round's frontier instead of expanding it.
+__device__ __forceinline__ void defer(Round &R, bool d, uint32_t u, uint32_t du)
+{
+    const SsspArgs &a = *R.a;
+    bool queued = false;
+    if (d) {
+        R.minv = min(R.minv, du);
+        if (a.stamp[u] != R.L + 2) {
+            a.stamp[u] = R.L + 2;
+            const uint32_t dv = a.g.row[u + 1] - a.g.row[u];
             R.scout += dv;
             R.count++;
             R.huge |= dv > kHugeDegree;
             queued = true;
         }
     }
-    stage(R.S, queued, v);
+    stage(R.S, queued, u);
 }
 
 // Relax the out-edges of up to 64 queued vertices (one per lane) with
@@ -93,13 +123,19 @@ __device__ __forceinline__ void push_expand(Round &R, bool valid, uint32_t u)
 {
     const DevGraph &g = R.a->g;
     uint32_t b = 0, d = 0, du = kInf;
+    bool later = false;
     if (valid) {
         b = g.row[u];
         d = g.row[u + 1] - b;
         if (d > kHugeDegree) d = 0;
         du = R.a->dist[u];
         if (du == kInf) d = 0;
+        if (du != kInf && du >= R.thr) {
+            later = true;
+            d = 0;
+        }
     }
+    defer(R, later, u, du);
     const uint32_t incl = wave_incl_scan(d);
     const uint32_t total = shfl(incl, kWave - 1);
     const uint32_t off = b - (incl - d);
@@ -245,7 +281,7 @@ __global__ __launch_bounds__(BS) void k_sssp(SsspArgs a)
     __shared__ unsigned long long s_red[2][WPB];
     __shared__ unsigned long long s_lvl[5];
     __shared__ uint32_t s_q[kLdsQ];
-    __shared__ uint32_t s_qn, s_flush[2];
+    __shared__ uint32_t s_qn, s_flush[2], s_cmin, s_bmin;
 
     const uint32_t src = a.source;
     for (uint32_t i = gtid; i < n; i += nthreads) {
@@ -266,6 +302,7 @@ __global__ __launch_bounds__(BS) void k_sssp(SsspArgs a)
             a.ctr[0].has_huge = deg > kHugeDegree;
             a.ctr[0].stripe[0].scout = deg;
             a.ctr[0].stripe[0].count = 1;
+            slot_cmin(&a.ctr[0]) = ~0u;  // smallest queued distance: 0
         }
     }
     bool ok = grid_sync(a.bar, xcc, gen, a.abort);
@@ -278,9 +315,18 @@ __global__ __launch_bounds__(BS) void k_sssp(SsspArgs a)
     unsigned long long last_count = ~0ull;
     while (ok) {
         read_counts(&a.ctr[L % 3], s_lvl);
-        if (threadIdx.x == 0) s_qn = 0;
+        if (threadIdx.x == 0) {
+            s_qn = 0;
+            s_cmin = __hip_atomic_load(&slot_cmin(&a.ctr[L % 3]), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
         __syncthreads();
         const RoundCounts rc = unpack_counts(s_lvl, a.qcap);
+        // priority mode: expand only the lowest delta-bucket present
+        uint32_t thr = kInf;
+        if (a.delta) {
+            const uint64_t t = ((uint64_t)(~s_cmin) / a.delta + 1) * a.delta;
+            thr = (uint32_t)min(t, (uint64_t)kInf);
+        }
         __syncthreads();
         last_count = rc.count;
         if (rc.count == 0) {
@@ -315,6 +361,8 @@ __global__ __launch_bounds__(BS) void k_sssp(SsspArgs a)
         R.scout = 0;
         R.count = 0;
         R.huge = false;
+        R.thr = thr;
+        R.minv = kInf;
         const uint32_t *qin = a.queue + (size_t)(L % 3) * a.qcap;
 
         if (!pull) {
@@ -334,6 +382,8 @@ __global__ __launch_bounds__(BS) void k_sssp(SsspArgs a)
                     const uint32_t u = shfl(dsc.x, t), beg = shfl(dsc.y, t);
                     const uint32_t end = min(g.row[u + 1], beg + kHugeChunk);
                     const uint32_t du = a.dist[u];
+                    // a deferred hub is carried by its queue entry
+                    if (du >= thr) continue;
                     for (uint32_t e0 = beg; e0 < end; e0 += kWave) {
                         const uint32_t e = e0 + ln;
                         const bool act = e < end && du != kInf;
@@ -391,6 +441,17 @@ __global__ __launch_bounds__(BS) void k_sssp(SsspArgs a)
         }
 
         publish_counts<WPB>(R.S.nxt, R.scout, R.count, R.huge, s_red);
+        if (a.delta) {
+            // block minimum of the queued distances, one atomic per block
+            uint32_t m = R.minv;
+#pragma unroll
+            for (int d = kWave / 2; d > 0; d >>= 1) m = min(m, (uint32_t)__shfl_xor(m, d, kWave));
+            if (threadIdx.x == 0) s_bmin = kInf;
+            __syncthreads();
+            if (ln == 0 && m != kInf) atomicMin(&s_bmin, m);
+            __syncthreads();
+            if (threadIdx.x == 0 && s_bmin != kInf) atomicMax(&slot_cmin(R.S.nxt), ~s_bmin);
+        }
         pol.pull_prev = pull;
         L++;
         ok = grid_sync(a.bar, xcc, gen, a.abort);
@@ -460,6 +521,20 @@ extern "C" int sg_sssp(sg_engine *e, uint32_t source)
     a.beta = o.beta;
     a.total_workload0 = (uint32_t)((double)d.m * o.edge_factor);
     a.max_rounds = (uint32_t)o.max_iteration;
+    if (o.priority == SG_PRIO_LOW_HIGH) {
+        // two-level priority (the reference's SetTwoLevelBasedPriority,
+        // algo_variants.cuh:605-685) runs push rounds only; the bucket width
+        // is the option's threshold, or c * mean weight / mean degree with c
+        // the wave width (hybrid_sssp.cu:183-199 uses the 32-lane warp)
+        double dl = o.priority_threshold;
+        if (!(dl >= 1.0)) {
+            const double avg_deg = d.n ? (double)d.m / d.n : 1.0;
+            const double avg_w = e->g->weighted ? e->g->avg_weight : 1.0;
+            dl = avg_deg > 0 ? kWave * avg_w / avg_deg : 1.0;
+        }
+        a.delta = (uint32_t)std::max(1.0, std::min(dl, 2147483647.0));
+        a.force = 0;
+    }
     e->last_algo = 1;
     const int bs = o.block_size ? o.block_size : 1024;
     switch (bs) {

@@ -137,3 +137,21 @@ def test_bfs_and_sssp_share_an_engine(sg, orc, gpu):
         assert np.array_equal(e.gather(np.uint32), orc.sssp_dijkstra(hg.row, hg.col, hg.w, s))
         e.bfs(s)
         assert np.array_equal(e.gather(np.int32), orc.bfs(hg.row, hg.col, s))
+
+
+@pytest.mark.parametrize("delta", [0.0, 1.0, 37.0, 5000.0])
+def test_sssp_two_level_priority(sg, orc, gpu, delta):
+    """The near/far schedule (SetTwoLevelBasedPriority) changes the order of
+    work, never the distances."""
+    hg = sg.rmat_graph(14, 16, symmetrize=False, seed=1, weights=(1, 64))
+    g = sg.Graph(hg)
+    e = sg.Engine(g, priority=1, priority_threshold=delta)
+    for s in sg.pick_sources(hg.row, 4, seed=3):
+        e.sssp(int(s))
+        assert np.array_equal(e.gather(np.uint32), orc.sssp_dijkstra(hg.row, hg.col, hg.w, int(s)))
+        assert e.stats()["pull_rounds"] == 0
+    gg = sg.grid_graph(96, 96, 0.1, seed=1, weights=(1, 1000))
+    e2 = sg.Engine(sg.Graph(gg), priority=1, priority_threshold=delta)
+    for s in (0, 4000):
+        e2.sssp(s)
+        assert np.array_equal(e2.gather(np.uint32), orc.sssp_dijkstra(gg.row, gg.col, gg.w, s))
