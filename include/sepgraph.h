@@ -154,7 +154,10 @@ typedef struct sg_engine_options {
     int32_t block_size;         /* threads per workgroup for traversal kernels; 0 = tuned */
     int32_t large_degree;       /* out-degree above which a push vertex is split across
                                    the grid; 0 = tuned */
-    int32_t reserved[8];
+    int32_t grid_blocks;        /* cap on the persistent grid's workgroups (0: fill the
+                                   GPU); small grids make rounds cheaper on high-diameter
+                                   graphs whose frontiers stay small */
+    int32_t reserved[7];
 } sg_engine_options;
 
 void sg_engine_options_default(sg_engine_options *opt);
@@ -331,6 +334,9 @@ typedef struct sg_tune_result {
     int32_t bfs_block_size;
     int32_t sssp_block_size;
     double sssp_alpha;
+    int32_t bfs_grid_blocks, sssp_grid_blocks;  /* 0: the full GPU */
+    int32_t sssp_priority;                      /* SG_PRIO_NONE or SG_PRIO_LOW_HIGH */
+    int32_t pad0;
     double bfs_ms, bfs_default_ms;    /* mean per query: tuned / alpha 15, beta 18, 1024 threads */
     double sssp_ms, sssp_default_ms;
     int32_t max_batch[3];             /* per SG_QUERY_* kind */

@@ -310,6 +310,8 @@ extern "C" int sg_engine_set_options(sg_engine *e, const sg_engine_options *o)
     if (o->block_size != 0 && (o->block_size % 64 || o->block_size < 64 || o->block_size > 1024))
         SG_FAIL(SG_ERR_INVALID, "sg_engine_set_options: block_size must be a multiple of 64 in [64, 1024]");
     if (o->max_iteration < 0) SG_FAIL(SG_ERR_INVALID, "sg_engine_set_options: max_iteration < 0");
+    if (o->grid_blocks < 0 || o->grid_blocks > (int)kMaxBlocks)
+        SG_FAIL(SG_ERR_INVALID, "sg_engine_set_options: grid_blocks must be in [0, 8192]");
     e->opt = *o;
     return SG_OK;
 }
@@ -340,6 +342,15 @@ static int probe_grid(sg_engine *e, sg_engine::PGrid &pg, bool *resident)
 
 static int size_grid(sg_engine *e, sg_engine::PGrid &pg, int per_cu)
 {
+    if (pg.cap > 0 && pg.cap < per_cu * e->g->num_cus) {
+        // a capped grid: the cap itself when it is co-resident
+        pg.blocks = pg.cap;
+        bool resident = false;
+        int rc = probe_grid(e, pg, &resident);
+        if (rc) return rc;
+        if (resident) return SG_OK;
+        per_cu = std::max(1, pg.cap / e->g->num_cus);
+    }
     for (int pc = per_cu; pc >= 1; pc--) {
         pg.blocks = pc * e->g->num_cus;
         bool resident = false;
@@ -356,14 +367,16 @@ int engine_persistent_grid(sg_engine *e, const void *fn, int bs, const std::func
     if (e->needs_bar_reset) {
         // an aborted run leaves counters mid-way: zero and re-probe every grid
         for (int i = 0; i < e->ngrids; i++) {
-            int rc = size_grid(e, e->grids[i], e->grids[i].blocks / e->g->num_cus);
+            sg_engine::PGrid &pg = e->grids[i];
+            int rc = size_grid(e, pg, std::max(1, pg.blocks / e->g->num_cus));
             if (rc) return rc;
         }
         SG_HIP(hipMemsetAsync(e->abort_flag, 0, 4, e->stream));
         e->needs_bar_reset = false;
     }
+    const int cap = std::max(0, e->opt.grid_blocks);
     for (int i = 0; i < e->ngrids; i++) {
-        if (e->grids[i].fn == fn && e->grids[i].bs == bs) {
+        if (e->grids[i].fn == fn && e->grids[i].bs == bs && e->grids[i].cap == cap) {
             *out = &e->grids[i];
             return SG_OK;
         }
@@ -375,6 +388,7 @@ int engine_persistent_grid(sg_engine *e, const void *fn, int bs, const std::func
     sg_engine::PGrid pg;
     pg.fn = fn;
     pg.bs = bs;
+    pg.cap = cap;
     pg.probe = probe;
     hipError_t err = hipMalloc((void **)&pg.words, kBarWords * 4);
     if (err != hipSuccess) {

@@ -142,6 +142,7 @@ struct sg_engine {
     struct PGrid {
         const void *fn = nullptr;
         int bs = 0;
+        int cap = 0;       // sg_engine_options.grid_blocks it was sized for
         int blocks = 0;
         uint32_t nxcd = 0;
         uint32_t cnt[sg::kMaxXcd] = {};   // blocks per XCD from the probe

@@ -18,10 +18,6 @@
 
 namespace {
 
-struct Saved {
-    sg_engine_options o;
-};
-
 // Mean kernel time (ms) of `run` over the sample sources; < 0 on error.
 template <typename F>
 double mean_ms(sg_engine *e, const uint32_t *src, int k, F run)
@@ -85,23 +81,70 @@ extern "C" int sg_autotune(sg_engine *e, const uint32_t *sources, int32_t k, con
                 }
         out->bfs_ms = best;
     }
+    if (c.tune_bfs) {
+        // then the grid size for the best BFS configuration
+        for (int gb : {128, 32}) {
+            sg_engine_options o = base;
+            o.alpha = out->bfs_alpha;
+            o.beta = out->bfs_beta;
+            o.block_size = out->bfs_block_size;
+            o.grid_blocks = gb;
+            if ((rc = sg_engine_set_options(e, &o))) return rc;
+            const double t = mean_ms(e, sources, k, [&](uint32_t s) { return sg_bfs(e, s); });
+            if (t < 0) return SG_ERR_HIP;
+            if (t < out->bfs_ms) {
+                out->bfs_ms = t;
+                out->bfs_grid_blocks = gb;
+            }
+        }
+    }
     if (c.tune_sssp) {
+        // SSSP queries cost ~20x a BFS: a staged search on up to 4 sources
+        const int ks = std::min(k, 4);
+        auto time_sssp = [&](const sg_engine_options &o) -> double {
+            if (sg_engine_set_options(e, &o) != SG_OK) return -1.0;
+            return mean_ms(e, sources, ks, [&](uint32_t s) { return sg_sssp(e, s); });
+        };
         double best = -1.0;
-        for (int bs : blocks)
-            for (double a : alphas) {
+        sg_engine_options bo = base;
+        for (int prio : {SG_PRIO_NONE, SG_PRIO_LOW_HIGH})
+            for (int bs : {512, 1024}) {
                 sg_engine_options o = base;
-                o.alpha = a;
+                o.priority = prio;
                 o.block_size = bs;
-                if ((rc = sg_engine_set_options(e, &o))) return rc;
-                const double t = mean_ms(e, sources, std::min(k, 4), [&](uint32_t s) { return sg_sssp(e, s); });
+                const double t = time_sssp(o);
                 if (t < 0) return SG_ERR_HIP;
+                if (prio == SG_PRIO_NONE && bs == 1024) out->sssp_default_ms = t;
                 if (best < 0 || t < best) {
                     best = t;
-                    out->sssp_alpha = a;
-                    out->sssp_block_size = bs;
+                    bo = o;
                 }
-                if (a == 15 && bs == 1024) out->sssp_default_ms = t;
             }
+        if (bo.priority == SG_PRIO_NONE)
+            for (double a : {4.0, 30.0}) {
+                sg_engine_options o = bo;
+                o.alpha = a;
+                const double t = time_sssp(o);
+                if (t < 0) return SG_ERR_HIP;
+                if (t < best) {
+                    best = t;
+                    bo = o;
+                }
+            }
+        for (int gb : {128, 32}) {
+            sg_engine_options o = bo;
+            o.grid_blocks = gb;
+            const double t = time_sssp(o);
+            if (t < 0) return SG_ERR_HIP;
+            if (t < best) {
+                best = t;
+                bo = o;
+            }
+        }
+        out->sssp_alpha = bo.alpha;
+        out->sssp_block_size = bo.block_size;
+        out->sssp_priority = bo.priority;
+        out->sssp_grid_blocks = bo.grid_blocks;
         out->sssp_ms = best;
     }
     if (c.tune_batches) {
@@ -142,6 +185,7 @@ extern "C" int sg_autotune(sg_engine *e, const uint32_t *sources, int32_t k, con
         o.alpha = out->bfs_alpha;
         o.beta = out->bfs_beta;
         o.block_size = out->bfs_block_size;
+        o.grid_blocks = out->bfs_grid_blocks;
     }
     return sg_engine_set_options(e, &o);
 }

@@ -70,7 +70,8 @@ class EngineOptions(C.Structure):
         ("max_iteration", C.c_int32),
         ("block_size", C.c_int32),
         ("large_degree", C.c_int32),
-        ("reserved", C.c_int32 * 8),
+        ("grid_blocks", C.c_int32),
+        ("reserved", C.c_int32 * 7),
     ]
 
 
@@ -549,6 +550,10 @@ class TuneResult(C.Structure):
         ("bfs_block_size", C.c_int32),
         ("sssp_block_size", C.c_int32),
         ("sssp_alpha", C.c_double),
+        ("bfs_grid_blocks", C.c_int32),
+        ("sssp_grid_blocks", C.c_int32),
+        ("sssp_priority", C.c_int32),
+        ("pad0", C.c_int32),
         ("bfs_ms", C.c_double),
         ("bfs_default_ms", C.c_double),
         ("sssp_ms", C.c_double),
