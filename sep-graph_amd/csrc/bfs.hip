@@ -17,25 +17,29 @@
 //     writers store the same value (the reference's non-atomic BFS min,
 //     hybrid_bfs.cu:128-134, on by default, hybrid_bfs/main.cpp:37), so a
 //     vertex may enter the next queue more than once, exactly as in the
-//     reference, and frontier membership is level[u] == L;
+//     reference; a discovery also sets the vertex's bit in the next
+//     frontier's bitmap (one atomicOr per discovery, none per edge), so every
+//     frontier is available as 512 KiB of L2-resident bits;
 //   * the next queue is staged in LDS per block and published with one
 //     atomic per block per round (same-address atomics serialise on MI355X);
 //   * vertices with out-degree > kHugeDegree are expanded through the graph's
 //     static chunk list, filtered by membership, so hub adjacencies spread
 //     over every wave without building any list at run time.
-// A pull round sweeps 64-vertex words (each wave owning a contiguous range);
-// every unvisited lane first tests its first in-edge (four words' tests in
-// flight at once), then the wave walks the concatenated remaining in-edge
-// lists 64 edges per step, re-planning after each step so a vertex stops at
-// its first parent (the reference's early break, pull_functor.h:138-188).
-// Pull rounds write whole words of the next frontier bitmap and of a visited
-// filter, so they need no atomics either.
+// A pull round sweeps 64-vertex words (each wave owning a contiguous range,
+// kPullGroup words at a time): every unvisited lane tests its first
+// in-neighbour from the graph's head array, then the group's words walk their
+// remaining in-edge lists in lock step, 64 edges per word per step with all
+// of the group's loads in flight together, re-planning after each step so a
+// vertex stops at its first parent (the reference's early break,
+// pull_functor.h:138-188). Pull rounds write whole words of the next frontier
+// bitmap and of a visited filter, so they need no atomics.
 #include "frontier.hpp"
 
 using namespace sg;
 
-#ifndef SG_PULL_STEPS
-#define SG_PULL_STEPS 1
+// Words of 64 vertices a wave works on at once in a pull round.
+#ifndef SG_PULL_GROUP
+#define SG_PULL_GROUP 4
 #endif
 // Minimum waves per SIMD the compiler must allow for k_bfs (0: no bound);
 // 8 caps registers at 64 so two 1024-thread blocks fit on a CU.
@@ -45,14 +49,13 @@ using namespace sg;
 
 namespace {
 
-// In-edges tested per lane per step of a pull walk (see pull_group).
-constexpr int kPullSteps = SG_PULL_STEPS;
+constexpr int kPullGroup = SG_PULL_GROUP;
 
 struct BfsArgs {
     DevGraph g;
     int32_t *level;
     uint64_t *vis;    // W words: visited filter kept by pull rounds (set bit => visited)
-    uint64_t *fbit;   // 2 * W words: frontier bitmaps written by pull rounds
+    uint64_t *fbit;   // 3 * W words: frontier L's bitmap in slot L % 3
     uint32_t *queue;  // 3 * qcap entries
     uint32_t qcap;
     FrontierCtr *ctr; // 3 slots
@@ -71,18 +74,14 @@ struct BfsArgs {
 struct Round {
     const BfsArgs *a;
     int32_t L;              // level being expanded
-    bool fbit_valid;        // frontier L is given by fcur (it came from a pull round)
-    const uint64_t *fcur;
-    uint64_t *fnext;
+    const uint64_t *fcur;   // frontier L as a bitmap
+    uint64_t *fnext;        // frontier L+1, built by this round
     Stager S;
     unsigned long long scout, count;  // this lane's share of the next frontier
     bool huge;                        // this lane discovered a vertex above kHugeDegree
 };
 
-__device__ __forceinline__ bool in_frontier(const Round &R, uint32_t u)
-{
-    return R.fbit_valid ? test_bit(R.fcur, u) : R.a->level[u] == R.L;
-}
+__device__ __forceinline__ bool in_frontier(const Round &R, uint32_t u) { return test_bit(R.fcur, u); }
 
 // Push edge -> v for `act` lanes; wave-convergent.
 __device__ __forceinline__ void push_visit(Round &R, bool act, uint32_t v)
@@ -91,6 +90,10 @@ __device__ __forceinline__ void push_visit(Round &R, bool act, uint32_t v)
     bool found = false;
     if (act && a.level[v] == -1) {
         a.level[v] = R.L + 1;
+        // the next frontier's bitmap, so that a pull round after this one
+        // tests membership in 512 KiB of L2-resident bits instead of the
+        // level array (one atomic per discovery, none per edge)
+        atomicOr((unsigned long long *)&R.fnext[v >> 6], 1ull << (v & 63));
         const uint32_t dv = a.g.row[v + 1] - a.g.row[v];
         R.scout += dv;
         R.count++;
@@ -127,8 +130,8 @@ __device__ __forceinline__ void push_expand(Round &R, bool valid, uint32_t u)
 }
 
 // Bottom-up step for the 64 vertices of each of ng (<= G) words wg[i] (vwg[i]:
-// visited filter word). The first-edge tests of all G words are issued
-// together so their dependent loads overlap; the chunked walks follow per word.
+// visited filter word). Every load of a stage is issued for all G words
+// before any is used, so the words' dependent chains overlap.
 template <int G>
 __device__ __forceinline__ void pull_group(Round &R, const uint32_t (&wg)[G], const uint64_t (&vwg)[G], int ng)
 {
@@ -144,60 +147,61 @@ __device__ __forceinline__ void pull_group(Round &R, const uint32_t (&wg)[G], co
         v[i] = wg[i] * 64 + ln;
         lv[i] = (i < ng && v[i] < g.n) ? a.level[v[i]] : 0;
     }
+    bool want[G];
+    uint32_t dv[G];
+    // One step issues, for every unvisited lane, the first in-neighbour (from
+    // the head array, coalesced across the word), the in-edge range and the
+    // out-degree, so the parent test, the walk and the frontier statistics
+    // wait on a single round of loads.
 #pragma unroll
     for (int i = 0; i < G; i++) {
         visited[i] = i < ng ? (ballot(v[i] >= g.n || lv[i] != -1) | vwg[i]) : ~0ull;
-        const bool want = !((visited[i] >> ln) & 1);
-        b[i] = want ? g.cstart[v[i]] : 0u;
-        e[i] = want ? g.cstart[v[i] + 1] : 0u;
-    }
-#pragma unroll
-    for (int i = 0; i < G; i++) u[i] = b[i] < e[i] ? g.csrc[b[i]] : 0u;
-#pragma unroll
-    for (int i = 0; i < G; i++) {
-        found[i] = b[i] < e[i] && in_frontier(R, u[i]);
-        if (b[i] < e[i]) b[i]++;
+        want[i] = !((visited[i] >> ln) & 1);
+        u[i] = want[i] ? g.chead[v[i]] : 0xFFFFFFFFu;
+        b[i] = want[i] ? g.cstart[v[i]] : 0u;
+        e[i] = want[i] ? g.cstart[v[i] + 1] : 0u;
+        dv[i] = want[i] ? g.row[v[i] + 1] - g.row[v[i]] : 0u;
     }
 #pragma unroll
     for (int i = 0; i < G; i++) {
-        // Each step tests up to kPullSteps x 64 in-edges of the concatenated
-        // remaining lists: all loads of a step are issued before any result
-        // is needed, so one memory latency covers kPullSteps x 64 tests.
-        for (;;) {
+        found[i] = u[i] != 0xFFFFFFFFu && in_frontier(R, u[i]);
+        // lanes still looking walk their in-edges past the head
+        if (found[i] || u[i] == 0xFFFFFFFFu) b[i] = e[i];
+        else b[i]++;
+    }
+    // Walk the remaining in-edge lists of every word of the group in lock
+    // step: each iteration plans one 64-edge step per word (the concatenated
+    // lists of its still-searching lanes), then issues all of the group's
+    // loads before any test, so one memory latency covers G steps. A lane
+    // stops at its first parent (re-planned every step).
+    for (;;) {
+        uint32_t idx[G], incl[G], excl[G];
+        bool valid[G];
+        bool any = false;
+#pragma unroll
+        for (int i = 0; i < G; i++) {
             const uint32_t r = found[i] ? 0u : e[i] - b[i];
-            const uint32_t incl = wave_incl_scan(r);
-            const uint32_t total = shfl(incl, kWave - 1);
-            if (total == 0) break;
-            const uint32_t excl = incl - r;
-            uint32_t idx[kPullSteps];
-            bool valid[kPullSteps];
+            incl[i] = wave_incl_scan(r);
+            excl[i] = incl[i] - r;
+            const uint32_t total = shfl(incl[i], kWave - 1);
+            any |= total != 0;
+            valid[i] = (uint32_t)ln < total;
+            const int k = owner_lane(incl[i], valid[i] ? (uint32_t)ln : 0u);
+            idx[i] = shfl(b[i], k) + ((uint32_t)ln - shfl(excl[i], k));
+        }
+        if (!any) break;
+        uint32_t src[G];
 #pragma unroll
-            for (int q = 0; q < kPullSteps; q++) {
-                const uint32_t j = (uint32_t)(q * kWave + ln);
-                valid[q] = j < total;
-                const int k = owner_lane(incl, valid[q] ? j : 0u);
-                idx[q] = shfl(b[i], k) + (j - shfl(excl, k));
-            }
-            uint32_t src[kPullSteps];
+        for (int i = 0; i < G; i++) src[i] = valid[i] ? g.csrc[idx[i]] : 0u;
 #pragma unroll
-            for (int q = 0; q < kPullSteps; q++) src[q] = valid[q] ? g.csrc[idx[q]] : 0u;
-            uint64_t H[kPullSteps];
-#pragma unroll
-            for (int q = 0; q < kPullSteps; q++) H[q] = ballot(valid[q] && in_frontier(R, src[q]));
-            const uint32_t span = min(total, (uint32_t)(kPullSteps * kWave));
-            const uint32_t lo = min(excl, span), hi = min(incl, span);
+        for (int i = 0; i < G; i++) {
+            const uint64_t H = ballot(valid[i] && in_frontier(R, src[i]));
+            const uint32_t lo = min(excl[i], 64u), hi = min(incl[i], 64u);
             if (hi > lo) {
-#pragma unroll
-                for (int q = 0; q < kPullSteps; q++) {
-                    const uint32_t s0 = (uint32_t)(q * kWave);
-                    const uint32_t a0 = max(lo, s0), a1 = min(hi, s0 + kWave);
-                    if (a1 > a0) {
-                        const uint32_t c = a1 - a0;
-                        const uint64_t msk = (c == 64 ? ~0ull : ((1ull << c) - 1)) << (a0 - s0);
-                        if (H[q] & msk) found[i] = true;
-                    }
-                }
-                b[i] += hi - lo;
+                const uint32_t c = hi - lo;
+                const uint64_t msk = (c == 64 ? ~0ull : ((1ull << c) - 1)) << lo;
+                if (H & msk) found[i] = true;
+                b[i] += c;
             }
         }
     }
@@ -212,9 +216,8 @@ __device__ __forceinline__ void pull_group(Round &R, const uint32_t (&wg)[G], co
         }
         if (found[i]) {
             a.level[v[i]] = R.L + 1;
-            const uint32_t dv = g.row[v[i] + 1] - g.row[v[i]];
-            R.scout += dv;
-            R.huge |= dv > kHugeDegree;
+            R.scout += dv[i];
+            R.huge |= dv[i] > kHugeDegree;
         }
     }
 }
@@ -259,6 +262,8 @@ __global__ SG_BFS_BOUNDS(BS) void k_bfs(BfsArgs a)
     for (uint32_t w = gtid; w < W; w += nthreads) {
         const uint64_t sb = (w == (src >> 6)) ? (1ull << (src & 63)) : 0ull;
         a.vis[w] = (g.inert ? g.inert[w] : 0ull) | sb;
+        a.fbit[w] = sb;              // frontier 0
+        a.fbit[(size_t)W + w] = 0;   // frontier 1, built by round 0
     }
     if (blockIdx.x == 0) {
         clear_ctr(&a.ctr[0]);
@@ -303,9 +308,13 @@ __global__ SG_BFS_BOUNDS(BS) void k_bfs(BfsArgs a)
         Round R;
         R.a = &a;
         R.L = (int32_t)L;
-        R.fbit_valid = pol.pull_prev;
-        R.fcur = a.fbit + (size_t)(L & 1) * W;
-        R.fnext = a.fbit + (size_t)((L + 1) & 1) * W;
+        R.fcur = a.fbit + (size_t)(L % 3) * W;
+        R.fnext = a.fbit + (size_t)((L + 1) % 3) * W;
+        {
+            // frontier L-1 is no longer read: clear its bitmap for round L+1
+            uint64_t *fclr = a.fbit + (size_t)((L + 2) % 3) * W;
+            for (uint32_t w = gtid; w < W; w += nthreads) fclr[w] = 0;
+        }
         R.S = Stager{a.queue + (size_t)((L + 1) % 3) * a.qcap, a.qcap, &a.ctr[(L + 1) % 3], s_q, &s_qn, kLdsQ};
         R.scout = 0;
         R.count = 0;
@@ -338,10 +347,9 @@ __global__ SG_BFS_BOUNDS(BS) void k_bfs(BfsArgs a)
                     }
                 }
             }
-            // (2) the rest of the frontier: from the bitmap after a pull round,
-            //     from the queue after a push round, from the level array if the
-            //     queue overflowed.
-            if (pol.pull_prev) {
+            // (2) the rest of the frontier: from the queue after a push round,
+            //     from the bitmap after a pull round or if the queue overflowed.
+            if (pol.pull_prev || rc.overflow) {
                 for (uint32_t wb = wbeg; wb < wend; wb += kWave) {
                     const uint32_t nw = min((uint32_t)kWave, wend - wb);
                     const uint64_t fl = (uint32_t)ln < nw ? R.fcur[wb + ln] : 0ull;
@@ -353,7 +361,7 @@ __global__ SG_BFS_BOUNDS(BS) void k_bfs(BfsArgs a)
                         push_expand(R, (fw >> ln) & 1, (wb + t) * 64 + ln);
                     }
                 }
-            } else if (!rc.overflow) {
+            } else {
                 // an even slice of the queue per wave (a short queue puts one
                 // vertex on each of many waves rather than 64 on a few)
                 const uint32_t per = (rc.qlen + nwaves - 1) / nwaves;
@@ -362,11 +370,6 @@ __global__ SG_BFS_BOUNDS(BS) void k_bfs(BfsArgs a)
                     const uint32_t i = i0 + ln;
                     const bool valid = i < qe;
                     push_expand(R, valid, valid ? qin[i] : 0u);
-                }
-            } else {
-                for (uint32_t w = gwave; w < W; w += nwaves) {
-                    const uint32_t v = w * 64 + ln;
-                    push_expand(R, v < n && a.level[v] == (int32_t)L, v);
                 }
             }
             flush_stage(R.S, s_flush);
@@ -378,7 +381,7 @@ __global__ SG_BFS_BOUNDS(BS) void k_bfs(BfsArgs a)
                 if ((uint32_t)ln < nw && vl == ~0ull) R.fnext[wb + ln] = 0;
                 uint64_t todo = ballot(vl != ~0ull);
                 while (todo) {
-                    constexpr int G = 4;
+                    constexpr int G = kPullGroup;
                     uint32_t wg[G];
                     uint64_t vwg[G];
                     int ng = 0;

@@ -165,6 +165,13 @@ extern "C" int sg_graph_create(uint32_t n, uint64_t m, const uint32_t *row_start
         }
     }
     if (!rc) {
+        const uint32_t *hcs = undirected ? row_start : cs;
+        const uint32_t *hsrc = undirected ? edge_dst : src;
+        std::vector<uint32_t> head(n);
+        for (uint32_t v = 0; v < n; v++) head[v] = hcs[v + 1] > hcs[v] ? hsrc[hcs[v]] : 0xFFFFFFFFu;
+        rc = up(head.data(), n, &d.chead);
+    }
+    if (!rc) {
         void *p = nullptr;
         rc = dev_alloc(g, &p, (uint64_t)d.words * 8);
         if (!rc) {
@@ -485,7 +492,8 @@ int engine_ensure_bfs_state(sg_engine *e)
     };
     int rc;
     if ((rc = engine_ensure_value(e, (uint64_t)d.n * 4))) return rc;
-    if ((rc = alloc((void **)&e->bitmaps, (uint64_t)d.words * 8 * 3))) return rc;
+    // BFS: visited filter + 3 rotating frontier bitmaps (SSSP uses the first 3)
+    if ((rc = alloc((void **)&e->bitmaps, (uint64_t)d.words * 8 * 4))) return rc;
     // Frontier queues may hold a vertex more than once (non-atomic discovery
     // across XCDs); past qcap a round falls back to scanning the level array.
     e->qcap = (uint32_t)std::min<uint64_t>(2ull * d.n + 4096, 0xFFFFFFF0ull);

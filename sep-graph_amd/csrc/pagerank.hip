@@ -92,8 +92,15 @@ __device__ __forceinline__ double seg_sum(double s, int k)
     return s;
 }
 
+// Sub-steps of 64 edges per iteration of a gather walk: all of an
+// iteration's loads are issued before its reductions, so one memory latency
+// covers kGatherSteps x 64 edges.
+constexpr int kGatherSteps = 4;
+
 // Sum over the in-edges [b, e) of each lane's vertex of val(edge) (FP64),
-// the wave walking the concatenated lists 64 edges per step.
+// the wave walking the concatenated lists kGatherSteps x 64 edges per
+// iteration. Each 64-edge sub-step reduces its segments in a fixed order, so
+// the result is deterministic.
 template <typename F>
 __device__ __forceinline__ double wave_gather_sum(uint32_t b, uint32_t e, F val)
 {
@@ -105,18 +112,27 @@ __device__ __forceinline__ double wave_gather_sum(uint32_t b, uint32_t e, F val)
         const uint32_t total = shfl(incl, kWave - 1);
         if (total == 0) break;
         const uint32_t excl = incl - r;
-        const uint32_t j = (uint32_t)ln;
-        const int k = owner_lane(incl, j);
-        const uint32_t bk = shfl(b, k), xk = shfl(excl, k);
-        double cand = 0.0;
-        if (j < total) cand = val(bk + (j - xk));
-        const double s = seg_sum(cand, k);
-        const uint32_t lo = min(excl, 64u), hi = min(incl, 64u), cov = hi - lo;
-        const double seg = __shfl(s, (int)(hi > 0 ? hi - 1 : 0), kWave);
-        if (cov) {
-            acc += seg;
-            b += cov;
+        int k[kGatherSteps];
+        double cand[kGatherSteps];
+#pragma unroll
+        for (int q = 0; q < kGatherSteps; q++) {
+            const uint32_t j = (uint32_t)(q * kWave + ln);
+            k[q] = owner_lane(incl, j < total ? j : total - 1);
+            const uint32_t bk = shfl(b, k[q]), xk = shfl(excl, k[q]);
+            cand[q] = j < total ? val(bk + (j - xk)) : 0.0;
         }
+        const uint32_t span = min(total, (uint32_t)(kGatherSteps * kWave));
+#pragma unroll
+        for (int q = 0; q < kGatherSteps; q++) {
+            const uint32_t s0 = (uint32_t)(q * kWave);
+            if (s0 >= span) break;  // wave-uniform
+            const double s = seg_sum(cand[q], k[q]);
+            // this lane's segment within sub-step q: positions [lo, hi)
+            const uint32_t lo = min(max(excl, s0), s0 + kWave), hi = min(max(incl, s0), min(span, s0 + kWave));
+            const double seg = __shfl(s, (int)(hi > s0 ? hi - 1 - s0 : 0), kWave);
+            if (hi > lo) acc += seg;
+        }
+        b += min(incl, span) - min(excl, span);
     }
     return acc;
 }

@@ -41,6 +41,10 @@ struct DevGraph {
     const uint32_t *csrc;     // CSC edge_src [m]
     const uint32_t *cw;       // CSC weights [m] or null
     const uint64_t *inert;    // bitmap: in-degree 0 (never reached unless source)
+    // First in-neighbour of every vertex (csrc[cstart[v]], 0xFFFFFFFF when it
+    // has none): a pull round's first parent test streams this n-word array
+    // instead of touching one csrc cache line per vertex.
+    const uint32_t *chead;
     // Static split of high-degree vertices: every vertex with out-degree >
     // kHugeDegree contributes ceil(deg / kHugeChunk) chunk descriptors
     // (vertex, first edge), so a push round can spread its edges over the grid.
@@ -122,7 +126,7 @@ struct sg_engine {
     void *value = nullptr;        // result of the last run (levels / distances / ranks / BC)
     uint64_t value_bytes = 0;     // bytes of that result
     uint64_t value_cap = 0;       // allocated bytes
-    uint64_t *bitmaps = nullptr;  // visited + 2 frontier bitmaps (3 * words)
+    uint64_t *bitmaps = nullptr;  // 4 * words: visited + 3 rotating frontier bitmaps
     uint32_t *queues = nullptr;   // 3 frontier queues of qcap entries
     uint32_t qcap = 0;
     uint32_t *stamp = nullptr;    // per-vertex round stamp (SSSP frontier membership)

@@ -86,6 +86,7 @@ __device__ __forceinline__ void relax(Round &R, bool act, uint32_t v, uint32_t n
             R.minv = min(R.minv, nd);
             if (a.stamp[v] != R.L + 2) {
                 a.stamp[v] = R.L + 2;
+                atomicOr((unsigned long long *)&R.fnext[v >> 6], 1ull << (v & 63));
                 const uint32_t dv = a.g.row[v + 1] - a.g.row[v];
                 R.scout += dv;
                 R.count++;
@@ -107,6 +108,7 @@ __device__ __forceinline__ void defer(Round &R, bool d, uint32_t u, uint32_t du)
         R.minv = min(R.minv, du);
         if (a.stamp[u] != R.L + 2) {
             a.stamp[u] = R.L + 2;
+            atomicOr((unsigned long long *)&R.fnext[u >> 6], 1ull << (u & 63));
             const uint32_t dv = a.g.row[u + 1] - a.g.row[u];
             R.scout += dv;
             R.count++;
@@ -288,8 +290,9 @@ __global__ __launch_bounds__(BS) void k_sssp(SsspArgs a)
         a.dist[i] = (i == src) ? 0u : kInf;
         a.stamp[i] = (i == src) ? 1u : 0u;
     }
-    // frontier bitmaps of rounds 0 and 1 start clear (round L clears L+2's)
-    for (uint32_t w = gtid; w < 2 * W; w += nthreads) a.fbit[w] = 0;
+    // frontier 0 = the source; frontier 1's bitmap starts clear (round L
+    // clears L+2's)
+    for (uint32_t w = gtid; w < 2 * W; w += nthreads) a.fbit[w] = w == (src >> 6) ? 1ull << (src & 63) : 0ull;
     if (blockIdx.x == 0) {
         clear_ctr(&a.ctr[0]);
         clear_ctr(&a.ctr[1]);
@@ -351,7 +354,9 @@ __global__ __launch_bounds__(BS) void k_sssp(SsspArgs a)
         Round R;
         R.a = &a;
         R.L = L;
-        R.fbit_valid = pol.pull_prev;
+        // every round leaves its next frontier in fnext (pull rounds by whole
+        // words, push rounds one atomicOr per queued vertex)
+        R.fbit_valid = true;
         R.fcur = a.fbit + (size_t)(L % 3) * W;
         R.fnext = a.fbit + (size_t)((L + 1) % 3) * W;
         // the bitmap for round L+2 is free (read in round L-1): clear it
