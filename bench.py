@@ -123,7 +123,7 @@ def serving_bench(sg, g, hg, e, args):
     srv.close()
     best = max(r["achieved_qps"] for r in runs)
     return {
-        "mix": "60% BFS / 30% SSSP / 10% BC-dependency, RMAT-%d directed" % args.scale,
+        "mix": f"60% BFS / 30% SSSP / 10% BC-dependency, RMAT-{args.scale} directed",
         "plain_qps": round(plain_qps, 1),
         "batched_qps": best,
         "speedup_vs_plain": round(best / plain_qps, 2),

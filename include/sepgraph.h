@@ -263,6 +263,16 @@ int sg_bfs_multi(sg_engine *e, const uint32_t *sources, int32_t k, int32_t batch
  * counterpart (one source per run there); serves batched SSSP queries. */
 int sg_sssp_multi(sg_engine *e, const uint32_t *sources, int32_t k, int32_t batch);
 
+/* SSSP from k sources, up to one concurrent query per XCD of the GPU (8 on
+ * MI355X): each query runs the single-source kernel on the blocks one XCD
+ * holds, with a barrier local to that XCD, so its rounds stay within one L2.
+ * The value array becomes k rows of n uint32 distances, row i equal to
+ * sg_sssp from sources[i]. The engine's options apply (with
+ * SG_PRIO_LOW_HIGH, every query uses the two-level schedule). No reference
+ * counterpart; serves batched SSSP queries with per-query latency close to
+ * a lone query's. */
+int sg_sssp_concurrent(sg_engine *e, const uint32_t *sources, int32_t k);
+
 /* ---------------------------------------------------------------- serving
  * No reference counterpart: the reference answers one query per process
  * (samples/hybrid_bfs/main.cpp and its siblings). A server keeps a graph resident and answers

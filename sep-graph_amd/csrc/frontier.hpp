@@ -239,7 +239,7 @@ __device__ __forceinline__ void write_trace(DevTrace *trace, uint32_t L, bool pu
 __device__ __forceinline__ void write_final(DevTrace *trace, DevStats *stats, uint32_t L, uint32_t pushes,
                                             uint32_t pulls, unsigned long long ptrace, bool ok, long long t_start)
 {
-    if (L < kTraceRows) {
+    if (trace && L < kTraceRows) {
         DevTrace t;
         t.round = L;
         t.variant = -1;

@@ -287,7 +287,7 @@ extern "C" int sg_engine_destroy(sg_engine *e)
     if (!e) return SG_OK;
     if (e->stream) (void)hipStreamSynchronize(e->stream);
     void *ptrs[] = {e->value, e->bitmaps, e->queues, e->stamp, e->ctr,
-                    e->abort_flag, e->stats, e->trace, e->exits, e->aux, e->srcbuf};
+                    e->abort_flag, e->stats, e->trace, e->exits, e->aux, e->srcbuf, e->garena};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     for (int i = 0; i < e->ngrids; i++) (void)hipFree(e->grids[i].words);

@@ -11,9 +11,12 @@
 //     the GPU is free, from whatever is queued at that time -- the kind whose
 //     oldest request has waited longest, up to that kind's batch limit (<= 64)
 //     -- so requests that arrive while a batch runs join the next one;
-//   * one launch per batch: the batched persistent kernels (bit-packed
-//     multi-source BFS, multi-source SSSP, multi-source Brandes) write each
-//     query's result straight into its row of the pool.
+//   * few launches per batch, each writing every query's result straight into
+//     its row of the pool: BFS and BC-dependency batches run the bit-packed
+//     multi-source kernels (up to 64 sources per launch); SSSP batches run up
+//     to eight single-source queries concurrently, one per XCD, because an
+//     SSSP query's rounds are latency-bound and its bit-packed batch does
+//     redundant relaxations.
 // Batches are serialised on the engine's stream: a persistent kernel assumes
 // all of its blocks are co-resident, which two concurrent launches would not
 // be. Host work (queueing, result hand-off) overlaps the running batch.
@@ -45,6 +48,9 @@ int kind_to_ms(int kind)
 }
 
 enum { kQueued = 0, kRunning = 1, kDone = 2, kFailed = 3 };
+
+// SSSP queries per concurrent launch: one per XCD of an MI355X.
+constexpr int kXcdQueries = 8;
 
 }  // namespace
 
@@ -137,11 +143,24 @@ struct sg_server {
             }
             lk.unlock();
 
-            bool ok = hipMemcpyAsync(d_src, h_src, b * 4, hipMemcpyHostToDevice, e->stream) == hipSuccess &&
-                      hipMemcpyAsync(d_rows, h_rows, b * 4, hipMemcpyHostToDevice, e->stream) == hipSuccess &&
-                      hipMemsetAsync(e->stats, 0, sizeof(DevStats), e->stream) == hipSuccess;
-            ok = ok && ms_launch(e, kind_to_ms(kind), d_src, b, pool, d_rows, true, true) == SG_OK;
-            ok = ok && sg_engine_sync(e) == SG_OK;
+            bool ok;
+            if (kind == SG_QUERY_SSSP) {
+                // one query per XCD, each launch writing rows of the pool
+                ok = true;
+                for (uint32_t i = 0; ok && i < b; i += kXcdQueries) {
+                    const int nq = (int)std::min<uint32_t>(kXcdQueries, b - i);
+                    uint32_t *out[kXcdQueries];
+                    for (int j = 0; j < nq; j++) out[j] = (uint32_t *)pool + (size_t)h_rows[i + j] * g->dev.n;
+                    ok = sssp_group_launch(e, h_src + i, nq, out) == SG_OK;
+                }
+                ok = ok && sssp_group_status(e) == SG_OK;
+            } else {
+                ok = hipMemcpyAsync(d_src, h_src, b * 4, hipMemcpyHostToDevice, e->stream) == hipSuccess &&
+                     hipMemcpyAsync(d_rows, h_rows, b * 4, hipMemcpyHostToDevice, e->stream) == hipSuccess &&
+                     hipMemsetAsync(e->stats, 0, sizeof(DevStats), e->stream) == hipSuccess;
+                ok = ok && ms_launch(e, kind_to_ms(kind), d_src, b, pool, d_rows, true, true) == SG_OK;
+                ok = ok && sg_engine_sync(e) == SG_OK;
+            }
             const double t1 = now_s();
 
             lk.lock();
@@ -197,8 +216,10 @@ extern "C" int sg_server_create(sg_graph *g, const sg_server_config *cfg, const 
         return code;
     };
     // working arenas of every kind (the largest wins) and the result pool
-    for (int k = 0; k < SG_QUERY_KINDS; k++)
-        if ((rc = ms_prepare(s->e, kind_to_ms(k)))) return fail(rc);
+    // SSSP runs as concurrent single-source queries (one per XCD) with their
+    // own state; BFS and BC-dependency batches use the bit-packed kernels
+    if ((rc = ms_prepare(s->e, kMsBfs))) return fail(rc);
+    if ((rc = ms_prepare(s->e, kMsBcDep))) return fail(rc);
     if ((rc = engine_alloc(&s->pool, (uint64_t)s->nrows * row_bytes))) return fail(rc);
     if ((rc = engine_alloc((void **)&s->d_src, 256))) return fail(rc);
     if ((rc = engine_alloc((void **)&s->d_rows, 256))) return fail(rc);

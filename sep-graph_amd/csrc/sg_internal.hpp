@@ -138,6 +138,9 @@ struct sg_engine {
     uint64_t aux_bytes = 0;
     uint32_t *srcbuf = nullptr;       // device copy of a batched call's sources
     uint64_t srcbuf_cap = 0;          // entries
+    void *garena = nullptr;           // per-query state of concurrent (one per XCD) SSSP
+    uint64_t garena_bytes = 0;
+    int32_t *group_status = nullptr;  // worst status of the last concurrent launch
     uint32_t *exits = nullptr;        // per-block exit records (diagnostics), kMaxBlocks x 4
     uint32_t trace_rows = 0;          // rows written by the last run (host copy)
     double wall_khz = 100000.0;       // device wall clock rate
@@ -184,6 +187,12 @@ int ms_prepare(sg_engine *e, int kind);
 // kMsBcSum, `out` is the n-float BC sum written by the batch with `last`.
 int ms_launch(sg_engine *e, int kind, const uint32_t *dev_src, uint32_t nsrc, void *out, const uint32_t *dev_orow,
               bool first, bool last);
+
+// Up to one SSSP query per XCD in one launch (the XCD's blocks form the
+// query's grid); query i writes its distances to the device row out[i].
+int sssp_group_launch(sg_engine *e, const uint32_t *sources, int nq, uint32_t *const *out);
+// Worst status of the last such launch (synchronises the stream).
+int sssp_group_status(sg_engine *e);
 
 // Device allocation owned by the engine (freed with it); reports SG_ERR_NOMEM.
 int engine_alloc(void **p, uint64_t bytes);

@@ -27,6 +27,16 @@ namespace {
 
 constexpr uint32_t kInf = 0xFFFFFFFFu;
 
+// Table of a launch with one query per XCD (passed by value).
+struct SsspGroups {
+    uint32_t *ticket;          // kMaxXcd counters zeroed before the launch: block ranks
+    int32_t *status;           // worst status over the queries (atomicMin), SG_OK before
+    uint32_t slot[kMaxXcd];    // query run by each XCD (>= nq: none)
+    uint32_t source[kMaxXcd];
+    uint32_t *out[kMaxXcd];    // distance row of each query
+    uint32_t nq;
+};
+
 struct SsspArgs {
     DevGraph g;
     uint32_t *dist;
@@ -47,6 +57,8 @@ struct SsspArgs {
     uint32_t probe;   // launch is a residency probe of the grid
     uint32_t *exits;  // per-block exit records (diagnostics)
     uint32_t delta;   // two-level priority bucket width (0: off)
+    uint32_t grouped;          // one query per XCD (gr), per-query state arrays
+    SsspGroups gr;
 };
 
 struct Round {
@@ -259,22 +271,17 @@ __device__ __forceinline__ void pull_hub_chunk(Round &R, uint32_t v, uint32_t be
 }
 
 template <int BS>
-__global__ __launch_bounds__(BS) void k_sssp(SsspArgs a)
+__device__ __forceinline__ void sssp_run(const SsspArgs &a, uint32_t brank, uint32_t nblocks, uint32_t xcc)
 {
-    if (a.probe) {
-        residency_probe(a.bar.words, 1u << 20);
-        return;
-    }
     constexpr int WPB = BS / kWave;
     constexpr uint32_t kLdsQ = 8u * BS;
     const DevGraph &g = a.g;
     const uint32_t n = g.n, W = g.words;
-    const uint32_t gtid = blockIdx.x * BS + threadIdx.x;
-    const uint32_t nthreads = gridDim.x * BS;
-    const uint32_t gwave = blockIdx.x * WPB + threadIdx.x / kWave;
-    const uint32_t nwaves = gridDim.x * WPB;
+    const uint32_t gtid = brank * BS + threadIdx.x;
+    const uint32_t nthreads = nblocks * BS;
+    const uint32_t gwave = brank * WPB + threadIdx.x / kWave;
+    const uint32_t nwaves = nblocks * WPB;
     const int ln = lane();
-    const uint32_t xcc = xcc_id();
     uint32_t gen = threadIdx.x == 0 ? grid_gen(a.bar) : 0u;
     const long long t_start = wall_clock64();
     const uint32_t wpw = (W + nwaves - 1) / nwaves;
@@ -293,7 +300,7 @@ __global__ __launch_bounds__(BS) void k_sssp(SsspArgs a)
     // frontier 0 = the source; frontier 1's bitmap starts clear (round L
     // clears L+2's)
     for (uint32_t w = gtid; w < 2 * W; w += nthreads) a.fbit[w] = w == (src >> 6) ? 1ull << (src & 63) : 0ull;
-    if (blockIdx.x == 0) {
+    if (brank == 0) {
         clear_ctr(&a.ctr[0]);
         clear_ctr(&a.ctr[1]);
         clear_ctr(&a.ctr[2]);
@@ -348,8 +355,8 @@ __global__ __launch_bounds__(BS) void k_sssp(SsspArgs a)
         } else {
             pushes++;
         }
-        if (gtid == 0) write_trace(a.trace, L, pull, rc.count, rc.scout, t_start);
-        if (blockIdx.x == 0) clear_ctr(&a.ctr[(L + 2) % 3]);
+        if (gtid == 0 && a.trace) write_trace(a.trace, L, pull, rc.count, rc.scout, t_start);
+        if (brank == 0) clear_ctr(&a.ctr[(L + 2) % 3]);
 
         Round R;
         R.a = &a;
@@ -463,19 +470,66 @@ __global__ __launch_bounds__(BS) void k_sssp(SsspArgs a)
     }
 
     if (gtid == 0) write_final(a.trace, a.stats, L, pushes, pulls, ptrace, ok, t_start);
-    if (threadIdx.x == 0 && blockIdx.x < kMaxBlocks) {
+    if (a.exits && threadIdx.x == 0 && brank < kMaxBlocks) {
         uint4 rec = make_uint4(L, reason, (uint32_t)min(last_count, 0xFFFFFFFFull), xcc);
-        *(uint4 *)&a.exits[blockIdx.x * 4] = rec;
+        *(uint4 *)&a.exits[brank * 4] = rec;
     }
 }
 
+// One query over the whole grid, or (a.groups) one query per XCD: the blocks
+// an XCD holds form that query's grid, ranked by an arrival ticket, with a
+// barrier of their own. Concurrent single-source queries each keep their
+// working set in one XCD's L2 and synchronise only within it, which suits
+// SSSP, whose rounds are latency-bound at far below the bandwidth of the chip.
 template <int BS>
-int launch_sssp(sg_engine *e, SsspArgs &args)
+__global__ __launch_bounds__(BS) void k_sssp(SsspArgs a)
 {
+    if (a.probe) {
+        residency_probe(a.bar.words, 1u << 20);
+        return;
+    }
+    const uint32_t xcc = xcc_id();
+    if (!a.grouped) {
+        sssp_run<BS>(a, blockIdx.x, gridDim.x, xcc);
+        return;
+    }
+    __shared__ uint32_t s_rank;
+    if (threadIdx.x == 0) s_rank = atomicAdd(&a.gr.ticket[xcc], 1u);
+    __syncthreads();
+    const uint32_t q = a.gr.slot[xcc];
+    if (q >= a.gr.nq) return;  // an XCD without a query this launch
+    const size_t n = a.g.n, W = a.g.words;
+    SsspArgs b = a;
+    b.dist = a.gr.out[q];
+    b.stamp = a.stamp + (size_t)q * n;
+    b.fbit = a.fbit + (size_t)q * 3 * W;
+    b.queue = a.queue + (size_t)q * 3 * a.qcap;
+    b.ctr = a.ctr + (size_t)q * 3;
+    b.stats = a.stats + q;
+    b.trace = nullptr;
+    b.exits = nullptr;
+    b.source = a.gr.source[q];
+    // the XCD's own barrier area (after the whole-grid one)
+    b.bar.words = a.bar.words + (size_t)(xcc + 1) * kBarWords;
+    b.bar.nxcd = 1;
+    sssp_run<BS>(b, s_rank, a.bar.cnt[xcc], xcc);
+    if (s_rank == 0 && threadIdx.x == 0) atomicMin(a.gr.status, b.stats->status);
+}
+
+// Persistent grid for k_sssp<BS>; grouped launches take the barrier area
+// `group_words` ((1 + kMaxXcd) areas) and give the XCDs holding blocks one
+// query each, in XCD order.
+template <int BS>
+int launch_sssp(sg_engine *e, SsspArgs &args, uint32_t *group_words)
+{
+    if (!args.abort || !args.stats || !args.dist && !args.grouped || !args.stamp || !args.fbit || !args.queue ||
+        !args.ctr || (args.grouped && (!group_words || !args.gr.ticket || !args.gr.status)))
+        SG_FAIL(SG_ERR_STATE, "sg_sssp: launch with unset device state");
     sg_engine::PGrid *pg = nullptr;
     auto probe = [e, args](int blocks, uint32_t *words) -> int {
         SsspArgs p = args;
         p.probe = 1;
+        p.grouped = 0;
         p.bar.words = words;
         hipLaunchKernelGGL(k_sssp<BS>, dim3(blocks), dim3(BS), 0, e->stream, p);
         SG_HIP(hipGetLastError());
@@ -483,44 +537,46 @@ int launch_sssp(sg_engine *e, SsspArgs &args)
     };
     int rc = engine_persistent_grid(e, (const void *)k_sssp<BS>, BS, probe, &pg);
     if (rc) return rc;
-    args.bar.words = pg->words;
     args.bar.nxcd = pg->nxcd;
     for (int x = 0; x < kMaxXcd; x++) args.bar.cnt[x] = pg->cnt[x];
-    SG_HIP(hipEventRecord(e->ev0, e->stream));
+    if (args.grouped) {
+        args.bar.words = group_words;
+        uint32_t q = 0;
+        for (int x = 0; x < kMaxXcd; x++) {
+            args.gr.slot[x] = kMaxXcd;
+            if (pg->cnt[x] && q < args.gr.nq) args.gr.slot[x] = q++;
+        }
+        if (q < args.gr.nq) SG_FAIL(SG_ERR_CAPACITY, "concurrent SSSP: more queries than XCDs holding blocks");
+    } else {
+        args.bar.words = pg->words;
+    }
     hipLaunchKernelGGL(k_sssp<BS>, dim3(pg->blocks), dim3(BS), 0, e->stream, args);
     SG_HIP(hipGetLastError());
-    SG_HIP(hipEventRecord(e->ev1, e->stream));
     return SG_OK;
 }
 
-}  // namespace
-
-extern "C" int sg_sssp(sg_engine *e, uint32_t source)
+int launch_sssp_bs(sg_engine *e, SsspArgs &a, uint32_t *group_words)
 {
-    if (!e) SG_FAIL(SG_ERR_INVALID, "sg_sssp: null engine");
+    const int bs = e->opt.block_size ? e->opt.block_size : 1024;
+    switch (bs) {
+    case 256: return launch_sssp<256>(e, a, group_words);
+    case 512: return launch_sssp<512>(e, a, group_words);
+    case 1024: return launch_sssp<1024>(e, a, group_words);
+    default: SG_FAIL(SG_ERR_UNSUPPORTED, "sg_sssp: block_size must be 256, 512 or 1024");
+    }
+}
+
+// Arguments shared by every SSSP launch of an engine (graph, options).
+int sssp_args(sg_engine *e, SsspArgs &a)
+{
     const DevGraph &d = e->g->dev;
-    if (source >= d.n) SG_FAIL(SG_ERR_RANGE, "sg_sssp: source >= n");
     const sg_engine_options &o = e->opt;
     const int force = forced_direction(o);
     if (force < -1) SG_FAIL(SG_ERR_UNSUPPORTED, "sg_sssp: only data-driven push/pull variants apply to SSSP");
-    int rc = engine_ensure_bfs_state(e);
-    if (rc) return rc;
-    if ((rc = engine_ensure_value(e, (uint64_t)d.n * 4))) return rc;
-    if ((rc = engine_ensure_stamp(e))) return rc;
-    SsspArgs a{};
+    a = SsspArgs{};
     a.g = d;
-    a.dist = (uint32_t *)e->value;
-    a.stamp = e->stamp;
-    a.fbit = e->bitmaps;  // three rotating frontier bitmaps
-    a.queue = e->queues;
-    a.qcap = e->qcap;
-    a.ctr = (FrontierCtr *)e->ctr;
     a.bar.spin_limit = 1u << 22;
     a.abort = e->abort_flag;
-    a.stats = e->stats;
-    a.trace = e->trace;
-    a.exits = e->exits;
-    a.source = source;
     a.force = force;
     a.alpha = o.alpha == 0.0 ? 1.0 : o.alpha;
     a.beta = o.beta;
@@ -540,15 +596,136 @@ extern "C" int sg_sssp(sg_engine *e, uint32_t source)
         a.delta = (uint32_t)std::max(1.0, std::min(dl, 2147483647.0));
         a.force = 0;
     }
-    e->last_algo = 1;
-    const int bs = o.block_size ? o.block_size : 1024;
-    switch (bs) {
-    case 256: rc = launch_sssp<256>(e, a); break;
-    case 512: rc = launch_sssp<512>(e, a); break;
-    case 1024: rc = launch_sssp<1024>(e, a); break;
-    default: SG_FAIL(SG_ERR_UNSUPPORTED, "sg_sssp: block_size must be 256, 512 or 1024");
-    }
+    return SG_OK;
+}
+
+}  // namespace
+
+namespace sg {
+
+// Up to one SSSP query per XCD in one launch (nq <= XCDs holding blocks);
+// query i writes distances to out[i] (device rows of n). Asynchronous; the
+// worst status lands in the engine's group status word.
+int sssp_group_launch(sg_engine *e, const uint32_t *sources, int nq, uint32_t *const *out)
+{
+    if (nq < 1 || nq > kMaxXcd) SG_FAIL(SG_ERR_RANGE, "concurrent SSSP: 1..16 queries per launch");
+    const DevGraph &d = e->g->dev;
+    for (int i = 0; i < nq; i++)
+        if (sources[i] >= d.n) SG_FAIL(SG_ERR_RANGE, "concurrent SSSP: source >= n");
+    int rc = engine_ensure_bfs_state(e);
     if (rc) return rc;
+    const uint64_t n = d.n, W = d.words, qc = e->qcap;
+    auto al = [](uint64_t x) { return (x + 255) / 256 * 256; };
+    const uint64_t o_stamp = 0;
+    const uint64_t o_fbit = o_stamp + al(kMaxXcd * n * 4);
+    const uint64_t o_queue = o_fbit + al(kMaxXcd * 3 * W * 8);
+    const uint64_t o_ctr = o_queue + al(kMaxXcd * 3 * qc * 4);
+    const uint64_t o_stats = o_ctr + al(kMaxXcd * 3 * sizeof(FrontierCtr));
+    const uint64_t o_bar = o_stats + al(kMaxXcd * sizeof(DevStats));
+    const uint64_t o_tick = o_bar + al((uint64_t)(1 + kMaxXcd) * kBarWords * 4);
+    const uint64_t total = o_tick + 256;
+    if (!e->garena || e->garena_bytes < total) {
+        if (e->garena) {
+            SG_HIP(hipStreamSynchronize(e->stream));
+            (void)hipFree(e->garena);
+            e->garena = nullptr;
+        }
+        if ((rc = engine_alloc(&e->garena, total))) return rc;
+        e->garena_bytes = total;
+        // barrier counters start at zero and stay consistent across launches
+        SG_HIP(hipMemsetAsync((char *)e->garena + o_bar, 0, (size_t)(total - o_bar), e->stream));
+    }
+    char *base = (char *)e->garena;
+    SsspArgs a;
+    if ((rc = sssp_args(e, a))) return rc;
+    a.stamp = (uint32_t *)(base + o_stamp);
+    a.fbit = (uint64_t *)(base + o_fbit);
+    a.queue = (uint32_t *)(base + o_queue);
+    a.qcap = e->qcap;
+    a.ctr = (FrontierCtr *)(base + o_ctr);
+    a.stats = (DevStats *)(base + o_stats);
+    a.grouped = 1;
+    a.gr.ticket = (uint32_t *)(base + o_tick);
+    a.gr.status = (int32_t *)(base + o_tick + 128);
+    a.gr.nq = (uint32_t)nq;
+    for (int i = 0; i < nq; i++) {
+        a.gr.source[i] = sources[i];
+        a.gr.out[i] = out[i];
+    }
+    // tickets and the status word are zeroed in order on the stream
+    SG_HIP(hipMemsetAsync(base + o_tick, 0, 256, e->stream));
+    e->last_algo = 6;
+    e->group_status = a.gr.status;
+    return launch_sssp_bs(e, a, (uint32_t *)(base + o_bar));
+}
+
+// Worst status of the last grouped launch (synchronises the stream).
+int sssp_group_status(sg_engine *e)
+{
+    if (!e->group_status) return SG_OK;
+    SG_HIP(hipStreamSynchronize(e->stream));
+    int32_t st = 0;
+    SG_HIP(hipMemcpy(&st, e->group_status, 4, hipMemcpyDeviceToHost));
+    if (st != SG_OK) {
+        e->needs_bar_reset = true;
+        SG_FAIL(st, "concurrent SSSP: a query reported an error");
+    }
+    return SG_OK;
+}
+
+}  // namespace sg
+
+extern "C" int sg_sssp(sg_engine *e, uint32_t source)
+{
+    if (!e) SG_FAIL(SG_ERR_INVALID, "sg_sssp: null engine");
+    const DevGraph &d = e->g->dev;
+    if (source >= d.n) SG_FAIL(SG_ERR_RANGE, "sg_sssp: source >= n");
+    // engine state first: sssp_args copies its device pointers
+    int rc = engine_ensure_bfs_state(e);
+    if (rc) return rc;
+    SsspArgs a;
+    if ((rc = sssp_args(e, a))) return rc;
+    if ((rc = engine_ensure_value(e, (uint64_t)d.n * 4))) return rc;
+    if ((rc = engine_ensure_stamp(e))) return rc;
+    a.dist = (uint32_t *)e->value;
+    a.stamp = e->stamp;
+    a.fbit = e->bitmaps;  // three rotating frontier bitmaps
+    a.queue = e->queues;
+    a.qcap = e->qcap;
+    a.ctr = (FrontierCtr *)e->ctr;
+    a.stats = e->stats;
+    a.trace = e->trace;
+    a.exits = e->exits;
+    a.source = source;
+    e->last_algo = 1;
+    SG_HIP(hipEventRecord(e->ev0, e->stream));
+    if ((rc = launch_sssp_bs(e, a, nullptr))) return rc;
+    SG_HIP(hipEventRecord(e->ev1, e->stream));
+    e->timed = true;
+    return SG_OK;
+}
+
+extern "C" int sg_sssp_concurrent(sg_engine *e, const uint32_t *sources, int32_t k)
+{
+    if (!e || !sources) SG_FAIL(SG_ERR_INVALID, "sg_sssp_concurrent: null argument");
+    if (k < 1) SG_FAIL(SG_ERR_RANGE, "sg_sssp_concurrent: need at least one source");
+    const DevGraph &d = e->g->dev;
+    for (int32_t i = 0; i < k; i++)
+        if (sources[i] >= d.n) SG_FAIL(SG_ERR_RANGE, "sg_sssp_concurrent: source >= n");
+    int rc = engine_ensure_bfs_state(e);
+    if (rc) return rc;
+    if ((rc = engine_ensure_value(e, (uint64_t)d.n * 4 * (uint64_t)k))) return rc;
+    // one query per XCD: MI355X has 8 (a launch on fewer XCDs reports
+    // SG_ERR_CAPACITY)
+    const int per = 8;
+    SG_HIP(hipEventRecord(e->ev0, e->stream));
+    for (int32_t i = 0; i < k; i += per) {
+        const int nq = std::min<int32_t>(per, k - i);
+        uint32_t *out[kMaxXcd];
+        for (int j = 0; j < nq; j++) out[j] = (uint32_t *)e->value + (size_t)(i + j) * d.n;
+        if ((rc = sssp_group_launch(e, sources + i, nq, out))) return rc;
+    }
+    SG_HIP(hipEventRecord(e->ev1, e->stream));
     e->timed = true;
     return SG_OK;
 }

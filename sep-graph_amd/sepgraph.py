@@ -143,6 +143,7 @@ def lib() -> C.CDLL:
             "sg_bc_dependency": ([_P, _P, C.c_int32, C.c_int32], C.c_int),
             "sg_bfs_multi": ([_P, _P, C.c_int32, C.c_int32], C.c_int),
             "sg_sssp_multi": ([_P, _P, C.c_int32, C.c_int32], C.c_int),
+            "sg_sssp_concurrent": ([_P, _P, C.c_int32], C.c_int),
             "sg_engine_sync": ([_P], C.c_int),
             "sg_engine_value_ptr": ([_P, C.POINTER(_P), C.POINTER(C.c_uint64)], C.c_int),
             "sg_engine_gather_value": ([_P, _P, C.c_uint64], C.c_int),
@@ -357,6 +358,11 @@ class Engine:
         """SSSP distances from every source (k x n uint32) into the value array."""
         src = _u32(sources)
         _check(lib().sg_sssp_multi(self.h, _ptr(src), len(src), int(batch)))
+
+    def sssp_concurrent(self, sources):
+        """SSSP from every source, one concurrent query per XCD (k x n uint32)."""
+        src = _u32(sources)
+        _check(lib().sg_sssp_concurrent(self.h, _ptr(src), len(src)))
 
     def sync(self):
         _check(lib().sg_engine_sync(self.h))

@@ -137,3 +137,25 @@ def test_sssp_multi_grid(sg, orc, gpu):
     out = rows(e.gather(np.uint32), 16, hg.n)
     for i, s in enumerate(srcs):
         assert np.array_equal(out[i], orc.sssp_dijkstra(hg.row, hg.col, hg.w, int(s))), i
+
+
+@pytest.mark.parametrize("prio", [0, 1])
+def test_sssp_concurrent_matches_single(sg, orc, gpu, prio):
+    """Up to eight SSSP queries at once, one per XCD, each equal to a lone
+    sg_sssp run; 13 sources exercise a full and a partial launch."""
+    hg = sg.rmat_graph(16, 16, symmetrize=False, seed=1, weights=(1, 64))
+    g = sg.Graph(hg)
+    e = sg.Engine(g, priority=prio)
+    srcs = sg.pick_sources(hg.row, 13, seed=3)
+    srcs[4] = srcs[9]
+    e.sssp_concurrent(srcs)
+    out = rows(e.gather(np.uint32), 13, hg.n).copy()
+    for i, s in enumerate(srcs):
+        assert np.array_equal(out[i], orc.sssp_dijkstra(hg.row, hg.col, hg.w, int(s))), i
+    gg = sg.grid_graph(64, 64, 0.1, seed=1, weights=(1, 1000))
+    e2 = sg.Engine(sg.Graph(gg), priority=prio)
+    s2 = sg.pick_sources(gg.row, 8, seed=4)
+    e2.sssp_concurrent(s2)
+    out2 = rows(e2.gather(np.uint32), 8, gg.n)
+    for i, s in enumerate(s2):
+        assert np.array_equal(out2[i], orc.sssp_dijkstra(gg.row, gg.col, gg.w, int(s))), i
