@@ -115,6 +115,17 @@ typedef struct sg_graph sg_graph;
 int sg_graph_create(uint32_t n, uint64_t m, const uint32_t *row_start, const uint32_t *edge_dst,
                     const uint32_t *weights, const uint32_t *col_start, const uint32_t *edge_src,
                     const uint32_t *csc_weights, int undirected, sg_graph **out);
+
+/* sg_graph_create with flags. By default the device copy orders vertices by
+ * total degree (hot vertices first, so per-vertex state shares cache lines)
+ * when the degree distribution is skewed; every entry point still takes and
+ * returns the caller's vertex ids, and results are identical.
+ * SG_GRAPH_RELABEL forces that order, SG_GRAPH_NO_RELABEL keeps the caller's.
+ * With a relabelled graph a caller-given CSC is rebuilt from the CSR. */
+enum { SG_GRAPH_UNDIRECTED = 1, SG_GRAPH_RELABEL = 2, SG_GRAPH_NO_RELABEL = 4 };
+int sg_graph_create_ex(uint32_t n, uint64_t m, const uint32_t *row_start, const uint32_t *edge_dst,
+                       const uint32_t *weights, const uint32_t *col_start, const uint32_t *edge_src,
+                       const uint32_t *csc_weights, uint32_t flags, sg_graph **out);
 int sg_graph_destroy(sg_graph *g);
 /* n, m, and bytes of device memory held by the graph. */
 int sg_graph_info(const sg_graph *g, uint32_t *n, uint64_t *m, uint64_t *device_bytes);

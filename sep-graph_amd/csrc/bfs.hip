@@ -447,10 +447,11 @@ extern "C" int sg_bfs(sg_engine *e, uint32_t source)
     if (force < -1) SG_FAIL(SG_ERR_UNSUPPORTED, "sg_bfs: only data-driven push/pull variants apply to BFS");
     int rc = engine_ensure_bfs_state(e);
     if (rc) return rc;
-    if ((rc = engine_ensure_value(e, (uint64_t)d.n * 4))) return rc;
+    void *res = nullptr;
+    if ((rc = engine_result_rows(e, 1, &res))) return rc;
     BfsArgs a{};
     a.g = d;
-    a.level = (int32_t *)e->value;
+    a.level = (int32_t *)res;
     a.vis = e->bitmaps;
     a.fbit = e->bitmaps + d.words;
     a.queue = e->queues;
@@ -460,7 +461,7 @@ extern "C" int sg_bfs(sg_engine *e, uint32_t source)
     a.abort = e->abort_flag;
     a.stats = e->stats;
     a.trace = e->trace;
-    a.source = source;
+    a.source = dev_vertex(e->g, source);
     a.force = force;
     a.alpha = o.alpha == 0.0 ? 1.0 : o.alpha;
     a.beta = o.beta;
@@ -475,6 +476,7 @@ extern "C" int sg_bfs(sg_engine *e, uint32_t source)
     default: SG_FAIL(SG_ERR_UNSUPPORTED, "sg_bfs: block_size must be 256, 512 or 1024");
     }
     if (rc) return rc;
+    if ((rc = engine_finish(e, 1))) return rc;
     e->timed = true;
     return SG_OK;
 }

@@ -24,6 +24,15 @@ __global__ void k_inert_bitmap(const uint32_t *cstart, uint32_t n, uint32_t word
     if (lane() == 0) inert[w] = m;
 }
 
+// Row r of dst (caller's vertex order) from row r of src (device order):
+// dst[v] = src[inv[v]]. Reads of inv and writes are sequential.
+__global__ void k_gather_rows(const uint32_t *src, uint32_t *dst, const uint32_t *inv, uint32_t n)
+{
+    const size_t off = (size_t)blockIdx.y * n;
+    for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < n; v += gridDim.x * blockDim.x)
+        dst[off + v] = src[off + inv[v]];
+}
+
 // Exact reach statistics of a traversal result: vertices whose 32-bit value
 // is not the 0xFFFFFFFF sentinel (-1 level / UINT32_MAX distance) and the sum
 // of their out-degrees. out[0], out[1] must be zero on entry.
@@ -89,10 +98,78 @@ extern "C" int sg_graph_destroy(sg_graph *g)
     return SG_OK;
 }
 
+namespace {
+
+// Degree-ordered relabelling of a CSR: new vertex i is old vertex perm[i],
+// ordered by total degree (out + in) descending, ties by id. Rows keep their
+// neighbours sorted. The hot vertices of a skewed graph then share a few
+// cache lines, so the random per-vertex accesses of every kernel (levels,
+// distances, frontier bits, PageRank contributions) mostly hit L2.
+void relabel_csr(uint32_t n, const uint32_t *row, const uint32_t *col, const uint32_t *w,
+                 std::vector<uint32_t> &perm, std::vector<uint32_t> &inv, std::vector<uint32_t> &nrow,
+                 std::vector<uint32_t> &ncol, std::vector<uint32_t> &nw)
+{
+    const uint64_t m = row[n];
+    std::vector<uint32_t> deg(n);
+    for (uint32_t v = 0; v < n; v++) deg[v] = row[v + 1] - row[v];
+    for (uint64_t e = 0; e < m; e++) deg[col[e]]++;
+    perm.resize(n);
+    for (uint32_t v = 0; v < n; v++) perm[v] = v;
+    std::stable_sort(perm.begin(), perm.end(), [&](uint32_t a, uint32_t b) { return deg[a] > deg[b]; });
+    inv.resize(n);
+    for (uint32_t i = 0; i < n; i++) inv[perm[i]] = i;
+    nrow.assign((size_t)n + 1, 0);
+    for (uint32_t i = 0; i < n; i++) nrow[i + 1] = nrow[i] + (row[perm[i] + 1] - row[perm[i]]);
+    ncol.resize(m ? m : 1);
+    if (w) nw.resize(m ? m : 1);
+    sg::parallel_for_dynamic(n, 4096, [&](uint64_t i0, uint64_t i1) {
+        std::vector<std::pair<uint32_t, uint32_t>> t;
+        for (uint64_t i = i0; i < i1; i++) {
+            const uint32_t u = perm[i];
+            const uint32_t b = row[u], d = row[u + 1] - b, o = nrow[i];
+            if (!w) {
+                for (uint32_t j = 0; j < d; j++) ncol[o + j] = inv[col[b + j]];
+                std::sort(ncol.begin() + o, ncol.begin() + o + d);
+            } else {
+                t.resize(d);
+                for (uint32_t j = 0; j < d; j++) t[j] = {inv[col[b + j]], w[b + j]};
+                std::sort(t.begin(), t.end());
+                for (uint32_t j = 0; j < d; j++) {
+                    ncol[o + j] = t[j].first;
+                    nw[o + j] = t[j].second;
+                }
+            }
+        }
+    });
+}
+
+// Relabel when the degree distribution is skewed (a hub far above the mean);
+// uniform-degree graphs such as grids keep their id order and its locality.
+bool want_relabel(uint32_t n, const uint32_t *row, uint32_t flags)
+{
+    if (flags & SG_GRAPH_NO_RELABEL) return false;
+    if (flags & SG_GRAPH_RELABEL) return true;
+    const uint64_t m = row[n];
+    uint32_t dmax = 0;
+    for (uint32_t v = 0; v < n; v++) dmax = std::max(dmax, row[v + 1] - row[v]);
+    return n > 4096 && (double)dmax > 64.0 * ((double)m / n + 1.0);
+}
+
+}  // namespace
+
 extern "C" int sg_graph_create(uint32_t n, uint64_t m, const uint32_t *row_start, const uint32_t *edge_dst,
                                const uint32_t *weights, const uint32_t *col_start, const uint32_t *edge_src,
                                const uint32_t *csc_weights, int undirected, sg_graph **out)
 {
+    return sg_graph_create_ex(n, m, row_start, edge_dst, weights, col_start, edge_src, csc_weights,
+                              undirected ? SG_GRAPH_UNDIRECTED : 0u, out);
+}
+
+extern "C" int sg_graph_create_ex(uint32_t n, uint64_t m, const uint32_t *row_start, const uint32_t *edge_dst,
+                                  const uint32_t *weights, const uint32_t *col_start, const uint32_t *edge_src,
+                                  const uint32_t *csc_weights, uint32_t flags, sg_graph **out)
+{
+    const bool undirected = (flags & SG_GRAPH_UNDIRECTED) != 0;
     if (!out) SG_FAIL(SG_ERR_INVALID, "sg_graph_create: null output handle");
     *out = nullptr;
     if (n == 0 || n >= (1u << 31)) SG_FAIL(SG_ERR_RANGE, "sg_graph_create: need 0 < n < 2^31");
@@ -103,7 +180,7 @@ extern "C" int sg_graph_create(uint32_t n, uint64_t m, const uint32_t *row_start
         if (row_start[v + 1] < row_start[v]) SG_FAIL(SG_ERR_INVALID, "sg_graph_create: row_start not monotone");
     for (uint64_t e = 0; e < m; e++)
         if (edge_dst[e] >= n) SG_FAIL(SG_ERR_RANGE, "sg_graph_create: edge_dst entry >= n");
-    const bool have_csc = col_start != nullptr;
+    bool have_csc = col_start != nullptr;
     if (!undirected && have_csc && (!edge_src || (weights && !csc_weights)))
         SG_FAIL(SG_ERR_INVALID, "sg_graph_create: incomplete CSC");
     int ndev = 0;
@@ -112,6 +189,18 @@ extern "C" int sg_graph_create(uint32_t n, uint64_t m, const uint32_t *row_start
         SG_FAIL(SG_ERR_HIP, "sg_graph_create: no HIP device visible");
     }
     SG_HIP(hipSetDevice(0));
+
+    // device-side vertex order (the caller's ids are kept at the boundary)
+    std::vector<uint32_t> perm, inv, r_row, r_col, r_w;
+    const bool relabel = want_relabel(n, row_start, flags);
+    if (relabel) {
+        relabel_csr(n, row_start, edge_dst, weights, perm, inv, r_row, r_col, r_w);
+        row_start = r_row.data();
+        edge_dst = r_col.data();
+        weights = weights ? r_w.data() : nullptr;
+        have_csc = false;  // rebuilt from the relabelled CSR below
+        col_start = edge_src = csc_weights = nullptr;
+    }
 
     // Host CSC when the caller did not pass one (directed graphs only).
     std::vector<uint32_t> h_cs, h_src, h_cw;
@@ -131,6 +220,8 @@ extern "C" int sg_graph_create(uint32_t n, uint64_t m, const uint32_t *row_start
     sg_graph *g = new sg_graph();
     g->undirected = undirected != 0;
     g->weighted = weights != nullptr;
+    g->relabeled = relabel;
+    g->h_inv = std::move(inv);
     DevGraph &d = g->dev;
     d.n = n;
     d.m = m;
@@ -170,6 +261,11 @@ extern "C" int sg_graph_create(uint32_t n, uint64_t m, const uint32_t *row_start
         std::vector<uint32_t> head(n);
         for (uint32_t v = 0; v < n; v++) head[v] = hcs[v + 1] > hcs[v] ? hsrc[hcs[v]] : 0xFFFFFFFFu;
         rc = up(head.data(), n, &d.chead);
+    }
+    if (!rc && relabel) {
+        const uint32_t *p = nullptr;
+        rc = up(g->h_inv.data(), n, &p);
+        g->d_inv = p;
     }
     if (!rc) {
         void *p = nullptr;
@@ -287,7 +383,7 @@ extern "C" int sg_engine_destroy(sg_engine *e)
     if (!e) return SG_OK;
     if (e->stream) (void)hipStreamSynchronize(e->stream);
     void *ptrs[] = {e->value, e->bitmaps, e->queues, e->stamp, e->ctr,
-                    e->abort_flag, e->stats, e->trace, e->exits, e->aux, e->srcbuf, e->garena};
+                    e->abort_flag, e->stats, e->trace, e->exits, e->aux, e->srcbuf, e->garena, e->work};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     for (int i = 0; i < e->ngrids; i++) (void)hipFree(e->grids[i].words);
@@ -428,6 +524,57 @@ int engine_ensure_stamp(sg_engine *e)
 {
     if (e->stamp) return SG_OK;
     return engine_alloc((void **)&e->stamp, (uint64_t)e->g->dev.n * 4);
+}
+
+uint32_t dev_vertex(const sg_graph *g, uint32_t v) { return g->relabeled ? g->h_inv[v] : v; }
+
+int engine_result_rows(sg_engine *e, uint64_t rows, void **buf)
+{
+    const uint64_t bytes = rows * (uint64_t)e->g->dev.n * 4;
+    int rc = engine_ensure_value(e, bytes);
+    if (rc) return rc;
+    if (!e->g->relabeled) {
+        *buf = e->value;
+        return SG_OK;
+    }
+    if ((rc = engine_ensure_work(e, bytes))) return rc;
+    *buf = e->work;
+    return SG_OK;
+}
+
+int engine_ensure_work(sg_engine *e, uint64_t bytes)
+{
+    int rc;
+    if (!e->work || e->work_cap < bytes) {
+        if (e->work) {
+            SG_HIP(hipStreamSynchronize(e->stream));
+            (void)hipFree(e->work);
+            e->work = nullptr;
+            e->work_cap = 0;
+        }
+        if ((rc = engine_alloc(&e->work, bytes))) return rc;
+        e->work_cap = bytes;
+    }
+    return SG_OK;
+}
+
+int engine_gather_rows(sg_engine *e, const void *src, void *dst, uint64_t rows)
+{
+    if (!e->g->relabeled) return SG_OK;
+    const uint32_t n = e->g->dev.n;
+    for (uint64_t r0 = 0; r0 < rows; r0 += 65535) {
+        const uint32_t nr = (uint32_t)std::min<uint64_t>(65535, rows - r0);
+        dim3 grid(std::min<uint32_t>((n + 255) / 256, 4096u), nr);
+        hipLaunchKernelGGL(k_gather_rows, grid, dim3(256), 0, e->stream, (const uint32_t *)src + r0 * n,
+                           (uint32_t *)dst + r0 * n, e->g->d_inv, n);
+        SG_HIP(hipGetLastError());
+    }
+    return SG_OK;
+}
+
+int engine_finish(sg_engine *e, uint64_t rows)
+{
+    return engine_gather_rows(e, e->work, e->value, rows);
 }
 
 int engine_ensure_srcbuf(sg_engine *e, uint64_t entries)
@@ -577,8 +724,10 @@ extern "C" int sg_engine_get_stats(sg_engine *e, sg_stats *out)
         const DevGraph &d = e->g->dev;
         // few blocks: each adds into the same two words once
         const uint32_t blocks = std::min<uint32_t>((d.n + 255) / 256, (uint32_t)e->g->num_cus);
-        hipLaunchKernelGGL(k_reach_stats, dim3(blocks), dim3(256), 0, e->stream, (const uint32_t *)e->value, d.row,
-                           d.n, acc);
+        // the result in device order pairs with the device CSR
+        const void *res = e->g->relabeled ? e->work : e->value;
+        hipLaunchKernelGGL(k_reach_stats, dim3(blocks), dim3(256), 0, e->stream, (const uint32_t *)res, d.row, d.n,
+                           acc);
         SG_HIP(hipGetLastError());
         unsigned long long h[2];
         SG_HIP(hipMemcpyAsync(h, acc, 16, hipMemcpyDeviceToHost, e->stream));

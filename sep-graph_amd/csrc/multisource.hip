@@ -137,6 +137,19 @@ __device__ __forceinline__ double masked_row_sum(const uint32_t *adj, const uint
             fm = M[u] & mask;
         }
         uint64_t mm = ballot(fm != 0);
+        uint64_t srcs = wave_or_u64(fm);
+        if (__popcll(srcs) < __popcll(mm)) {
+            // fewer sources than matching neighbours (small batches): per
+            // source, every lane loads its neighbour's value and the wave sums
+            while (srcs) {
+                const int s = first_lane(srcs);
+                srcs &= srcs - 1;
+                double x = ((fm >> s) & 1) ? row[(size_t)u * 64 + s] : 0.0;
+                x = wave_sum_f64(x);
+                if (ln == s) acc += x;
+            }
+            continue;
+        }
         while (mm) {
             const int j = first_lane(mm);
             mm &= mm - 1;
@@ -1079,20 +1092,26 @@ int ms_run(sg_engine *e, int kind, const uint32_t *sources, int32_t k, int32_t b
     if (rc) return rc;
     const uint64_t n = d.n;
     const uint64_t rows = kind == kMsBcSum ? 1 : (uint64_t)k;
-    if ((rc = engine_ensure_value(e, n * 4 * rows))) return rc;
+    void *res = nullptr;
+    if ((rc = engine_result_rows(e, rows, &res))) return rc;
     if ((rc = engine_ensure_srcbuf(e, (uint64_t)k))) return rc;
-    SG_HIP(hipMemcpyAsync(e->srcbuf, sources, (size_t)k * 4, hipMemcpyHostToDevice, e->stream));
+    std::vector<uint32_t> dsrc(k);
+    for (int32_t i = 0; i < k; i++) dsrc[i] = dev_vertex(e->g, sources[i]);
+    // the source buffer may still be read by an earlier launch on the stream
+    SG_HIP(hipStreamSynchronize(e->stream));
+    SG_HIP(hipMemcpy(e->srcbuf, dsrc.data(), (size_t)k * 4, hipMemcpyHostToDevice));
     SG_HIP(hipMemsetAsync(e->stats, 0, sizeof(DevStats), e->stream));
     e->last_algo = 3 + kind;
     SG_HIP(hipEventRecord(e->ev0, e->stream));
     const int nb = (k + batch - 1) / batch;
     for (int i = 0; i < nb; i++) {
         const uint32_t ns = (uint32_t)std::min(batch, k - i * batch);
-        void *out = kind == kMsBcSum ? e->value : (void *)((uint32_t *)e->value + (size_t)i * batch * n);
+        void *out = kind == kMsBcSum ? res : (void *)((uint32_t *)res + (size_t)i * batch * n);
         rc = ms_launch(e, kind, e->srcbuf + (size_t)i * batch, ns, out, nullptr, i == 0, i == nb - 1);
         if (rc) return rc;
     }
     SG_HIP(hipEventRecord(e->ev1, e->stream));
+    if ((rc = engine_finish(e, rows))) return rc;
     e->timed = true;
     return SG_OK;
 }

@@ -632,7 +632,8 @@ extern "C" int sg_pagerank(sg_engine *e, double damping, double tol, int32_t max
     const DevGraph &d = e->g->dev;
     int rc = engine_ensure_bfs_state(e);
     if (rc) return rc;
-    if ((rc = engine_ensure_value(e, (uint64_t)d.n * 4))) return rc;
+    void *res = nullptr;
+    if ((rc = engine_result_rows(e, 1, &res))) return rc;
     const uint64_t n = d.n;
     const int bs = o.block_size ? o.block_size : 1024;
     if (bs != 256 && bs != 512 && bs != 1024) SG_FAIL(SG_ERR_UNSUPPORTED, "sg_pagerank: block_size must be 256, 512 or 1024");
@@ -646,7 +647,7 @@ extern "C" int sg_pagerank(sg_engine *e, double damping, double tol, int32_t max
         char *base = (char *)e->aux;
         PrArgs a{};
         a.g = d;
-        a.x = (float *)e->value;
+        a.x = (float *)res;
         a.contrib = (float *)base;
         a.hubpart = (double *)(base + off_hub);
         a.part = (double2 *)(base + off_part);
@@ -671,7 +672,7 @@ extern "C" int sg_pagerank(sg_engine *e, double damping, double tol, int32_t max
         char *base = (char *)e->aux;
         PrDdArgs a{};
         a.g = d;
-        a.x = (float *)e->value;
+        a.x = (float *)res;
         a.y = (double *)base;
         a.r = (double *)(base + n * 8);
         a.p = (double *)(base + 2 * n * 8);
@@ -697,6 +698,7 @@ extern "C" int sg_pagerank(sg_engine *e, double damping, double tol, int32_t max
         }
     }
     if (rc) return rc;
+    if ((rc = engine_finish(e, 1))) return rc;
     e->timed = true;
     return SG_OK;
 }

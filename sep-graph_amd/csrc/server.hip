@@ -143,24 +143,34 @@ struct sg_server {
             }
             lk.unlock();
 
+            // A graph kept in degree order on the device computes into the
+            // engine's work rows 0..b-1, then gathers each into its pool row
+            // in the caller's vertex order; otherwise kernels write the pool.
+            const bool via_work = g->relabeled;
+            const size_t n = g->dev.n;
+            uint32_t *work = (uint32_t *)e->work;
             bool ok;
             if (kind == SG_QUERY_SSSP) {
-                // one query per XCD, each launch writing rows of the pool
+                // one query per XCD, each launch writing a row per query
                 ok = true;
                 for (uint32_t i = 0; ok && i < b; i += kXcdQueries) {
                     const int nq = (int)std::min<uint32_t>(kXcdQueries, b - i);
                     uint32_t *out[kXcdQueries];
-                    for (int j = 0; j < nq; j++) out[j] = (uint32_t *)pool + (size_t)h_rows[i + j] * g->dev.n;
+                    for (int j = 0; j < nq; j++)
+                        out[j] = via_work ? work + (i + j) * n : (uint32_t *)pool + (size_t)h_rows[i + j] * n;
                     ok = sssp_group_launch(e, h_src + i, nq, out) == SG_OK;
                 }
-                ok = ok && sssp_group_status(e) == SG_OK;
             } else {
                 ok = hipMemcpyAsync(d_src, h_src, b * 4, hipMemcpyHostToDevice, e->stream) == hipSuccess &&
                      hipMemcpyAsync(d_rows, h_rows, b * 4, hipMemcpyHostToDevice, e->stream) == hipSuccess &&
                      hipMemsetAsync(e->stats, 0, sizeof(DevStats), e->stream) == hipSuccess;
-                ok = ok && ms_launch(e, kind_to_ms(kind), d_src, b, pool, d_rows, true, true) == SG_OK;
-                ok = ok && sg_engine_sync(e) == SG_OK;
+                ok = ok && ms_launch(e, kind_to_ms(kind), d_src, b, via_work ? (void *)work : pool,
+                                     via_work ? nullptr : d_rows, true, true) == SG_OK;
             }
+            for (uint32_t j = 0; ok && via_work && j < b; j++)
+                ok = engine_gather_rows(e, work + j * n, (uint32_t *)pool + (size_t)h_rows[j] * n, 1) == SG_OK;
+            if (kind == SG_QUERY_SSSP) ok = ok && sssp_group_status(e) == SG_OK;
+            else ok = ok && sg_engine_sync(e) == SG_OK;
             const double t1 = now_s();
 
             lk.lock();
@@ -220,6 +230,7 @@ extern "C" int sg_server_create(sg_graph *g, const sg_server_config *cfg, const 
     // own state; BFS and BC-dependency batches use the bit-packed kernels
     if ((rc = ms_prepare(s->e, kMsBfs))) return fail(rc);
     if ((rc = ms_prepare(s->e, kMsBcDep))) return fail(rc);
+    if (g->relabeled && (rc = engine_ensure_work(s->e, 64ull * g->dev.n * 4))) return fail(rc);
     if ((rc = engine_alloc(&s->pool, (uint64_t)s->nrows * row_bytes))) return fail(rc);
     if ((rc = engine_alloc((void **)&s->d_src, 256))) return fail(rc);
     if ((rc = engine_alloc((void **)&s->d_rows, 256))) return fail(rc);
@@ -267,7 +278,7 @@ int submit_locked(sg_server *s, int32_t kind, uint32_t source, double t_arrival,
     const uint64_t t = s->next_ticket++;
     sg_server::Req r{};
     r.kind = kind;
-    r.source = source;
+    r.source = dev_vertex(s->g, source);  // device vertex order
     r.row = -1;
     r.state = kQueued;
     r.status = SG_OK;

@@ -80,6 +80,11 @@ struct sg_graph {
     int device = 0;
     int num_cus = 0;
     double avg_weight = 0.0;  // for the SSSP delta heuristic
+    // Degree-ordered device ids: the caller's vertex v is device vertex
+    // h_inv[v]; results are gathered back into the caller's order.
+    bool relabeled = false;
+    std::vector<uint32_t> h_inv;
+    const uint32_t *d_inv = nullptr;
 };
 
 namespace sg {
@@ -138,6 +143,8 @@ struct sg_engine {
     uint64_t aux_bytes = 0;
     uint32_t *srcbuf = nullptr;       // device copy of a batched call's sources
     uint64_t srcbuf_cap = 0;          // entries
+    void *work = nullptr;             // results in device vertex order (relabelled graphs)
+    uint64_t work_cap = 0;
     void *garena = nullptr;           // per-query state of concurrent (one per XCD) SSSP
     uint64_t garena_bytes = 0;
     int32_t *group_status = nullptr;  // worst status of the last concurrent launch
@@ -176,6 +183,20 @@ int engine_ensure_value(sg_engine *e, uint64_t bytes);
 // stream before replacing a smaller one; contents are not preserved).
 int engine_ensure_aux(sg_engine *e, uint64_t bytes);
 int engine_ensure_srcbuf(sg_engine *e, uint64_t entries);
+
+// Relabelled graphs keep vertices in degree order on the device.
+// The device id of the caller's vertex v.
+uint32_t dev_vertex(const sg_graph *g, uint32_t v);
+// Where a run writes `rows` rows of n 32-bit results: the value array, or,
+// for a relabelled graph, the engine's work array (device order), which
+// engine_finish then gathers into the value array in the caller's order.
+int engine_result_rows(sg_engine *e, uint64_t rows, void **buf);
+// The work array alone, at least `bytes` (grown when needed).
+int engine_ensure_work(sg_engine *e, uint64_t bytes);
+int engine_finish(sg_engine *e, uint64_t rows);
+// dst rows (caller's order) from src rows (device order); no-op for graphs
+// kept in the caller's order. Asynchronous on the engine's stream.
+int engine_gather_rows(sg_engine *e, const void *src, void *dst, uint64_t rows);
 
 // Batched (bit-packed, up to 64 sources per launch) query kinds.
 enum MsKind { kMsBfs = 0, kMsSssp = 1, kMsBcDep = 2, kMsBcSum = 3 };

@@ -522,7 +522,7 @@ __global__ __launch_bounds__(BS) void k_sssp(SsspArgs a)
 template <int BS>
 int launch_sssp(sg_engine *e, SsspArgs &args, uint32_t *group_words)
 {
-    if (!args.abort || !args.stats || !args.dist && !args.grouped || !args.stamp || !args.fbit || !args.queue ||
+    if (!args.abort || !args.stats || (!args.dist && !args.grouped) || !args.stamp || !args.fbit || !args.queue ||
         !args.ctr || (args.grouped && (!group_words || !args.gr.ticket || !args.gr.status)))
         SG_FAIL(SG_ERR_STATE, "sg_sssp: launch with unset device state");
     sg_engine::PGrid *pg = nullptr;
@@ -685,9 +685,10 @@ extern "C" int sg_sssp(sg_engine *e, uint32_t source)
     if (rc) return rc;
     SsspArgs a;
     if ((rc = sssp_args(e, a))) return rc;
-    if ((rc = engine_ensure_value(e, (uint64_t)d.n * 4))) return rc;
+    void *res = nullptr;
+    if ((rc = engine_result_rows(e, 1, &res))) return rc;
     if ((rc = engine_ensure_stamp(e))) return rc;
-    a.dist = (uint32_t *)e->value;
+    a.dist = (uint32_t *)res;
     a.stamp = e->stamp;
     a.fbit = e->bitmaps;  // three rotating frontier bitmaps
     a.queue = e->queues;
@@ -696,11 +697,12 @@ extern "C" int sg_sssp(sg_engine *e, uint32_t source)
     a.stats = e->stats;
     a.trace = e->trace;
     a.exits = e->exits;
-    a.source = source;
+    a.source = dev_vertex(e->g, source);
     e->last_algo = 1;
     SG_HIP(hipEventRecord(e->ev0, e->stream));
     if ((rc = launch_sssp_bs(e, a, nullptr))) return rc;
     SG_HIP(hipEventRecord(e->ev1, e->stream));
+    if ((rc = engine_finish(e, 1))) return rc;
     e->timed = true;
     return SG_OK;
 }
@@ -714,7 +716,10 @@ extern "C" int sg_sssp_concurrent(sg_engine *e, const uint32_t *sources, int32_t
         if (sources[i] >= d.n) SG_FAIL(SG_ERR_RANGE, "sg_sssp_concurrent: source >= n");
     int rc = engine_ensure_bfs_state(e);
     if (rc) return rc;
-    if ((rc = engine_ensure_value(e, (uint64_t)d.n * 4 * (uint64_t)k))) return rc;
+    void *res = nullptr;
+    if ((rc = engine_result_rows(e, (uint64_t)k, &res))) return rc;
+    std::vector<uint32_t> dsrc(k);
+    for (int32_t i = 0; i < k; i++) dsrc[i] = dev_vertex(e->g, sources[i]);
     // one query per XCD: MI355X has 8 (a launch on fewer XCDs reports
     // SG_ERR_CAPACITY)
     const int per = 8;
@@ -722,10 +727,11 @@ extern "C" int sg_sssp_concurrent(sg_engine *e, const uint32_t *sources, int32_t
     for (int32_t i = 0; i < k; i += per) {
         const int nq = std::min<int32_t>(per, k - i);
         uint32_t *out[kMaxXcd];
-        for (int j = 0; j < nq; j++) out[j] = (uint32_t *)e->value + (size_t)(i + j) * d.n;
-        if ((rc = sssp_group_launch(e, sources + i, nq, out))) return rc;
+        for (int j = 0; j < nq; j++) out[j] = (uint32_t *)res + (size_t)(i + j) * d.n;
+        if ((rc = sssp_group_launch(e, dsrc.data() + i, nq, out))) return rc;
     }
     SG_HIP(hipEventRecord(e->ev1, e->stream));
+    if ((rc = engine_finish(e, (uint64_t)k))) return rc;
     e->timed = true;
     return SG_OK;
 }

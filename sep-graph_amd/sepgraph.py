@@ -33,6 +33,7 @@ ERRORS = {
 }
 CSR_SYMMETRIZE, CSR_DROP_SELF_LOOPS, CSR_DEDUP = 1, 2, 4
 PR_AUTO, PR_PULL_TD, PR_PUSH_DD = 0, 1, 2
+GRAPH_UNDIRECTED, GRAPH_RELABEL, GRAPH_NO_RELABEL = 1, 2, 4
 
 VARIANTS = {
     "auto": -1,
@@ -129,6 +130,7 @@ def lib() -> C.CDLL:
             "sg_csc_from_csr": ([C.c_uint32, _P, _P, _P, _P, _P, _P], C.c_int),
             "sg_pick_sources": ([C.c_uint32, _P, C.c_int, C.c_uint64, _P], C.c_int),
             "sg_graph_create": ([C.c_uint32, C.c_uint64, _P, _P, _P, _P, _P, _P, C.c_int, C.POINTER(_P)], C.c_int),
+            "sg_graph_create_ex": ([C.c_uint32, C.c_uint64, _P, _P, _P, _P, _P, _P, C.c_uint32, C.POINTER(_P)], C.c_int),
             "sg_graph_destroy": ([_P], C.c_int),
             "sg_graph_info": ([_P, _U32P, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)], C.c_int),
             "sg_engine_options_default": ([C.POINTER(EngineOptions)], None),
@@ -277,12 +279,15 @@ def grid_graph(rows=2048, cols=2048, p_delete=0.1, seed=1, weights=(1, 1000), we
 class Graph:
     """A device-resident graph (sg_graph)."""
 
-    def __init__(self, hg: HostGraph, with_weights: bool = True):
+    def __init__(self, hg: HostGraph, with_weights: bool = True, relabel: bool | None = None):
+        """relabel: None lets the library decide (degree order on the device for
+        skewed graphs), True / False force it either way."""
         self.host = hg
         h = _P()
         w = _u32(hg.w) if (with_weights and hg.w is not None) else None
         row, col = _u32(hg.row), _u32(hg.col)
-        _check(lib().sg_graph_create(hg.n, hg.m, _ptr(row), _ptr(col), _ptr(w), None, None, None, int(hg.undirected), C.byref(h)))
+        flags = (GRAPH_UNDIRECTED if hg.undirected else 0) | {None: 0, True: GRAPH_RELABEL, False: GRAPH_NO_RELABEL}[relabel]
+        _check(lib().sg_graph_create_ex(hg.n, hg.m, _ptr(row), _ptr(col), _ptr(w), None, None, None, flags, C.byref(h)))
         self.h = h
 
     def info(self):
