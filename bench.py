@@ -153,7 +153,7 @@ def main():
                     help="Poisson arrival rates (queries/s) for the serving run")
     ap.add_argument("--serve-count", type=int, default=1200, help="queries per arrival rate")
     ap.add_argument("--serve-plain", type=int, default=100, help="queries timed one at a time (plain baseline)")
-    ap.add_argument("--serve-batch", type=str, default="64,8,32", help="max sources per launch: bfs,sssp,bc")
+    ap.add_argument("--serve-batch", type=str, default="64,4,32", help="max queries per batch: bfs,sssp,bc")
     ap.add_argument("--copy-mb", type=int, default=0,
                     help="only run a plain device copy of this many MiB (counter calibration)")
     args = ap.parse_args()

@@ -350,29 +350,31 @@ __global__ SG_BFS_BOUNDS(BS) void k_bfs(BfsArgs a)
             // (2) the rest of the frontier: from the queue after a push round,
             //     from the bitmap after a pull round or if the queue overflowed.
             if (pol.pull_prev || rc.overflow) {
-                for (uint32_t wb = wbeg; wb < wend; wb += kWave) {
-                    const uint32_t nw = min((uint32_t)kWave, wend - wb);
-                    const uint64_t fl = (uint32_t)ln < nw ? R.fcur[wb + ln] : 0ull;
+                // words dealt round-robin: in a degree-ordered graph adjacent
+                // words hold vertices of similar degree
+                for (uint32_t wb = 0; wb < W; wb += nwaves * kWave) {
+                    const uint32_t w = wb + (uint32_t)ln * nwaves + gwave;
+                    const uint64_t fl = w < W ? R.fcur[w] : 0ull;
                     uint64_t todo = ballot(fl != 0);
                     while (todo) {
                         const int t = first_lane(todo);
                         todo &= todo - 1;
                         const uint64_t fw = shfl(fl, t);
-                        push_expand(R, (fw >> ln) & 1, (wb + t) * 64 + ln);
+                        push_expand(R, (fw >> ln) & 1, (wb + (uint32_t)t * nwaves + gwave) * 64 + ln);
                     }
                 }
             } else {
-                // an even slice of the queue per wave (a short queue puts one
-                // vertex on each of many waves rather than 64 on a few)
-                const uint32_t per = (rc.qlen + nwaves - 1) / nwaves;
-                const uint32_t qb = min(rc.qlen, gwave * per), qe = min(rc.qlen, qb + per);
-                for (uint32_t i0 = qb; i0 < qe; i0 += kWave) {
-                    const uint32_t i = i0 + ln;
-                    const bool valid = i < qe;
+                // entries dealt round-robin, lane ln of wave gw taking entry
+                // base + ln * nwaves + gw: a short queue puts one vertex on each
+                // of many waves, and no wave gets a run of entries staged
+                // together (neighbours of one hub)
+                for (uint32_t i0 = 0; i0 < rc.qlen; i0 += nwaves * kWave) {
+                    const uint32_t i = i0 + (uint32_t)ln * nwaves + gwave;
+                    const bool valid = i < rc.qlen;
+                    if (!ballot(valid)) break;
                     push_expand(R, valid, valid ? qin[i] : 0u);
                 }
             }
-            flush_stage(R.S, s_flush);
         } else {
             // Pull: sweep this wave's words, skipping ones the filter marks full.
             for (uint32_t wb = wbeg; wb < wend; wb += kWave) {
@@ -401,6 +403,7 @@ __global__ SG_BFS_BOUNDS(BS) void k_bfs(BfsArgs a)
                 }
             }
         }
+        flush_stage(R.S, s_flush);
 
         publish_counts<WPB>(R.S.nxt, R.scout, R.count, R.huge, s_red);
         pol.pull_prev = pull;

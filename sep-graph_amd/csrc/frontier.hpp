@@ -118,8 +118,12 @@ struct Stager {
 };
 
 // Stage this wave's `found` vertices: into LDS while the block buffer has
-// room, the remainder straight to the global queue. Wave-convergent.
-__device__ __forceinline__ void stage(const Stager &S, bool found, uint32_t v)
+// room, the remainder straight to the global queue (spill), or, without
+// spill, mark the next queue overflowed so the round after reads the frontier
+// bitmap instead (pull rounds: a frontier too large for the LDS buffers is
+// followed by another pull round, and same-address global appends from every
+// wave would serialise). Wave-convergent.
+__device__ __forceinline__ void stage(const Stager &S, bool found, uint32_t v, bool spill = true)
 {
     const uint64_t m = ballot(found);
     if (!m) return;
@@ -132,6 +136,10 @@ __device__ __forceinline__ void stage(const Stager &S, bool found, uint32_t v)
     if (to_lds) S.s_q[slot] = v;
     const bool to_glb = found && !to_lds;
     const uint64_t gm = ballot(to_glb);
+    if (gm && !spill) {
+        if (lane() == first_lane(gm)) S.nxt->overflow = 1;
+        return;
+    }
     if (gm) {
         const int l2 = first_lane(gm);
         uint32_t gb = 0;

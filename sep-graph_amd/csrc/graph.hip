@@ -299,19 +299,19 @@ extern "C" int sg_graph_create_ex(uint32_t n, uint64_t m, const uint32_t *row_st
         *count = (uint32_t)h.size();
         return r;
     };
-    auto chunk_list = [&](const uint32_t *offs, uint32_t threshold, std::vector<uint2> &huge,
+    auto chunk_list = [&](const uint32_t *offs, uint32_t threshold, uint32_t chunk, std::vector<uint2> &huge,
                           std::vector<uint2> *hubs) {
         for (uint32_t u = 0; u < n; u++) {
             const uint32_t deg = offs[u + 1] - offs[u];
             if (deg <= threshold) continue;
-            if (hubs) hubs->push_back(make_uint2((uint32_t)huge.size(), (deg + kHugeChunk - 1) / kHugeChunk));
-            for (uint32_t c = 0; c < deg; c += kHugeChunk) huge.push_back(make_uint2(u, offs[u] + c));
+            if (hubs) hubs->push_back(make_uint2((uint32_t)huge.size(), (deg + chunk - 1) / chunk));
+            for (uint32_t c = 0; c < deg; c += chunk) huge.push_back(make_uint2(u, offs[u] + c));
         }
     };
     {
         std::vector<uint2> out_chunks, in_chunks, in_hubs;
-        chunk_list(row_start, kHugeDegree, out_chunks, nullptr);
-        chunk_list(undirected ? row_start : cs, kPullHubDegree, in_chunks, &in_hubs);
+        chunk_list(row_start, kHugeDegree, kHugeChunk, out_chunks, nullptr);
+        chunk_list(undirected ? row_start : cs, kPullHubDegree, kPullChunk, in_chunks, &in_hubs);
         if (!rc) rc = upload2(out_chunks, &d.huge, &d.nhuge);
         if (!rc) rc = upload2(in_chunks, &d.hugein, &d.nhugein);
         if (!rc) rc = upload2(in_hubs, &d.hubin, &d.nhubin);

@@ -231,7 +231,7 @@ __device__ __forceinline__ void ms_discover(const DevGraph &g, const uint64_t *s
             const uint2 dsc = g.hugein[c];
             const uint64_t unseen = bm & ~seen[dsc.x];
             if (!unseen) continue;
-            const uint32_t end = min(g.cstart[dsc.x + 1], dsc.y + kHugeChunk);
+            const uint32_t end = min(g.cstart[dsc.x + 1], dsc.y + kPullChunk);
             uint64_t o = 0;
             for (uint32_t e0 = dsc.y; e0 < end; e0 += kWave) {
                 const uint32_t e = e0 + ln;
@@ -423,7 +423,7 @@ __global__ __launch_bounds__(BS) void k_bc(BcArgs a)
             const uint2 dsc = g.hugein[c];
             const uint64_t nw = Fn[dsc.x];
             if (!nw) continue;
-            const uint32_t end = min(g.cstart[dsc.x + 1], dsc.y + kHugeChunk);
+            const uint32_t end = min(g.cstart[dsc.x + 1], dsc.y + kPullChunk);
             const double s = masked_row_sum(g.csrc, Fc, a.sigma, dsc.y, end, nw);
             if (s != 0.0) atomicAdd(&a.sigma[(size_t)dsc.x * 64 + ln], s);
         }
@@ -726,14 +726,23 @@ struct MsRelax {
     unsigned long long scout, count;
 };
 
+// Edge steps a group of lanes keeps in flight in the batched relaxation.
+#ifndef SG_MS_RELAX_STEPS
+#define SG_MS_RELAX_STEPS 4
+#endif
+constexpr int kMsRelaxSteps = SG_MS_RELAX_STEPS;
+
 // Relax the out-edges [b, e) of u for the queries in `mask`. The wave is
 // split into 64 / 2^qbits groups of 2^qbits lanes (lane q of a group = query
-// q) and each group relaxes one edge per step, so a batch of B queries keeps
-// about 64 / B edges in flight per wave. A group's leader publishes the
-// queries whose distance it lowered into the next round's mask of the edge's
-// head, and queues the head when it is the first to do so.
+// q); each group takes kMsRelaxSteps edges per step, with the loads and
+// atomics of all of them issued before any result is used, so a batch of B
+// queries keeps about 64 / B x kMsRelaxSteps edges in flight per wave. A
+// group's leader publishes the queries whose distance it lowered into the
+// next round's mask of the edge's head; the wave queues the heads whose mask
+// it was first to set, with one append per wave and step.
 __device__ __forceinline__ void ms_relax_range(MsRelax &R, uint32_t u, uint32_t b, uint32_t e, uint64_t mask)
 {
+    constexpr int U = kMsRelaxSteps;
     const MsSsspArgs &a = *R.a;
     const DevGraph &g = a.g;
     const int ln = lane();
@@ -742,29 +751,48 @@ __device__ __forceinline__ void ms_relax_range(MsRelax &R, uint32_t u, uint32_t 
     const uint64_t gmask = qb == 6 ? ~0ull : ((1ull << (1u << qb)) - 1);
     const bool mine = (mask >> q) & 1;
     const uint32_t du = mine ? a.dist[(size_t)u * 64 + q] : 0xFFFFFFFFu;
-    for (uint32_t e0 = b; e0 < e; e0 += ngrp) {
-        const uint32_t ed = e0 + grp;
-        bool imp = false;
-        uint32_t v = 0;
-        if (ed < e) {
-            v = g.col[ed];
-            if (mine) {
-                const uint32_t w = g.w ? g.w[ed] : 1u;
-                const uint32_t nd = du > 0xFFFFFFFFu - w ? 0xFFFFFFFFu : du + w;
-                uint32_t *p = &a.dist[(size_t)v * 64 + q];
-                if (nd < *p) imp = nd < atomicMin(p, nd);
-            }
+    for (uint32_t e0 = b; e0 < e; e0 += ngrp * U) {
+        uint32_t v[U], nd[U], cur[U];
+        bool act[U];
+#pragma unroll
+        for (int s = 0; s < U; s++) {
+            const uint32_t ed = e0 + grp + (uint32_t)s * ngrp;
+            act[s] = ed < e;
+            v[s] = act[s] ? g.col[ed] : 0u;
+            const uint32_t w = (act[s] && mine) ? (g.w ? g.w[ed] : 1u) : 0u;
+            nd[s] = du > 0xFFFFFFFFu - w ? 0xFFFFFFFFu : du + w;
+            act[s] = act[s] && mine;
         }
-        const uint64_t im = ballot(imp);
-        if (!im) continue;
-        const uint64_t gm = (im >> (grp << qb)) & gmask;
-        if (q == 0 && gm) {
-            const unsigned long long old = atomicOr((unsigned long long *)&R.Fn[v], (unsigned long long)gm);
-            if (old == 0) {
-                const uint32_t pos = atomicAdd(R.cnt_next, 1u);
-                if (pos < g.n) R.ring_next[pos] = v;
+#pragma unroll
+        for (int s = 0; s < U; s++) cur[s] = act[s] ? a.dist[(size_t)v[s] * 64 + q] : 0u;
+        bool imp[U];
+#pragma unroll
+        for (int s = 0; s < U; s++) {
+            imp[s] = false;
+            if (act[s] && nd[s] < cur[s]) imp[s] = nd[s] < atomicMin(&a.dist[(size_t)v[s] * 64 + q], nd[s]);
+        }
+        bool first[U];
+#pragma unroll
+        for (int s = 0; s < U; s++) {
+            const uint64_t im = ballot(imp[s]);
+            const uint64_t gm = (im >> (grp << qb)) & gmask;
+            first[s] = false;
+            if (q == 0 && gm)
+                first[s] = atomicOr((unsigned long long *)&R.Fn[v[s]], (unsigned long long)gm) == 0;
+        }
+#pragma unroll
+        for (int s = 0; s < U; s++) {
+            const uint64_t fm = ballot(first[s]);
+            if (!fm) continue;
+            const int l0 = first_lane(fm);
+            uint32_t base = 0;
+            if (ln == l0) base = atomicAdd(R.cnt_next, (uint32_t)__popcll(fm));
+            base = shfl(base, l0);
+            if (first[s]) {
+                const uint32_t pos = base + (uint32_t)rank_in(fm);
+                if (pos < g.n) R.ring_next[pos] = v[s];
                 else a.ctl[1] = 1;
-                R.scout += g.row[v + 1] - g.row[v];
+                R.scout += g.row[v[s] + 1] - g.row[v[s]];
                 R.count++;
             }
         }

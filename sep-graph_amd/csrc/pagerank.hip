@@ -223,7 +223,7 @@ __global__ __launch_bounds__(BS) void k_pr_pull(PrArgs a)
         // (A1) chunk sums of the in-hubs
         for (uint32_t c = gwave; c < g.nhugein; c += nwaves) {
             const uint2 dsc = g.hugein[c];
-            const uint32_t end = min(g.cstart[dsc.x + 1], dsc.y + kHugeChunk);
+            const uint32_t end = min(g.cstart[dsc.x + 1], dsc.y + kPullChunk);
             double s = 0.0;
             for (uint32_t e0 = dsc.y; e0 < end; e0 += kWave) {
                 const uint32_t e = e0 + ln;
@@ -513,7 +513,7 @@ __global__ __launch_bounds__(BS) void k_pr_dd(PrDdArgs a)
         } else {
             for (uint32_t c = gwave; c < g.nhugein; c += nwaves) {
                 const uint2 dsc = g.hugein[c];
-                const uint32_t end = min(g.cstart[dsc.x + 1], dsc.y + kHugeChunk);
+                const uint32_t end = min(g.cstart[dsc.x + 1], dsc.y + kPullChunk);
                 double s = 0.0;
                 for (uint32_t e0 = dsc.y; e0 < end; e0 += kWave) {
                     const uint32_t e = e0 + ln;

@@ -13,10 +13,10 @@
 //     -- so requests that arrive while a batch runs join the next one;
 //   * few launches per batch, each writing every query's result straight into
 //     its row of the pool: BFS and BC-dependency batches run the bit-packed
-//     multi-source kernels (up to 64 sources per launch); SSSP batches run up
-//     to eight single-source queries concurrently, one per XCD, because an
-//     SSSP query's rounds are latency-bound and its bit-packed batch does
-//     redundant relaxations.
+//     multi-source kernels (up to 64 sources per launch); SSSP batches run
+//     their queries one after another, each over the whole grid, which on
+//     RMAT-22 is faster per query than a bit-packed batch (whose
+//     Bellman-Ford rounds repeat relaxations) or one query per XCD.
 // Batches are serialised on the engine's stream: a persistent kernel assumes
 // all of its blocks are co-resident, which two concurrent launches would not
 // be. Host work (queueing, result hand-off) overlaps the running batch.
@@ -48,9 +48,6 @@ int kind_to_ms(int kind)
 }
 
 enum { kQueued = 0, kRunning = 1, kDone = 2, kFailed = 3 };
-
-// SSSP queries per concurrent launch: one per XCD of an MI355X.
-constexpr int kXcdQueries = 8;
 
 }  // namespace
 
@@ -151,14 +148,11 @@ struct sg_server {
             uint32_t *work = (uint32_t *)e->work;
             bool ok;
             if (kind == SG_QUERY_SSSP) {
-                // one query per XCD, each launch writing a row per query
+                // one whole-grid query after another, each into its row
                 ok = true;
-                for (uint32_t i = 0; ok && i < b; i += kXcdQueries) {
-                    const int nq = (int)std::min<uint32_t>(kXcdQueries, b - i);
-                    uint32_t *out[kXcdQueries];
-                    for (int j = 0; j < nq; j++)
-                        out[j] = via_work ? work + (i + j) * n : (uint32_t *)pool + (size_t)h_rows[i + j] * n;
-                    ok = sssp_group_launch(e, h_src + i, nq, out) == SG_OK;
+                for (uint32_t i = 0; ok && i < b; i++) {
+                    uint32_t *out = via_work ? work + i * n : (uint32_t *)pool + (size_t)h_rows[i] * n;
+                    ok = sssp_single_launch(e, h_src[i], out) == SG_OK && sssp_single_status(e) == SG_OK;
                 }
             } else {
                 ok = hipMemcpyAsync(d_src, h_src, b * 4, hipMemcpyHostToDevice, e->stream) == hipSuccess &&
@@ -169,8 +163,7 @@ struct sg_server {
             }
             for (uint32_t j = 0; ok && via_work && j < b; j++)
                 ok = engine_gather_rows(e, work + j * n, (uint32_t *)pool + (size_t)h_rows[j] * n, 1) == SG_OK;
-            if (kind == SG_QUERY_SSSP) ok = ok && sssp_group_status(e) == SG_OK;
-            else ok = ok && sg_engine_sync(e) == SG_OK;
+            ok = ok && sg_engine_sync(e) == SG_OK;
             const double t1 = now_s();
 
             lk.lock();
@@ -225,9 +218,9 @@ extern "C" int sg_server_create(sg_graph *g, const sg_server_config *cfg, const 
         sg_server_destroy(p);
         return code;
     };
-    // working arenas of every kind (the largest wins) and the result pool
-    // SSSP runs as concurrent single-source queries (one per XCD) with their
-    // own state; BFS and BC-dependency batches use the bit-packed kernels
+    // working arenas of every kind (the largest wins) and the result pool;
+    // SSSP runs single-source queries on the engine's own frontier state,
+    // BFS and BC-dependency batches use the bit-packed kernels
     if ((rc = ms_prepare(s->e, kMsBfs))) return fail(rc);
     if ((rc = ms_prepare(s->e, kMsBcDep))) return fail(rc);
     if (g->relabeled && (rc = engine_ensure_work(s->e, 64ull * g->dev.n * 4))) return fail(rc);

@@ -62,11 +62,21 @@ struct DevGraph {
 };
 
 // A vertex whose out-degree exceeds this is expanded through the static chunk
-// list instead of by a single wave.
-constexpr uint32_t kHugeDegree = 1024;
-constexpr uint32_t kHugeChunk = 256;
-// In-degree above which a pull round splits a vertex's in-edges into chunks.
+// list instead of by a single wave: a wave expands the out-edges of up to 64
+// vertices together, one 64-edge step (a few dependent memory round trips)
+// at a time, so the bound keeps any wave's share of a push round short.
+#ifndef SG_HUGE_DEGREE
+#define SG_HUGE_DEGREE 1024
+#endif
+#ifndef SG_HUGE_CHUNK
+#define SG_HUGE_CHUNK 256
+#endif
+constexpr uint32_t kHugeDegree = SG_HUGE_DEGREE;
+constexpr uint32_t kHugeChunk = SG_HUGE_CHUNK;
+// In-degree above which a pull round splits a vertex's in-edges into chunks
+// of kPullChunk.
 constexpr uint32_t kPullHubDegree = 64;
+constexpr uint32_t kPullChunk = 256;
 
 }  // namespace sg
 
@@ -214,6 +224,10 @@ int ms_launch(sg_engine *e, int kind, const uint32_t *dev_src, uint32_t nsrc, vo
 int sssp_group_launch(sg_engine *e, const uint32_t *sources, int nq, uint32_t *const *out);
 // Worst status of the last such launch (synchronises the stream).
 int sssp_group_status(sg_engine *e);
+// One SSSP query over the whole grid (device source id) into the device row
+// `out`, asynchronously; sssp_single_status synchronises and reports it.
+int sssp_single_launch(sg_engine *e, uint32_t source, uint32_t *out);
+int sssp_single_status(sg_engine *e);
 
 // Device allocation owned by the engine (freed with it); reports SG_ERR_NOMEM.
 int engine_alloc(void **p, uint64_t bytes);

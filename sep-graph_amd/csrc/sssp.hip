@@ -27,6 +27,17 @@ namespace {
 
 constexpr uint32_t kInf = 0xFFFFFFFFu;
 
+// 64-edge batches a pull step keeps in flight per word, and hub in-edge
+// chunks a wave pulls at once.
+#ifndef SG_SSSP_PULL_STEPS
+#define SG_SSSP_PULL_STEPS 4
+#endif
+#ifndef SG_SSSP_HUB_CHUNKS
+#define SG_SSSP_HUB_CHUNKS 2
+#endif
+constexpr int kPullSteps = SG_SSSP_PULL_STEPS;
+constexpr int kHubChunks = SG_SSSP_HUB_CHUNKS;
+
 // Table of a launch with one query per XCD (passed by value).
 struct SsspGroups {
     uint32_t *ticket;          // kMaxXcd counters zeroed before the launch: block ranks
@@ -170,7 +181,11 @@ __device__ __forceinline__ void push_expand(Round &R, bool valid, uint32_t u)
 }
 
 // Pull step for the 64 vertices of word w: each lane's vertex takes the
-// minimum of dist[u] + w over in-neighbours u in the frontier.
+// minimum of dist[u] + w over in-neighbours u in the frontier. The word's
+// concatenated in-edge lists are walked U x 64 edges per step, every load of
+// a stage issued for all U batches before any is used, so one memory latency
+// covers U batches.
+template <int U>
 __device__ __forceinline__ void pull_word(Round &R, uint32_t w)
 {
     const SsspArgs &a = *R.a;
@@ -191,32 +206,41 @@ __device__ __forceinline__ void pull_word(Round &R, uint32_t w)
         const uint32_t total = shfl(incl, kWave - 1);
         if (total == 0) break;
         const uint32_t excl = incl - r;
-        const uint32_t j = (uint32_t)ln;
-        const int k = owner_lane(incl, j);
-        const uint32_t bk = shfl(b, k), xk = shfl(excl, k);
-        uint32_t cand = kInf;
-        if (j < total) {
-            const uint32_t ed = bk + (j - xk);
-            const uint32_t u = g.csrc[ed];
-            if (in_frontier(R, u)) {
-                const uint32_t du = a.dist[u];
-                if (du != kInf) cand = sat_add(du, edge_w(g.cw, ed));
-            }
-        }
-        // segmented minimum over lanes with the same owner (owners are contiguous)
-        uint32_t m = cand;
+        int k[U];
+        uint32_t ed[U], u[U];
 #pragma unroll
-        for (int d = 1; d < kWave; d <<= 1) {
-            const uint32_t y = __shfl_up(m, d, kWave);
-            const int ko = __shfl_up(k, d, kWave);
-            if (ln >= d && ko == k) m = min(m, y);
+        for (int s = 0; s < U; s++) {
+            const uint32_t j = (uint32_t)(s * kWave + ln);
+            k[s] = owner_lane(incl, j < total ? j : 0u);
+            ed[s] = shfl(b, k[s]) + (j - shfl(excl, k[s]));
+            u[s] = kInf;
+            if (j < total) u[s] = g.csrc[ed[s]];
         }
-        const uint32_t lo = min(excl, 64u), hi = min(incl, 64u), cov = hi - lo;
-        const uint32_t seg = shfl(m, (int)(hi > 0 ? hi - 1 : 0));
-        if (cov) {
-            best = min(best, seg);
-            b += cov;
+        uint32_t du[U], wt[U];
+#pragma unroll
+        for (int s = 0; s < U; s++) {
+            const bool f = u[s] != kInf && in_frontier(R, u[s]);
+            du[s] = f ? a.dist[u[s]] : kInf;
+            wt[s] = f ? edge_w(g.cw, ed[s]) : 0u;
         }
+#pragma unroll
+        for (int s = 0; s < U; s++) {
+            // segmented minimum over lanes with the same owner (owners are
+            // contiguous), then each vertex takes the last lane of its run
+            uint32_t m = du[s] != kInf ? sat_add(du[s], wt[s]) : kInf;
+#pragma unroll
+            for (int d = 1; d < kWave; d <<= 1) {
+                const uint32_t y = __shfl_up(m, d, kWave);
+                const int ko = __shfl_up(k[s], d, kWave);
+                if (ln >= d && ko == k[s]) m = min(m, y);
+            }
+            const uint32_t base = (uint32_t)(s * kWave);
+            const uint32_t lo = min(max(excl, base), base + 64u) - base;
+            const uint32_t hi = min(max(incl, base), base + 64u) - base;
+            const uint32_t seg = shfl(m, (int)(hi > 0 ? hi - 1 : 0));
+            if (hi > lo) best = min(best, seg);
+        }
+        b += min(incl, (uint32_t)(U * kWave)) - min(excl, (uint32_t)(U * kWave));
     }
     const bool improved = v < g.n && best < dv0;
     if (improved) {
@@ -234,37 +258,56 @@ __device__ __forceinline__ void pull_word(Round &R, uint32_t w)
     }
 }
 
-// Pull for one chunk of a hub's in-edges: the chunk's minimum candidate is
-// merged with atomicMin, as several waves share the hub.
-__device__ __forceinline__ void pull_hub_chunk(Round &R, uint32_t v, uint32_t beg)
+// Pull for up to NC chunks of hubs' in-edges at once (each load of a stage
+// issued for every chunk before any is used). A chunk's minimum candidate is
+// merged into its hub with atomicMin, as several waves share a hub.
+template <int NC>
+__device__ __forceinline__ void pull_hub_chunks(Round &R, const uint2 (&dsc)[NC], int nc)
 {
     const SsspArgs &a = *R.a;
     const DevGraph &g = a.g;
     const int ln = lane();
-    const uint32_t end = min(g.cstart[v + 1], beg + kHugeChunk);
-    uint32_t m = kInf;
-    for (uint32_t e0 = beg; e0 < end; e0 += kWave) {
-        const uint32_t ed = e0 + ln;
-        if (ed < end) {
-            const uint32_t u = g.csrc[ed];
-            if (in_frontier(R, u)) {
-                const uint32_t du = a.dist[u];
-                if (du != kInf) m = min(m, sat_add(du, edge_w(g.cw, ed)));
-            }
-        }
-    }
+    constexpr int U = kPullChunk / kWave;
+    uint32_t end[NC], u[NC][U], du[NC][U], wt[NC][U];
 #pragma unroll
-    for (int d = kWave / 2; d > 0; d >>= 1) m = min(m, (uint32_t)__shfl_xor(m, d, kWave));
-    if (ln == 0 && m < a.dist[v]) {
-        const uint32_t old = atomicMin(&a.dist[v], m);
-        if (m < old) {
-            atomicOr((unsigned long long *)&R.fnext[v >> 6], 1ull << (v & 63));
-            if (a.stamp[v] != R.L + 2) {
-                a.stamp[v] = R.L + 2;
-                const uint32_t dv = g.row[v + 1] - g.row[v];
-                R.scout += dv;
-                R.count++;
-                R.huge |= dv > kHugeDegree;
+    for (int c = 0; c < NC; c++) end[c] = c < nc ? min(g.cstart[dsc[c].x + 1], dsc[c].y + kPullChunk) : 0u;
+#pragma unroll
+    for (int c = 0; c < NC; c++)
+#pragma unroll
+        for (int s = 0; s < U; s++) {
+            const uint32_t ed = dsc[c].y + (uint32_t)(s * kWave + ln);
+            u[c][s] = ed < end[c] ? g.csrc[ed] : kInf;
+        }
+#pragma unroll
+    for (int c = 0; c < NC; c++)
+#pragma unroll
+        for (int s = 0; s < U; s++) {
+            const uint32_t ed = dsc[c].y + (uint32_t)(s * kWave + ln);
+            const bool f = u[c][s] != kInf && in_frontier(R, u[c][s]);
+            du[c][s] = f ? a.dist[u[c][s]] : kInf;
+            wt[c][s] = f ? edge_w(g.cw, ed) : 0u;
+        }
+#pragma unroll
+    for (int c = 0; c < NC; c++) {
+        if (c >= nc) break;
+        const uint32_t v = dsc[c].x;
+        uint32_t m = kInf;
+#pragma unroll
+        for (int s = 0; s < U; s++)
+            if (du[c][s] != kInf) m = min(m, sat_add(du[c][s], wt[c][s]));
+#pragma unroll
+        for (int d = kWave / 2; d > 0; d >>= 1) m = min(m, (uint32_t)__shfl_xor(m, d, kWave));
+        if (ln == 0 && m < a.dist[v]) {
+            const uint32_t old = atomicMin(&a.dist[v], m);
+            if (m < old) {
+                atomicOr((unsigned long long *)&R.fnext[v >> 6], 1ull << (v & 63));
+                if (a.stamp[v] != R.L + 2) {
+                    a.stamp[v] = R.L + 2;
+                    const uint32_t dv = g.row[v + 1] - g.row[v];
+                    R.scout += dv;
+                    R.count++;
+                    R.huge |= dv > kHugeDegree;
+                }
             }
         }
     }
@@ -284,8 +327,6 @@ __device__ __forceinline__ void sssp_run(const SsspArgs &a, uint32_t brank, uint
     const int ln = lane();
     uint32_t gen = threadIdx.x == 0 ? grid_gen(a.bar) : 0u;
     const long long t_start = wall_clock64();
-    const uint32_t wpw = (W + nwaves - 1) / nwaves;
-    const uint32_t wbeg = min(W, gwave * wpw), wend = min(W, wbeg + wpw);
 
     __shared__ unsigned long long s_red[2][WPB];
     __shared__ unsigned long long s_lvl[5];
@@ -409,23 +450,28 @@ __device__ __forceinline__ void sssp_run(const SsspArgs &a, uint32_t brank, uint
                 }
             }
             if (pol.pull_prev) {
-                for (uint32_t wb = wbeg; wb < wend; wb += kWave) {
-                    const uint32_t nw = min((uint32_t)kWave, wend - wb);
-                    const uint64_t fl = (uint32_t)ln < nw ? R.fcur[wb + ln] : 0ull;
+                // after a pull round, from its bitmap, words dealt round-robin
+                // (in a degree-ordered graph adjacent words hold vertices of
+                // similar degree)
+                for (uint32_t wb = 0; wb < W; wb += nwaves * kWave) {
+                    const uint32_t w = wb + (uint32_t)ln * nwaves + gwave;
+                    const uint64_t fl = w < W ? R.fcur[w] : 0ull;
                     uint64_t todo = ballot(fl != 0);
                     while (todo) {
                         const int t = first_lane(todo);
                         todo &= todo - 1;
                         const uint64_t fw = shfl(fl, t);
-                        push_expand(R, (fw >> ln) & 1, (wb + t) * 64 + ln);
+                        push_expand(R, (fw >> ln) & 1, (wb + (uint32_t)t * nwaves + gwave) * 64 + ln);
                     }
                 }
             } else if (!rc.overflow) {
-                const uint32_t per = (rc.qlen + nwaves - 1) / nwaves;
-                const uint32_t qb = min(rc.qlen, gwave * per), qe = min(rc.qlen, qb + per);
-                for (uint32_t i0 = qb; i0 < qe; i0 += kWave) {
-                    const uint32_t i = i0 + ln;
-                    const bool valid = i < qe;
+                // the queue, entries dealt round-robin: lane ln of wave gw
+                // takes entry base + ln * nwaves + gw, so no wave gets a run of
+                // entries staged together
+                for (uint32_t i0 = 0; i0 < rc.qlen; i0 += nwaves * kWave) {
+                    const uint32_t i = i0 + (uint32_t)ln * nwaves + gwave;
+                    const bool valid = i < rc.qlen;
+                    if (!ballot(valid)) break;
                     push_expand(R, valid, valid ? qin[i] : 0u);
                 }
             } else {
@@ -434,23 +480,30 @@ __device__ __forceinline__ void sssp_run(const SsspArgs &a, uint32_t brank, uint
                     push_expand(R, v < n && a.stamp[v] == L + 1, v);
                 }
             }
-            flush_stage(R.S, s_flush);
         } else {
             // Pull: (1) hubs' in-edges through the static chunk list, spread
-            // over all waves; (2) every other vertex with in-edges, a word at a
-            // time from this wave's range.
-            for (uint32_t c = gwave; c < g.nhugein; c += nwaves) {
-                const uint2 dsc = g.hugein[c];
-                pull_hub_chunk(R, dsc.x, dsc.y);
+            // over all waves, kHubChunks chunks per wave at a time; (2) every
+            // other vertex with in-edges, a word at a time.
+            for (uint32_t c0 = gwave; c0 < g.nhugein; c0 += kHubChunks * nwaves) {
+                uint2 dsc[kHubChunks];
+                int nc = 0;
+#pragma unroll
+                for (int i = 0; i < kHubChunks; i++) {
+                    const uint32_t c = c0 + (uint32_t)i * nwaves;
+                    dsc[i] = c < g.nhugein ? g.hugein[c] : make_uint2(0u, 0u);
+                    nc += c < g.nhugein ? 1 : 0;
+                }
+                pull_hub_chunks<kHubChunks>(R, dsc, nc);
             }
-            // words interleaved over waves: RMAT's heavy vertices cluster at
-            // low ids, so contiguous ranges would leave a few waves with most
-            // of the in-edges
+            // words interleaved over waves: heavy vertices cluster at low ids
+            // (RMAT, degree-ordered graphs), so contiguous ranges would leave a
+            // few waves with most of the in-edges
             for (uint32_t w = gwave; w < W; w += nwaves) {
                 if (g.inert && g.inert[w] == ~0ull) continue;
-                pull_word(R, w);
+                pull_word<kPullSteps>(R, w);
             }
         }
+        flush_stage(R.S, s_flush);
 
         publish_counts<WPB>(R.S.nxt, R.scout, R.count, R.huge, s_red);
         if (a.delta) {
@@ -657,6 +710,44 @@ int sssp_group_launch(sg_engine *e, const uint32_t *sources, int nq, uint32_t *c
     e->last_algo = 6;
     e->group_status = a.gr.status;
     return launch_sssp_bs(e, a, (uint32_t *)(base + o_bar));
+}
+
+// One SSSP query over the whole grid from device vertex `source`, writing its
+// distances to the device row `out`. Asynchronous; its status lands in the
+// engine's stats, overwritten by the next run (see sssp_single_status).
+int sssp_single_launch(sg_engine *e, uint32_t source, uint32_t *out)
+{
+    if (source >= e->g->dev.n || !out) SG_FAIL(SG_ERR_RANGE, "sssp_single_launch: bad source or row");
+    int rc = engine_ensure_bfs_state(e);
+    if (rc) return rc;
+    if ((rc = engine_ensure_stamp(e))) return rc;
+    SsspArgs a;
+    if ((rc = sssp_args(e, a))) return rc;
+    a.dist = out;
+    a.stamp = e->stamp;
+    a.fbit = e->bitmaps;
+    a.queue = e->queues;
+    a.qcap = e->qcap;
+    a.ctr = (FrontierCtr *)e->ctr;
+    a.stats = e->stats;
+    a.trace = nullptr;
+    a.exits = nullptr;
+    a.source = source;
+    e->last_algo = 1;
+    return launch_sssp_bs(e, a, nullptr);
+}
+
+// Status of the last single-query launch (synchronises the stream).
+int sssp_single_status(sg_engine *e)
+{
+    SG_HIP(hipStreamSynchronize(e->stream));
+    int32_t st = 0;
+    SG_HIP(hipMemcpy(&st, (const char *)e->stats + offsetof(DevStats, status), 4, hipMemcpyDeviceToHost));
+    if (st != SG_OK) {
+        e->needs_bar_reset = true;
+        SG_FAIL(st, "SSSP: the query reported an error");
+    }
+    return SG_OK;
 }
 
 // Worst status of the last grouped launch (synchronises the stream).

@@ -156,6 +156,19 @@ extern "C" int sg_autotune(sg_engine *e, const uint32_t *sources, int32_t k, con
         for (int i = 0; i < 64; i++) src[i] = sources[i % k];
         for (int kind = 0; kind < SG_QUERY_KINDS; kind++) {
             double best_qps = -1.0;
+            if (kind == SG_QUERY_SSSP) {
+                // the server runs an SSSP batch as whole-grid queries one
+                // after another: throughput is one query's, so the limit only
+                // bounds how long a batch holds the GPU (half the budget)
+                const double t = mean_ms(e, sources, std::min(k, 4), [&](uint32_t s) { return sg_sssp(e, s); });
+                if (t < 0) return SG_ERR_HIP;
+                int b = 1;
+                while (b < 8 && 2 * b * t <= 0.5 * c.latency_budget_ms) b *= 2;
+                out->max_batch[kind] = b;
+                out->batch_ms[kind] = b * t;
+                out->batch_qps[kind] = t > 0 ? 1e3 / t : 0.0;
+                continue;
+            }
             for (int b : sizes) {
                 auto run = [&]() {
                     return kind == SG_QUERY_BFS ? sg_bfs_multi(e, src.data(), b, b)
