@@ -69,6 +69,10 @@ struct BfsArgs {
     uint32_t total_workload0;
     uint32_t max_rounds;
     uint32_t probe;   // launch is a residency probe of the grid
+    // degree-ordered graphs: level is in device order, and the kernel ends by
+    // writing out[v] = level[inv[v]] (the caller's order); null otherwise
+    int32_t *out;
+    const uint32_t *inv;
 };
 
 struct Round {
@@ -90,16 +94,16 @@ __device__ __forceinline__ void push_visit(Round &R, bool act, uint32_t v)
     bool found = false;
     if (act && a.level[v] == -1) {
         a.level[v] = R.L + 1;
-        // the next frontier's bitmap, so that a pull round after this one
-        // tests membership in 512 KiB of L2-resident bits instead of the
-        // level array (one atomic per discovery, none per edge)
-        atomicOr((unsigned long long *)&R.fnext[v >> 6], 1ull << (v & 63));
         const uint32_t dv = a.g.row[v + 1] - a.g.row[v];
         R.scout += dv;
         R.count++;
         R.huge |= dv > kHugeDegree;
         found = true;
     }
+    // the next frontier's bitmap, so that a pull round after this one tests
+    // membership in 512 KiB of L2-resident bits instead of the level array
+    // (at most one atomic per discovery and word run, none per edge)
+    wave_set_bits(R.fnext, found, v);
     stage(R.S, found, v);
 }
 
@@ -247,9 +251,6 @@ __global__ SG_BFS_BOUNDS(BS) void k_bfs(BfsArgs a)
     const uint32_t xcc = xcc_id();
     uint32_t gen = threadIdx.x == 0 ? grid_gen(a.bar) : 0u;
     const long long t_start = wall_clock64();
-    // contiguous range of bitmap words swept by this wave
-    const uint32_t wpw = (W + nwaves - 1) / nwaves;
-    const uint32_t wbeg = min(W, gwave * wpw), wend = min(W, wbeg + wpw);
 
     __shared__ unsigned long long s_red[2][WPB];
     __shared__ unsigned long long s_lvl[5];
@@ -376,11 +377,15 @@ __global__ SG_BFS_BOUNDS(BS) void k_bfs(BfsArgs a)
                 }
             }
         } else {
-            // Pull: sweep this wave's words, skipping ones the filter marks full.
-            for (uint32_t wb = wbeg; wb < wend; wb += kWave) {
-                const uint32_t nw = min((uint32_t)kWave, wend - wb);
-                const uint64_t vl = (uint32_t)ln < nw ? a.vis[wb + ln] : ~0ull;
-                if ((uint32_t)ln < nw && vl == ~0ull) R.fnext[wb + ln] = 0;
+            // Pull: sweep this wave's words, skipping ones the filter marks
+            // full. Words are dealt round-robin (lane ln of wave gw takes word
+            // base + ln * nwaves + gw): unreached high-in-degree vertices walk
+            // long in-edge lists, and they cluster at low ids (RMAT,
+            // degree-ordered graphs).
+            for (uint32_t wb = 0; wb < W; wb += nwaves * kWave) {
+                const uint32_t wl = wb + (uint32_t)ln * nwaves + gwave;
+                const uint64_t vl = wl < W ? a.vis[wl] : ~0ull;
+                if (wl < W && vl == ~0ull) R.fnext[wl] = 0;
                 uint64_t todo = ballot(vl != ~0ull);
                 while (todo) {
                     constexpr int G = kPullGroup;
@@ -394,7 +399,7 @@ __global__ SG_BFS_BOUNDS(BS) void k_bfs(BfsArgs a)
                         if (todo) {
                             const int t = first_lane(todo);
                             todo &= todo - 1;
-                            wg[i] = wb + t;
+                            wg[i] = wb + (uint32_t)t * nwaves + gwave;
                             vwg[i] = shfl(vl, t);
                             ng++;
                         }
@@ -411,6 +416,10 @@ __global__ SG_BFS_BOUNDS(BS) void k_bfs(BfsArgs a)
         ok = grid_sync(a.bar, xcc, gen, a.abort);
     }
 
+    // levels in the caller's vertex order: every block left the loop after
+    // the same barrier, which made the last round's levels visible
+    if (ok && a.out)
+        for (uint32_t v = gtid; v < n; v += nthreads) a.out[v] = a.level[a.inv[v]];
     if (gtid == 0) write_final(a.trace, a.stats, L, pushes, pulls, ptrace, ok, t_start);
 }
 
@@ -470,6 +479,11 @@ extern "C" int sg_bfs(sg_engine *e, uint32_t source)
     a.beta = o.beta;
     a.total_workload0 = (uint32_t)((double)d.m * o.edge_factor);
     a.max_rounds = (uint32_t)o.max_iteration;
+    if (e->g->relabeled) {
+        // the kernel gathers the result into the value array itself
+        a.out = (int32_t *)e->value;
+        a.inv = e->g->d_inv;
+    }
     e->last_algo = 0;
     const int bs = o.block_size ? o.block_size : 1024;
     switch (bs) {
@@ -479,7 +493,6 @@ extern "C" int sg_bfs(sg_engine *e, uint32_t source)
     default: SG_FAIL(SG_ERR_UNSUPPORTED, "sg_bfs: block_size must be 256, 512 or 1024");
     }
     if (rc) return rc;
-    if ((rc = engine_finish(e, 1))) return rc;
     e->timed = true;
     return SG_OK;
 }

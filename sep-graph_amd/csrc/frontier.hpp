@@ -107,6 +107,30 @@ __device__ __forceinline__ RoundCounts unpack_counts(const unsigned long long *s
     return r;
 }
 
+// Set bit v of the bitmap `bits` for every lane with p, with one atomicOr per
+// run of adjacent lanes whose v fall in the same 64-bit word: neighbour lists
+// are sorted, so a wave's 64 edges often hit one word, and 64 same-address
+// atomics from a wave (and from every wave expanding the same hubs)
+// serialise. A segmented OR-scan gathers each run's bits into its last lane;
+// lanes of one word that are not adjacent form runs of their own, so every
+// bit is still set. Wave-convergent.
+__device__ __forceinline__ void wave_set_bits(uint64_t *bits, bool p, uint32_t v)
+{
+    if (!ballot(p)) return;
+    const int ln = lane();
+    const uint32_t w = p ? (v >> 6) : 0xFFFFFFFFu;
+    uint64_t m = p ? (1ull << (v & 63)) : 0ull;
+#pragma unroll
+    for (int d = 1; d < kWave; d <<= 1) {
+        const uint64_t y = __shfl_up(m, d, kWave);
+        const uint32_t wy = __shfl_up(w, d, kWave);
+        if (ln >= d && wy == w) m |= y;
+    }
+    const uint32_t wn = __shfl_down(w, 1, kWave);
+    const bool last = ln == kWave - 1 || wn != w;
+    if (p && last) atomicOr((unsigned long long *)&bits[w], (unsigned long long)m);
+}
+
 // Per-block staging of the next queue in LDS.
 struct Stager {
     uint32_t *qout;        // global queue of the next round

@@ -150,9 +150,11 @@ struct sg_server {
             if (kind == SG_QUERY_SSSP) {
                 // one whole-grid query after another, each into its row
                 ok = true;
+                // (a relabelled graph's kernel gathers into the pool row)
                 for (uint32_t i = 0; ok && i < b; i++) {
-                    uint32_t *out = via_work ? work + i * n : (uint32_t *)pool + (size_t)h_rows[i] * n;
-                    ok = sssp_single_launch(e, h_src[i], out) == SG_OK && sssp_single_status(e) == SG_OK;
+                    uint32_t *row = (uint32_t *)pool + (size_t)h_rows[i] * n;
+                    ok = sssp_single_launch(e, h_src[i], via_work ? work : row, row) == SG_OK &&
+                         sssp_single_status(e) == SG_OK;
                 }
             } else {
                 ok = hipMemcpyAsync(d_src, h_src, b * 4, hipMemcpyHostToDevice, e->stream) == hipSuccess &&
@@ -161,7 +163,7 @@ struct sg_server {
                 ok = ok && ms_launch(e, kind_to_ms(kind), d_src, b, via_work ? (void *)work : pool,
                                      via_work ? nullptr : d_rows, true, true) == SG_OK;
             }
-            for (uint32_t j = 0; ok && via_work && j < b; j++)
+            for (uint32_t j = 0; ok && via_work && kind != SG_QUERY_SSSP && j < b; j++)
                 ok = engine_gather_rows(e, work + j * n, (uint32_t *)pool + (size_t)h_rows[j] * n, 1) == SG_OK;
             ok = ok && sg_engine_sync(e) == SG_OK;
             const double t1 = now_s();

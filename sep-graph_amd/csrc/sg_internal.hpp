@@ -224,9 +224,11 @@ int ms_launch(sg_engine *e, int kind, const uint32_t *dev_src, uint32_t nsrc, vo
 int sssp_group_launch(sg_engine *e, const uint32_t *sources, int nq, uint32_t *const *out);
 // Worst status of the last such launch (synchronises the stream).
 int sssp_group_status(sg_engine *e);
-// One SSSP query over the whole grid (device source id) into the device row
-// `out`, asynchronously; sssp_single_status synchronises and reports it.
-int sssp_single_launch(sg_engine *e, uint32_t source, uint32_t *out);
+// One SSSP query over the whole grid (device source id) computing in the
+// device row `dist` (device order); a relabelled graph's result is gathered
+// into the row `out` (caller's order) by the kernel. Asynchronous;
+// sssp_single_status synchronises and reports it.
+int sssp_single_launch(sg_engine *e, uint32_t source, uint32_t *dist, uint32_t *out);
 int sssp_single_status(sg_engine *e);
 
 // Device allocation owned by the engine (freed with it); reports SG_ERR_NOMEM.

@@ -68,6 +68,10 @@ struct SsspArgs {
     uint32_t probe;   // launch is a residency probe of the grid
     uint32_t *exits;  // per-block exit records (diagnostics)
     uint32_t delta;   // two-level priority bucket width (0: off)
+    // degree-ordered graphs: dist is in device order, and the kernel ends by
+    // writing out[v] = dist[inv[v]] (the caller's order); null otherwise
+    uint32_t *out;
+    const uint32_t *inv;
     uint32_t grouped;          // one query per XCD (gr), per-query state arrays
     SsspGroups gr;
 };
@@ -109,7 +113,6 @@ __device__ __forceinline__ void relax(Round &R, bool act, uint32_t v, uint32_t n
             R.minv = min(R.minv, nd);
             if (a.stamp[v] != R.L + 2) {
                 a.stamp[v] = R.L + 2;
-                atomicOr((unsigned long long *)&R.fnext[v >> 6], 1ull << (v & 63));
                 const uint32_t dv = a.g.row[v + 1] - a.g.row[v];
                 R.scout += dv;
                 R.count++;
@@ -118,6 +121,7 @@ __device__ __forceinline__ void relax(Round &R, bool act, uint32_t v, uint32_t n
             }
         }
     }
+    wave_set_bits(R.fnext, queued, v);
     stage(R.S, queued, v);
 }
 
@@ -131,7 +135,6 @@ __device__ __forceinline__ void defer(Round &R, bool d, uint32_t u, uint32_t du)
         R.minv = min(R.minv, du);
         if (a.stamp[u] != R.L + 2) {
             a.stamp[u] = R.L + 2;
-            atomicOr((unsigned long long *)&R.fnext[u >> 6], 1ull << (u & 63));
             const uint32_t dv = a.g.row[u + 1] - a.g.row[u];
             R.scout += dv;
             R.count++;
@@ -139,6 +142,7 @@ __device__ __forceinline__ void defer(Round &R, bool d, uint32_t u, uint32_t du)
             queued = true;
         }
     }
+    wave_set_bits(R.fnext, queued, u);
     stage(R.S, queued, u);
 }
 
@@ -522,6 +526,10 @@ __device__ __forceinline__ void sssp_run(const SsspArgs &a, uint32_t brank, uint
         ok = grid_sync(a.bar, xcc, gen, a.abort);
     }
 
+    // distances in the caller's vertex order: every block left the loop after
+    // the same barrier, which made the last round's distances visible
+    if (ok && a.out)
+        for (uint32_t v = gtid; v < n; v += nthreads) a.out[v] = a.dist[a.inv[v]];
     if (gtid == 0) write_final(a.trace, a.stats, L, pushes, pulls, ptrace, ok, t_start);
     if (a.exits && threadIdx.x == 0 && brank < kMaxBlocks) {
         uint4 rec = make_uint4(L, reason, (uint32_t)min(last_count, 0xFFFFFFFFull), xcc);
@@ -712,18 +720,25 @@ int sssp_group_launch(sg_engine *e, const uint32_t *sources, int nq, uint32_t *c
     return launch_sssp_bs(e, a, (uint32_t *)(base + o_bar));
 }
 
-// One SSSP query over the whole grid from device vertex `source`, writing its
-// distances to the device row `out`. Asynchronous; its status lands in the
-// engine's stats, overwritten by the next run (see sssp_single_status).
-int sssp_single_launch(sg_engine *e, uint32_t source, uint32_t *out)
+// One SSSP query over the whole grid from device vertex `source`, with its
+// distances in the device row `dist` (device vertex order) and, for a
+// relabelled graph, gathered into the row `out` in the caller's order.
+// Asynchronous; its status lands in the engine's stats, overwritten by the
+// next run (see sssp_single_status).
+int sssp_single_launch(sg_engine *e, uint32_t source, uint32_t *dist, uint32_t *out)
 {
-    if (source >= e->g->dev.n || !out) SG_FAIL(SG_ERR_RANGE, "sssp_single_launch: bad source or row");
+    if (source >= e->g->dev.n || !dist || (e->g->relabeled && !out))
+        SG_FAIL(SG_ERR_RANGE, "sssp_single_launch: bad source or row");
     int rc = engine_ensure_bfs_state(e);
     if (rc) return rc;
     if ((rc = engine_ensure_stamp(e))) return rc;
     SsspArgs a;
     if ((rc = sssp_args(e, a))) return rc;
-    a.dist = out;
+    a.dist = dist;
+    if (e->g->relabeled) {
+        a.out = out;
+        a.inv = e->g->d_inv;
+    }
     a.stamp = e->stamp;
     a.fbit = e->bitmaps;
     a.queue = e->queues;
@@ -789,11 +804,15 @@ extern "C" int sg_sssp(sg_engine *e, uint32_t source)
     a.trace = e->trace;
     a.exits = e->exits;
     a.source = dev_vertex(e->g, source);
+    if (e->g->relabeled) {
+        // the kernel gathers the result into the value array itself
+        a.out = (uint32_t *)e->value;
+        a.inv = e->g->d_inv;
+    }
     e->last_algo = 1;
     SG_HIP(hipEventRecord(e->ev0, e->stream));
     if ((rc = launch_sssp_bs(e, a, nullptr))) return rc;
     SG_HIP(hipEventRecord(e->ev1, e->stream));
-    if ((rc = engine_finish(e, 1))) return rc;
     e->timed = true;
     return SG_OK;
 }
