@@ -11,21 +11,22 @@
 //
 // Here the whole traversal is one launch. Every block evaluates the same
 // alpha/beta rule on device from counters read after a grid barrier, so no
-// round needs the host. The level array is the only per-vertex state a push
-// round writes:
-//   * discovery is a plain check-and-store, level[v] == -1 -> L+1. Racing
-//     writers store the same value (the reference's non-atomic BFS min,
-//     hybrid_bfs.cu:128-134, on by default, hybrid_bfs/main.cpp:37), so a
-//     vertex may enter the next queue more than once, exactly as in the
-//     reference; a discovery also sets the vertex's bit in the next
-//     frontier's bitmap (one atomicOr per discovery, none per edge), so every
-//     frontier is available as 512 KiB of L2-resident bits;
+// round needs the host. Per vertex, a push round writes the level and two
+// bits:
+//   * discovery is a plain check-and-store: an unset bit in the visited
+//     filter -> level L+1. Racing writers store the same value (the
+//     reference's non-atomic BFS min, hybrid_bfs.cu:128-134, on by default,
+//     hybrid_bfs/main.cpp:37), so a vertex may enter the next queue more than
+//     once, exactly as in the reference; a discovery sets the vertex's bits in
+//     the visited filter and in the next frontier's bitmap (one atomicOr per
+//     run of a wave's discoveries in one word, none per edge), so visited and
+//     frontier tests read 512 KiB of L2-resident bits, never the level array;
 //   * the next queue is staged in LDS per block and published with one
 //     atomic per block per round (same-address atomics serialise on MI355X);
 //   * vertices with out-degree > kHugeDegree are expanded through the graph's
 //     static chunk list, filtered by membership, so hub adjacencies spread
 //     over every wave without building any list at run time.
-// A pull round sweeps 64-vertex words (each wave owning a contiguous range,
+// A pull round sweeps 64-vertex words (dealt round-robin over the waves,
 // kPullGroup words at a time): every unvisited lane tests its first
 // in-neighbour from the graph's head array, then the group's words walk their
 // remaining in-edge lists in lock step, 64 edges per word per step with all
@@ -54,7 +55,7 @@ constexpr int kPullGroup = SG_PULL_GROUP;
 struct BfsArgs {
     DevGraph g;
     int32_t *level;
-    uint64_t *vis;    // W words: visited filter kept by pull rounds (set bit => visited)
+    uint64_t *vis;    // W words: visited filter (set bit => visited or in-degree 0)
     uint64_t *fbit;   // 3 * W words: frontier L's bitmap in slot L % 3
     uint32_t *queue;  // 3 * qcap entries
     uint32_t qcap;
@@ -92,7 +93,7 @@ __device__ __forceinline__ void push_visit(Round &R, bool act, uint32_t v)
 {
     const BfsArgs &a = *R.a;
     bool found = false;
-    if (act && a.level[v] == -1) {
+    if (act && !test_bit(a.vis, v)) {
         a.level[v] = R.L + 1;
         const uint32_t dv = a.g.row[v + 1] - a.g.row[v];
         R.scout += dv;
@@ -100,10 +101,11 @@ __device__ __forceinline__ void push_visit(Round &R, bool act, uint32_t v)
         R.huge |= dv > kHugeDegree;
         found = true;
     }
-    // the next frontier's bitmap, so that a pull round after this one tests
-    // membership in 512 KiB of L2-resident bits instead of the level array
-    // (at most one atomic per discovery and word run, none per edge)
-    wave_set_bits(R.fnext, found, v);
+    // the visited filter and the next frontier's bitmap, so that visited
+    // tests here and membership tests in a pull round read 512 KiB of
+    // L2-resident bits instead of the level array (at most one atomic per
+    // discovery and word run, none per edge)
+    wave_set_bits(a.vis, found, v, R.fnext);
     stage(R.S, found, v);
 }
 
@@ -143,14 +145,10 @@ __device__ __forceinline__ void pull_group(Round &R, const uint32_t (&wg)[G], co
     const DevGraph &g = a.g;
     const int ln = lane();
     uint32_t v[G], b[G], e[G], u[G];
-    int32_t lv[G];
     uint64_t visited[G];
     bool found[G];
 #pragma unroll
-    for (int i = 0; i < G; i++) {
-        v[i] = wg[i] * 64 + ln;
-        lv[i] = (i < ng && v[i] < g.n) ? a.level[v[i]] : 0;
-    }
+    for (int i = 0; i < G; i++) v[i] = wg[i] * 64 + ln;
     bool want[G];
     uint32_t dv[G];
     // One step issues, for every unvisited lane, the first in-neighbour (from
@@ -159,7 +157,9 @@ __device__ __forceinline__ void pull_group(Round &R, const uint32_t (&wg)[G], co
     // wait on a single round of loads.
 #pragma unroll
     for (int i = 0; i < G; i++) {
-        visited[i] = i < ng ? (ballot(v[i] >= g.n || lv[i] != -1) | vwg[i]) : ~0ull;
+        // the filter holds every discovery (pull rounds write whole words,
+        // push rounds set bits), the inert vertices and the source
+        visited[i] = i < ng ? (ballot(v[i] >= g.n) | vwg[i]) : ~0ull;
         want[i] = !((visited[i] >> ln) & 1);
         u[i] = want[i] ? g.chead[v[i]] : 0xFFFFFFFFu;
         b[i] = want[i] ? g.cstart[v[i]] : 0u;
@@ -259,6 +259,8 @@ __global__ SG_BFS_BOUNDS(BS) void k_bfs(BfsArgs a)
 
     // ---- prologue: reset per-query state, seed the source (one barrier)
     const uint32_t src = a.source;
+    // (a full reset, also where the epilogue gathers: it leaves the array in
+    // the last-level caches for that gather's random reads)
     for (uint32_t i = gtid; i < n; i += nthreads) a.level[i] = (i == src) ? 0 : -1;
     for (uint32_t w = gtid; w < W; w += nthreads) {
         const uint64_t sb = (w == (src >> 6)) ? (1ull << (src & 63)) : 0ull;
@@ -418,8 +420,7 @@ __global__ SG_BFS_BOUNDS(BS) void k_bfs(BfsArgs a)
 
     // levels in the caller's vertex order: every block left the loop after
     // the same barrier, which made the last round's levels visible
-    if (ok && a.out)
-        for (uint32_t v = gtid; v < n; v += nthreads) a.out[v] = a.level[a.inv[v]];
+    if (ok && a.out) gather_caller_order(a.out, a.level, a.inv, n, gtid, nthreads);
     if (gtid == 0) write_final(a.trace, a.stats, L, pushes, pulls, ptrace, ok, t_start);
 }
 

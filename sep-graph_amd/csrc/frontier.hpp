@@ -78,20 +78,27 @@ struct RoundCounts {
 __device__ __forceinline__ void read_counts(const FrontierCtr *c, unsigned long long *s_lvl)
 {
     if (threadIdx.x < kWave) {
+        // every load in flight at once (one memory round trip): stripes in
+        // lanes 0..15, the slot's header words in lanes 16..18
         const int ln = lane();
         unsigned long long cnt = 0, sc = 0;
+        uint32_t hdr = 0;
         if (ln < kStripes) {
             cnt = ld_relaxed64(&c->stripe[ln].count);
             sc = ld_relaxed64(&c->stripe[ln].scout);
+        } else if (ln < kStripes + 3) {
+            const uint32_t *h = ln == kStripes ? &c->qlen : ln == kStripes + 1 ? &c->overflow : &c->has_huge;
+            hdr = __hip_atomic_load(h, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         }
         cnt = wave_sum_u64(cnt);
         sc = wave_sum_u64(sc);
+        const uint32_t qlen = shfl(hdr, kStripes), ovf = shfl(hdr, kStripes + 1), hg = shfl(hdr, kStripes + 2);
         if (ln == 0) {
             s_lvl[0] = cnt;
             s_lvl[1] = sc;
-            s_lvl[2] = __hip_atomic_load(&c->qlen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            s_lvl[3] = __hip_atomic_load(&c->overflow, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            s_lvl[4] = __hip_atomic_load(&c->has_huge, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            s_lvl[2] = qlen;
+            s_lvl[3] = ovf;
+            s_lvl[4] = hg;
         }
     }
 }
@@ -113,8 +120,8 @@ __device__ __forceinline__ RoundCounts unpack_counts(const unsigned long long *s
 // atomics from a wave (and from every wave expanding the same hubs)
 // serialise. A segmented OR-scan gathers each run's bits into its last lane;
 // lanes of one word that are not adjacent form runs of their own, so every
-// bit is still set. Wave-convergent.
-__device__ __forceinline__ void wave_set_bits(uint64_t *bits, bool p, uint32_t v)
+// bit is still set. bits2, when given, receives the same bits. Wave-convergent.
+__device__ __forceinline__ void wave_set_bits(uint64_t *bits, bool p, uint32_t v, uint64_t *bits2 = nullptr)
 {
     if (!ballot(p)) return;
     const int ln = lane();
@@ -128,7 +135,37 @@ __device__ __forceinline__ void wave_set_bits(uint64_t *bits, bool p, uint32_t v
     }
     const uint32_t wn = __shfl_down(w, 1, kWave);
     const bool last = ln == kWave - 1 || wn != w;
-    if (p && last) atomicOr((unsigned long long *)&bits[w], (unsigned long long)m);
+    if (p && last) {
+        atomicOr((unsigned long long *)&bits[w], (unsigned long long)m);
+        if (bits2) atomicOr((unsigned long long *)&bits2[w], (unsigned long long)m);
+    }
+}
+
+// out[v] = src[inv[v]] for v = t, t + nt, ... < n: a kernel epilogue putting a
+// result kept in device vertex order back into the caller's order. Four
+// elements per thread and step, their index loads and then their gathers
+// issued together, since each gather is a random read.
+template <typename T>
+__device__ __forceinline__ void gather_caller_order(T *out, const T *src, const uint32_t *inv, uint32_t n, uint32_t t,
+                                                    uint32_t nt)
+{
+    constexpr int K = 4;
+    for (uint64_t v0 = t; v0 < n; v0 += (uint64_t)K * nt) {
+        uint32_t idx[K];
+        T x[K];
+#pragma unroll
+        for (int k = 0; k < K; k++) {
+            const uint64_t v = v0 + (uint64_t)k * nt;
+            idx[k] = v < n ? inv[v] : 0u;
+        }
+#pragma unroll
+        for (int k = 0; k < K; k++) x[k] = v0 + (uint64_t)k * nt < n ? src[idx[k]] : T();
+#pragma unroll
+        for (int k = 0; k < K; k++) {
+            const uint64_t v = v0 + (uint64_t)k * nt;
+            if (v < n) out[v] = x[k];
+        }
+    }
 }
 
 // Per-block staging of the next queue in LDS.

@@ -25,7 +25,9 @@ __global__ void k_inert_bitmap(const uint32_t *cstart, uint32_t n, uint32_t word
 }
 
 // Row r of dst (caller's vertex order) from row r of src (device order):
-// dst[v] = src[inv[v]]. Reads of inv and writes are sequential.
+// dst[v] = src[inv[v]]. Reads of inv and writes are sequential (on MI355X
+// this gather beats the scatter dst[perm[i]] = src[i], whose partial-line
+// random writes cost more than random reads).
 __global__ void k_gather_rows(const uint32_t *src, uint32_t *dst, const uint32_t *inv, uint32_t n)
 {
     const size_t off = (size_t)blockIdx.y * n;
@@ -33,17 +35,21 @@ __global__ void k_gather_rows(const uint32_t *src, uint32_t *dst, const uint32_t
         dst[off + v] = src[off + inv[v]];
 }
 
-// Exact reach statistics of a traversal result: vertices whose 32-bit value
-// is not the 0xFFFFFFFF sentinel (-1 level / UINT32_MAX distance) and the sum
-// of their out-degrees. out[0], out[1] must be zero on entry.
-__global__ void k_reach_stats(const uint32_t *value, const uint32_t *row, uint32_t n, unsigned long long *out)
+// Exact reach statistics of a traversal result in the caller's vertex order:
+// vertices whose 32-bit value is not the 0xFFFFFFFF sentinel (-1 level /
+// UINT32_MAX distance) and the sum of their out-degrees, read from the device
+// CSR through inv (null: the orders agree). out[0], out[1] must be zero on
+// entry.
+__global__ void k_reach_stats(const uint32_t *value, const uint32_t *row, const uint32_t *inv, uint32_t n,
+                              unsigned long long *out)
 {
     __shared__ unsigned long long s[2][16];
     unsigned long long cnt = 0, deg = 0;
     for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < n; v += gridDim.x * blockDim.x) {
         if (value[v] != 0xFFFFFFFFu) {
+            const uint32_t d = inv ? inv[v] : v;
             cnt++;
-            deg += row[v + 1] - row[v];
+            deg += row[d + 1] - row[d];
         }
     }
     cnt = wave_sum_u64(cnt);
@@ -724,10 +730,9 @@ extern "C" int sg_engine_get_stats(sg_engine *e, sg_stats *out)
         const DevGraph &d = e->g->dev;
         // few blocks: each adds into the same two words once
         const uint32_t blocks = std::min<uint32_t>((d.n + 255) / 256, (uint32_t)e->g->num_cus);
-        // the result in device order pairs with the device CSR
-        const void *res = e->g->relabeled ? e->work : e->value;
-        hipLaunchKernelGGL(k_reach_stats, dim3(blocks), dim3(256), 0, e->stream, (const uint32_t *)res, d.row, d.n,
-                           acc);
+        // the result is in the caller's order; degrees come through inv
+        hipLaunchKernelGGL(k_reach_stats, dim3(blocks), dim3(256), 0, e->stream, (const uint32_t *)e->value, d.row,
+                           e->g->relabeled ? e->g->d_inv : nullptr, d.n, acc);
         SG_HIP(hipGetLastError());
         unsigned long long h[2];
         SG_HIP(hipMemcpyAsync(h, acc, 16, hipMemcpyDeviceToHost, e->stream));

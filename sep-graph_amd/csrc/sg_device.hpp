@@ -78,11 +78,12 @@ __device__ __forceinline__ void wave_append(bool pred, uint32_t v, uint32_t *q, 
 // XCC_ID; each XCD has its own L2). Data produced before the barrier reaches
 // the writer's L2 when the block's waves pass __syncthreads (vector L1 is
 // write-through); the last block to arrive on an XCD writes that L2 back once
-// (agent-scope release) and signals the top counter. The last XCD flips the
-// generation word. On the way out, each XCD's leader invalidates its L2 once
-// (agent-scope acquire) and releases the XCD's other blocks, which only
-// invalidate their CU's L1. This costs one L2 write-back and one invalidate per
-// XCD per barrier instead of one per block.
+// (agent-scope release) and signals the top counter, so the barrier costs one
+// L2 write-back per XCD instead of one per block. The last XCD flips the
+// generation word, which every block polls (system scope, so no cache can
+// answer) before its agent-scope acquire. Relaying the flip through a per-XCD
+// flag set by the XCD's leader measured slower on MI355X (one more store/poll
+// hop on the critical path).
 //
 // The number of blocks each XCD receives for a given grid is measured once per
 // engine by a census kernel (dispatch is deterministic); a launch whose blocks
@@ -215,15 +216,12 @@ __device__ __forceinline__ bool grid_sync(const GridBarrier &gb, uint32_t xcc, u
                 __builtin_amdgcn_s_waitcnt(0);
                 flag_store(w + kBarGen, gen0 + 1);
             }
-            ok = spin_wait(w + kBarGen, gen0, true, gb.spin_limit, abort);
-            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-            __builtin_amdgcn_s_waitcnt(0);
-            flag_store(w + bar_release(xcc), gen0 + 1);
-        } else {
-            ok = spin_wait(w + bar_release(xcc), gen0 + 1, false, gb.spin_limit, abort);
-            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-            __builtin_amdgcn_s_waitcnt(0);
         }
+        // every block polls the generation word itself (a relay through a
+        // per-XCD flag costs one more store/poll hop on the critical path)
+        ok = spin_wait(w + kBarGen, gen0, true, gb.spin_limit, abort);
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+        __builtin_amdgcn_s_waitcnt(0);
         s_ok = ok;
     }
     __syncthreads();

@@ -66,10 +66,10 @@ struct DevGraph {
 // vertices together, one 64-edge step (a few dependent memory round trips)
 // at a time, so the bound keeps any wave's share of a push round short.
 #ifndef SG_HUGE_DEGREE
-#define SG_HUGE_DEGREE 1024
+#define SG_HUGE_DEGREE 256
 #endif
 #ifndef SG_HUGE_CHUNK
-#define SG_HUGE_CHUNK 256
+#define SG_HUGE_CHUNK 128
 #endif
 constexpr uint32_t kHugeDegree = SG_HUGE_DEGREE;
 constexpr uint32_t kHugeChunk = SG_HUGE_CHUNK;

@@ -528,8 +528,7 @@ __device__ __forceinline__ void sssp_run(const SsspArgs &a, uint32_t brank, uint
 
     // distances in the caller's vertex order: every block left the loop after
     // the same barrier, which made the last round's distances visible
-    if (ok && a.out)
-        for (uint32_t v = gtid; v < n; v += nthreads) a.out[v] = a.dist[a.inv[v]];
+    if (ok && a.out) gather_caller_order(a.out, a.dist, a.inv, n, gtid, nthreads);
     if (gtid == 0) write_final(a.trace, a.stats, L, pushes, pulls, ptrace, ok, t_start);
     if (a.exits && threadIdx.x == 0 && brank < kMaxBlocks) {
         uint4 rec = make_uint4(L, reason, (uint32_t)min(last_count, 0xFFFFFFFFull), xcc);
