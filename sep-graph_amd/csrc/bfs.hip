@@ -54,7 +54,12 @@ constexpr int kPullGroup = SG_PULL_GROUP;
 
 struct BfsArgs {
     DevGraph g;
+    // levels in the caller's vertex order: device vertex v's level goes to
+    // level[perm[v]] (perm null: the orders agree). Nothing reads it during
+    // the traversal (visited tests use the filter), so a degree-ordered graph
+    // needs no device-order copy and no gather at the end.
     int32_t *level;
+    const uint32_t *perm;
     uint64_t *vis;    // W words: visited filter (set bit => visited or in-degree 0)
     uint64_t *fbit;   // 3 * W words: frontier L's bitmap in slot L % 3
     uint32_t *queue;  // 3 * qcap entries
@@ -64,17 +69,16 @@ struct BfsArgs {
     uint32_t *abort;
     DevStats *stats;
     DevTrace *trace;
-    uint32_t source;
+    uint32_t source;     // device vertex id
+    uint32_t source_id;  // the caller's id of the same vertex
     int32_t force;    // -1 auto, 0 push, 1 pull
     double alpha, beta;
     uint32_t total_workload0;
     uint32_t max_rounds;
     uint32_t probe;   // launch is a residency probe of the grid
-    // degree-ordered graphs: level is in device order, and the kernel ends by
-    // writing out[v] = level[inv[v]] (the caller's order); null otherwise
-    int32_t *out;
-    const uint32_t *inv;
 };
+
+__device__ __forceinline__ uint32_t caller_id(const BfsArgs &a, uint32_t v) { return a.perm ? a.perm[v] : v; }
 
 struct Round {
     const BfsArgs *a;
@@ -94,7 +98,7 @@ __device__ __forceinline__ void push_visit(Round &R, bool act, uint32_t v)
     const BfsArgs &a = *R.a;
     bool found = false;
     if (act && !test_bit(a.vis, v)) {
-        a.level[v] = R.L + 1;
+        a.level[caller_id(a, v)] = R.L + 1;
         const uint32_t dv = a.g.row[v + 1] - a.g.row[v];
         R.scout += dv;
         R.count++;
@@ -219,7 +223,7 @@ __device__ __forceinline__ void pull_group(Round &R, const uint32_t (&wg)[G], co
             R.count += __popcll(fm);
         }
         if (found[i]) {
-            a.level[v[i]] = R.L + 1;
+            a.level[caller_id(a, v[i])] = R.L + 1;
             R.scout += dv[i];
             R.huge |= dv[i] > kHugeDegree;
         }
@@ -259,9 +263,7 @@ __global__ SG_BFS_BOUNDS(BS) void k_bfs(BfsArgs a)
 
     // ---- prologue: reset per-query state, seed the source (one barrier)
     const uint32_t src = a.source;
-    // (a full reset, also where the epilogue gathers: it leaves the array in
-    // the last-level caches for that gather's random reads)
-    for (uint32_t i = gtid; i < n; i += nthreads) a.level[i] = (i == src) ? 0 : -1;
+    for (uint32_t i = gtid; i < n; i += nthreads) a.level[i] = (i == a.source_id) ? 0 : -1;
     for (uint32_t w = gtid; w < W; w += nthreads) {
         const uint64_t sb = (w == (src >> 6)) ? (1ull << (src & 63)) : 0ull;
         a.vis[w] = (g.inert ? g.inert[w] : 0ull) | sb;
@@ -420,7 +422,6 @@ __global__ SG_BFS_BOUNDS(BS) void k_bfs(BfsArgs a)
 
     // levels in the caller's vertex order: every block left the loop after
     // the same barrier, which made the last round's levels visible
-    if (ok && a.out) gather_caller_order(a.out, a.level, a.inv, n, gtid, nthreads);
     if (gtid == 0) write_final(a.trace, a.stats, L, pushes, pulls, ptrace, ok, t_start);
 }
 
@@ -460,11 +461,11 @@ extern "C" int sg_bfs(sg_engine *e, uint32_t source)
     if (force < -1) SG_FAIL(SG_ERR_UNSUPPORTED, "sg_bfs: only data-driven push/pull variants apply to BFS");
     int rc = engine_ensure_bfs_state(e);
     if (rc) return rc;
-    void *res = nullptr;
-    if ((rc = engine_result_rows(e, 1, &res))) return rc;
+    if ((rc = engine_ensure_value(e, (uint64_t)d.n * 4))) return rc;
     BfsArgs a{};
     a.g = d;
-    a.level = (int32_t *)res;
+    a.level = (int32_t *)e->value;
+    a.perm = e->g->relabeled ? e->g->d_perm : nullptr;
     a.vis = e->bitmaps;
     a.fbit = e->bitmaps + d.words;
     a.queue = e->queues;
@@ -475,16 +476,12 @@ extern "C" int sg_bfs(sg_engine *e, uint32_t source)
     a.stats = e->stats;
     a.trace = e->trace;
     a.source = dev_vertex(e->g, source);
+    a.source_id = source;
     a.force = force;
     a.alpha = o.alpha == 0.0 ? 1.0 : o.alpha;
     a.beta = o.beta;
     a.total_workload0 = (uint32_t)((double)d.m * o.edge_factor);
     a.max_rounds = (uint32_t)o.max_iteration;
-    if (e->g->relabeled) {
-        // the kernel gathers the result into the value array itself
-        a.out = (int32_t *)e->value;
-        a.inv = e->g->d_inv;
-    }
     e->last_algo = 0;
     const int bs = o.block_size ? o.block_size : 1024;
     switch (bs) {

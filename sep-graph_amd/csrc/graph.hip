@@ -272,6 +272,8 @@ extern "C" int sg_graph_create_ex(uint32_t n, uint64_t m, const uint32_t *row_st
         const uint32_t *p = nullptr;
         rc = up(g->h_inv.data(), n, &p);
         g->d_inv = p;
+        if (!rc) rc = up(perm.data(), n, &p);
+        g->d_perm = p;
     }
     if (!rc) {
         void *p = nullptr;

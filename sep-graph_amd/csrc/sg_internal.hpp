@@ -95,6 +95,9 @@ struct sg_graph {
     bool relabeled = false;
     std::vector<uint32_t> h_inv;
     const uint32_t *d_inv = nullptr;
+    // device vertex i is the caller's vertex d_perm[i] (for kernels that
+    // write results in the caller's order as they go)
+    const uint32_t *d_perm = nullptr;
 };
 
 namespace sg {
