@@ -13,13 +13,14 @@
 // alpha/beta rule on device from counters read after a grid barrier, so no
 // round needs the host. Per vertex, a push round writes the level and two
 // bits:
-//   * discovery is a plain check-and-store: an unset bit in the visited
-//     filter -> level L+1. Racing writers store the same value (the
-//     reference's non-atomic BFS min, hybrid_bfs.cu:128-134, on by default,
-//     hybrid_bfs/main.cpp:37), so a vertex may enter the next queue more than
-//     once, exactly as in the reference; a discovery sets the vertex's bits in
-//     the visited filter and in the next frontier's bitmap (one atomicOr per
-//     run of a wave's discoveries in one word, none per edge), so visited and
+//   * discovery: an edge whose head shows an unset bit in the visited filter
+//     claims that bit (one returning atomicOr per run of a wave's candidates
+//     in one word, none per edge); the claimer stores level L+1 and queues the
+//     vertex. The reference's default non-atomic BFS min
+//     (hybrid_bfs.cu:128-134, hybrid_bfs/main.cpp:37) lets racing writers
+//     queue a vertex several times; claiming keeps the queue exact, which
+//     matters on high-diameter graphs where duplicates compound round after
+//     round. Candidates also set the next frontier's bitmap, so visited and
 //     frontier tests read 512 KiB of L2-resident bits, never the level array;
 //   * the next queue is staged in LDS per block and published with one
 //     atomic per block per round (same-address atomics serialise on MI355X);
@@ -96,20 +97,22 @@ __device__ __forceinline__ bool in_frontier(const Round &R, uint32_t u) { return
 __device__ __forceinline__ void push_visit(Round &R, bool act, uint32_t v)
 {
     const BfsArgs &a = *R.a;
-    bool found = false;
-    if (act && !test_bit(a.vis, v)) {
+    // candidates: not visited as far as this lane can see (bits set by other
+    // waves in this round may not be visible yet). Claiming the bit in the
+    // visited filter picks one discoverer; every candidate was unvisited
+    // before this round, so all of them belong in the next frontier's bitmap.
+    // Visited tests here and membership tests in a pull round read 512 KiB
+    // of L2-resident bits instead of the level array, with one atomic per
+    // run of a wave's candidates in one word, none per edge.
+    const bool cand = act && !test_bit(a.vis, v);
+    const bool found = wave_claim_bits(a.vis, cand, v, R.fnext);
+    if (found) {
         a.level[caller_id(a, v)] = R.L + 1;
         const uint32_t dv = a.g.row[v + 1] - a.g.row[v];
         R.scout += dv;
         R.count++;
         R.huge |= dv > kHugeDegree;
-        found = true;
     }
-    // the visited filter and the next frontier's bitmap, so that visited
-    // tests here and membership tests in a pull round read 512 KiB of
-    // L2-resident bits instead of the level array (at most one atomic per
-    // discovery and word run, none per edge)
-    wave_set_bits(a.vis, found, v, R.fnext);
     stage(R.S, found, v);
 }
 

@@ -141,6 +141,47 @@ __device__ __forceinline__ void wave_set_bits(uint64_t *bits, bool p, uint32_t v
     }
 }
 
+// wave_set_bits on `bits`, returning for each lane with p whether this call
+// set its bit (the bit was clear before): each run's atomicOr returns the old
+// word, broadcast to the run's lanes; of adjacent lanes with the same v only
+// the first claims it. Racing waves therefore claim a vertex once (an
+// unclaimed copy would enter the next queue again, and duplicates compound
+// over the rounds of a high-diameter traversal). bits2 gets every p lane's
+// bit with a non-returning atomicOr. Wave-convergent.
+__device__ __forceinline__ bool wave_claim_bits(uint64_t *bits, bool p, uint32_t v, uint64_t *bits2 = nullptr)
+{
+    if (!ballot(p)) return false;
+    const int ln = lane();
+    const uint32_t w = p ? (v >> 6) : 0xFFFFFFFFu;
+    const uint64_t mine = p ? (1ull << (v & 63)) : 0ull;
+    // runs are maximal groups of adjacent lanes with the same word; the scan
+    // must not reach into an earlier run of the same word, whose lanes claim
+    // through their own atomic (a bit ORed by both could be taken from them)
+    const uint32_t wprev = __shfl_up(w, 1, kWave);
+    const bool first = ln == 0 || wprev != w;
+    const uint32_t run = (uint32_t)__popcll(ballot(first) & (~0ull >> (kWave - 1 - ln)));
+    uint64_t m = mine;
+#pragma unroll
+    for (int d = 1; d < kWave; d <<= 1) {
+        const uint64_t y = __shfl_up(m, d, kWave);
+        const uint32_t ry = __shfl_up(run, d, kWave);
+        if (ln >= d && ry == run) m |= y;
+    }
+    const uint32_t wn = __shfl_down(w, 1, kWave);
+    const bool last = ln == kWave - 1 || wn != w;
+    uint64_t old = 0;
+    if (p && last) {
+        old = atomicOr((unsigned long long *)&bits[w], (unsigned long long)m);
+        if (bits2) atomicOr((unsigned long long *)&bits2[w], (unsigned long long)m);
+    }
+    // a lane's run ends at the first lane at or above it flagged last
+    const uint64_t lm = ballot(last) >> ln;
+    old = __shfl(old, ln + __builtin_ctzll(lm), kWave);
+    const uint32_t vp = __shfl_up(v, 1, kWave), wp = __shfl_up(w, 1, kWave);
+    const bool dup = ln > 0 && wp == w && vp == v;
+    return p && !(old & mine) && !dup;
+}
+
 // out[v] = src[inv[v]] for v = t, t + nt, ... < n: a kernel epilogue putting a
 // result kept in device vertex order back into the caller's order. Four
 // elements per thread and step, their index loads and then their gathers

@@ -390,7 +390,7 @@ extern "C" int sg_engine_destroy(sg_engine *e)
 {
     if (!e) return SG_OK;
     if (e->stream) (void)hipStreamSynchronize(e->stream);
-    void *ptrs[] = {e->value, e->bitmaps, e->queues, e->stamp, e->ctr,
+    void *ptrs[] = {e->value, e->bitmaps, e->queues, e->ctr,
                     e->abort_flag, e->stats, e->trace, e->exits, e->aux, e->srcbuf, e->garena, e->work};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
@@ -528,11 +528,6 @@ int engine_alloc(void **p, uint64_t bytes)
     return SG_OK;
 }
 
-int engine_ensure_stamp(sg_engine *e)
-{
-    if (e->stamp) return SG_OK;
-    return engine_alloc((void **)&e->stamp, (uint64_t)e->g->dev.n * 4);
-}
 
 uint32_t dev_vertex(const sg_graph *g, uint32_t v) { return g->relabeled ? g->h_inv[v] : v; }
 

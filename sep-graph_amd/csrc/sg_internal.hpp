@@ -147,7 +147,6 @@ struct sg_engine {
     uint64_t *bitmaps = nullptr;  // 4 * words: visited + 3 rotating frontier bitmaps
     uint32_t *queues = nullptr;   // 3 frontier queues of qcap entries
     uint32_t qcap = 0;
-    uint32_t *stamp = nullptr;    // per-vertex round stamp (SSSP frontier membership)
     void *ctr = nullptr;          // kCtrBytes of per-round counters
     uint32_t *abort_flag = nullptr;
     sg::DevStats *stats = nullptr;
@@ -188,7 +187,6 @@ struct sg_engine {
 
 namespace sg {
 int engine_ensure_bfs_state(sg_engine *e);
-int engine_ensure_stamp(sg_engine *e);
 // Make the value array hold `bytes` (grown when needed, synchronising the
 // stream first) and record that the next result has that size.
 int engine_ensure_value(sg_engine *e, uint64_t bytes);

@@ -14,9 +14,11 @@
 //   * a relaxation first compares against a plain load of dist[v] and only
 //     issues atomicMin when it can lower it (atomics run at ~27 G/s on
 //     MI355X, loads several times faster);
-//   * stamp[v] = r + 1 marks v as a member of round r's frontier; a vertex
-//     lowered several times in a round is queued once per racing winner at
-//     most, and membership tests (hub chunks, pull) read the stamp;
+//   * round r's frontier is a bitmap; a vertex lowered in round r is queued
+//     for round r+1 by the lane that claims its bit there (one returning
+//     atomicOr per run of a wave's lanes in one word), so it is queued once
+//     however many relaxations lowered it, and membership tests (hub chunks,
+//     pull) read 512 KiB of L2-resident bits;
 //   * a pull round owns each vertex with one lane, so the minimum over its
 //     frontier in-neighbours is a segmented wave reduction, not atomics.
 #include "frontier.hpp"
@@ -51,8 +53,7 @@ struct SsspGroups {
 struct SsspArgs {
     DevGraph g;
     uint32_t *dist;
-    uint32_t *stamp;  // frontier r  <=>  stamp == r + 1
-    uint64_t *fbit;   // 3 * W words: rotating frontier bitmaps written by pull rounds
+    uint64_t *fbit;   // 3 * W words: round r's frontier bitmap in slot r % 3
     uint32_t *queue;  // 3 * qcap entries
     uint32_t qcap;
     FrontierCtr *ctr;
@@ -79,7 +80,6 @@ struct SsspArgs {
 struct Round {
     const SsspArgs *a;
     uint32_t L;
-    bool fbit_valid;
     const uint64_t *fcur;
     uint64_t *fnext;
     Stager S;
@@ -95,10 +95,7 @@ __device__ __forceinline__ uint32_t &slot_cmin(FrontierCtr *c) { return c->pad[0
 
 __device__ __forceinline__ uint32_t sat_add(uint32_t d, uint32_t w) { return d > kInf - w ? kInf : d + w; }
 
-__device__ __forceinline__ bool in_frontier(const Round &R, uint32_t u)
-{
-    return R.fbit_valid ? test_bit(R.fcur, u) : R.a->stamp[u] == R.L + 1;
-}
+__device__ __forceinline__ bool in_frontier(const Round &R, uint32_t u) { return test_bit(R.fcur, u); }
 
 __device__ __forceinline__ uint32_t edge_w(const uint32_t *w, uint32_t e) { return w ? w[e] : 1u; }
 
@@ -106,22 +103,19 @@ __device__ __forceinline__ uint32_t edge_w(const uint32_t *w, uint32_t e) { retu
 __device__ __forceinline__ void relax(Round &R, bool act, uint32_t v, uint32_t nd)
 {
     const SsspArgs &a = *R.a;
-    bool queued = false;
+    bool lowered = false;
     if (act && nd < a.dist[v]) {
-        const uint32_t old = atomicMin(&a.dist[v], nd);
-        if (nd < old) {
-            R.minv = min(R.minv, nd);
-            if (a.stamp[v] != R.L + 2) {
-                a.stamp[v] = R.L + 2;
-                const uint32_t dv = a.g.row[v + 1] - a.g.row[v];
-                R.scout += dv;
-                R.count++;
-                R.huge |= dv > kHugeDegree;
-                queued = true;
-            }
-        }
+        lowered = nd < atomicMin(&a.dist[v], nd);
+        if (lowered) R.minv = min(R.minv, nd);
     }
-    wave_set_bits(R.fnext, queued, v);
+    // the lane claiming v's bit in the next frontier queues it
+    const bool queued = wave_claim_bits(R.fnext, lowered, v);
+    if (queued) {
+        const uint32_t dv = a.g.row[v + 1] - a.g.row[v];
+        R.scout += dv;
+        R.count++;
+        R.huge |= dv > kHugeDegree;
+    }
     stage(R.S, queued, v);
 }
 
@@ -130,19 +124,14 @@ __device__ __forceinline__ void relax(Round &R, bool act, uint32_t v, uint32_t n
 __device__ __forceinline__ void defer(Round &R, bool d, uint32_t u, uint32_t du)
 {
     const SsspArgs &a = *R.a;
-    bool queued = false;
-    if (d) {
-        R.minv = min(R.minv, du);
-        if (a.stamp[u] != R.L + 2) {
-            a.stamp[u] = R.L + 2;
-            const uint32_t dv = a.g.row[u + 1] - a.g.row[u];
-            R.scout += dv;
-            R.count++;
-            R.huge |= dv > kHugeDegree;
-            queued = true;
-        }
+    if (d) R.minv = min(R.minv, du);
+    const bool queued = wave_claim_bits(R.fnext, d, u);
+    if (queued) {
+        const uint32_t dv = a.g.row[u + 1] - a.g.row[u];
+        R.scout += dv;
+        R.count++;
+        R.huge |= dv > kHugeDegree;
     }
-    wave_set_bits(R.fnext, queued, u);
     stage(R.S, queued, u);
 }
 
@@ -249,7 +238,6 @@ __device__ __forceinline__ void pull_word(Round &R, uint32_t w)
     const bool improved = v < g.n && best < dv0;
     if (improved) {
         a.dist[v] = best;
-        a.stamp[v] = R.L + 2;
         const uint32_t dv = g.row[v + 1] - g.row[v];
         R.scout += dv;
         R.huge |= dv > kHugeDegree;
@@ -304,9 +292,9 @@ __device__ __forceinline__ void pull_hub_chunks(Round &R, const uint2 (&dsc)[NC]
         if (ln == 0 && m < a.dist[v]) {
             const uint32_t old = atomicMin(&a.dist[v], m);
             if (m < old) {
-                atomicOr((unsigned long long *)&R.fnext[v >> 6], 1ull << (v & 63));
-                if (a.stamp[v] != R.L + 2) {
-                    a.stamp[v] = R.L + 2;
+                // several chunks may lower the hub: the one setting its bit counts it
+                const uint64_t bit = 1ull << (v & 63);
+                if (!(atomicOr((unsigned long long *)&R.fnext[v >> 6], (unsigned long long)bit) & bit)) {
                     const uint32_t dv = g.row[v + 1] - g.row[v];
                     R.scout += dv;
                     R.count++;
@@ -338,10 +326,7 @@ __device__ __forceinline__ void sssp_run(const SsspArgs &a, uint32_t brank, uint
     __shared__ uint32_t s_qn, s_flush[2], s_cmin, s_bmin;
 
     const uint32_t src = a.source;
-    for (uint32_t i = gtid; i < n; i += nthreads) {
-        a.dist[i] = (i == src) ? 0u : kInf;
-        a.stamp[i] = (i == src) ? 1u : 0u;
-    }
+    for (uint32_t i = gtid; i < n; i += nthreads) a.dist[i] = (i == src) ? 0u : kInf;
     // frontier 0 = the source; frontier 1's bitmap starts clear (round L
     // clears L+2's)
     for (uint32_t w = gtid; w < 2 * W; w += nthreads) a.fbit[w] = w == (src >> 6) ? 1ull << (src & 63) : 0ull;
@@ -407,8 +392,7 @@ __device__ __forceinline__ void sssp_run(const SsspArgs &a, uint32_t brank, uint
         R.a = &a;
         R.L = L;
         // every round leaves its next frontier in fnext (pull rounds by whole
-        // words, push rounds one atomicOr per queued vertex)
-        R.fbit_valid = true;
+        // words, push rounds one atomicOr per run of queued vertices)
         R.fcur = a.fbit + (size_t)(L % 3) * W;
         R.fnext = a.fbit + (size_t)((L + 1) % 3) * W;
         // the bitmap for round L+2 is free (read in round L-1): clear it
@@ -481,7 +465,7 @@ __device__ __forceinline__ void sssp_run(const SsspArgs &a, uint32_t brank, uint
             } else {
                 for (uint32_t w = gwave; w < W; w += nwaves) {
                     const uint32_t v = w * 64 + ln;
-                    push_expand(R, v < n && a.stamp[v] == L + 1, v);
+                    push_expand(R, v < n && test_bit(R.fcur, v), v);
                 }
             }
         } else {
@@ -558,10 +542,9 @@ __global__ __launch_bounds__(BS) void k_sssp(SsspArgs a)
     __syncthreads();
     const uint32_t q = a.gr.slot[xcc];
     if (q >= a.gr.nq) return;  // an XCD without a query this launch
-    const size_t n = a.g.n, W = a.g.words;
+    const size_t W = a.g.words;
     SsspArgs b = a;
     b.dist = a.gr.out[q];
-    b.stamp = a.stamp + (size_t)q * n;
     b.fbit = a.fbit + (size_t)q * 3 * W;
     b.queue = a.queue + (size_t)q * 3 * a.qcap;
     b.ctr = a.ctr + (size_t)q * 3;
@@ -582,7 +565,7 @@ __global__ __launch_bounds__(BS) void k_sssp(SsspArgs a)
 template <int BS>
 int launch_sssp(sg_engine *e, SsspArgs &args, uint32_t *group_words)
 {
-    if (!args.abort || !args.stats || (!args.dist && !args.grouped) || !args.stamp || !args.fbit || !args.queue ||
+    if (!args.abort || !args.stats || (!args.dist && !args.grouped) || !args.fbit || !args.queue ||
         !args.ctr || (args.grouped && (!group_words || !args.gr.ticket || !args.gr.status)))
         SG_FAIL(SG_ERR_STATE, "sg_sssp: launch with unset device state");
     sg_engine::PGrid *pg = nullptr;
@@ -674,10 +657,9 @@ int sssp_group_launch(sg_engine *e, const uint32_t *sources, int nq, uint32_t *c
         if (sources[i] >= d.n) SG_FAIL(SG_ERR_RANGE, "concurrent SSSP: source >= n");
     int rc = engine_ensure_bfs_state(e);
     if (rc) return rc;
-    const uint64_t n = d.n, W = d.words, qc = e->qcap;
+    const uint64_t W = d.words, qc = e->qcap;
     auto al = [](uint64_t x) { return (x + 255) / 256 * 256; };
-    const uint64_t o_stamp = 0;
-    const uint64_t o_fbit = o_stamp + al(kMaxXcd * n * 4);
+    const uint64_t o_fbit = 0;
     const uint64_t o_queue = o_fbit + al(kMaxXcd * 3 * W * 8);
     const uint64_t o_ctr = o_queue + al(kMaxXcd * 3 * qc * 4);
     const uint64_t o_stats = o_ctr + al(kMaxXcd * 3 * sizeof(FrontierCtr));
@@ -698,7 +680,6 @@ int sssp_group_launch(sg_engine *e, const uint32_t *sources, int nq, uint32_t *c
     char *base = (char *)e->garena;
     SsspArgs a;
     if ((rc = sssp_args(e, a))) return rc;
-    a.stamp = (uint32_t *)(base + o_stamp);
     a.fbit = (uint64_t *)(base + o_fbit);
     a.queue = (uint32_t *)(base + o_queue);
     a.qcap = e->qcap;
@@ -730,7 +711,6 @@ int sssp_single_launch(sg_engine *e, uint32_t source, uint32_t *dist, uint32_t *
         SG_FAIL(SG_ERR_RANGE, "sssp_single_launch: bad source or row");
     int rc = engine_ensure_bfs_state(e);
     if (rc) return rc;
-    if ((rc = engine_ensure_stamp(e))) return rc;
     SsspArgs a;
     if ((rc = sssp_args(e, a))) return rc;
     a.dist = dist;
@@ -738,7 +718,6 @@ int sssp_single_launch(sg_engine *e, uint32_t source, uint32_t *dist, uint32_t *
         a.out = out;
         a.inv = e->g->d_inv;
     }
-    a.stamp = e->stamp;
     a.fbit = e->bitmaps;
     a.queue = e->queues;
     a.qcap = e->qcap;
@@ -792,9 +771,7 @@ extern "C" int sg_sssp(sg_engine *e, uint32_t source)
     if ((rc = sssp_args(e, a))) return rc;
     void *res = nullptr;
     if ((rc = engine_result_rows(e, 1, &res))) return rc;
-    if ((rc = engine_ensure_stamp(e))) return rc;
     a.dist = (uint32_t *)res;
-    a.stamp = e->stamp;
     a.fbit = e->bitmaps;  // three rotating frontier bitmaps
     a.queue = e->queues;
     a.qcap = e->qcap;
