@@ -290,8 +290,9 @@ int sg_sssp_concurrent(sg_engine *e, const uint32_t *sources, int32_t k);
  * BFS / SSSP / BC-dependency queries with continuous batching: a worker
  * forms each batch (up to 64 sources of one kind, the kind whose oldest
  * query has waited longest) when the GPU frees up, from whatever is queued.
- * BFS and BC-dependency batches run as one bit-packed multi-source launch;
- * an SSSP batch runs its queries back to back, each over the whole GPU.
+ * BFS and BC-dependency batches run as one bit-packed multi-source launch,
+ * as do SSSP batches of 8 or more queries; a smaller SSSP batch runs its
+ * queries back to back, each over the whole GPU.
  * Admission is by device memory: each query holds one result row of n
  * 32-bit values (int32 levels, uint32 distances, FP32 0.5*delta) from the
  * start of its batch until released; rows = memory_budget / (4 n). Results
@@ -344,9 +345,9 @@ int sg_server_replay(sg_server *s, const int32_t *kinds, const uint32_t *sources
  * on k sample sources of the engine's graph and picks: alpha/beta and block
  * size for sg_bfs, alpha and block size for sg_sssp, and per query kind the
  * sources per launch with the highest queries/s whose launch time stays
- * within latency_budget_ms (for SSSP, whose server batches run whole-GPU
- * queries back to back, the largest of 1, 2, 4, 8 queries within half the
- * budget); feed out->max_batch to sg_server_config. Leaves
+ * within latency_budget_ms (for SSSP, between whole-GPU queries run back to
+ * back, 1, 2 or 4 per batch, and bit-packed launches of 8 to 64); feed
+ * out->max_batch to sg_server_config. Leaves
  * the engine on the tuned BFS options. Results never depend on these knobs. */
 typedef struct sg_tune_config {
     double latency_budget_ms;  /* per batched launch (default 25) */

@@ -429,7 +429,7 @@ __global__ SG_BFS_BOUNDS(BS) void k_bfs(BfsArgs a)
 }
 
 template <int BS>
-int launch_bfs(sg_engine *e, BfsArgs &args)
+int launch_bfs(sg_engine *e, BfsArgs &args, bool timed)
 {
     sg_engine::PGrid *pg = nullptr;
     auto probe = [e, args](int blocks, uint32_t *words) -> int {
@@ -445,29 +445,32 @@ int launch_bfs(sg_engine *e, BfsArgs &args)
     args.bar.words = pg->words;
     args.bar.nxcd = pg->nxcd;
     for (int x = 0; x < kMaxXcd; x++) args.bar.cnt[x] = pg->cnt[x];
-    SG_HIP(hipEventRecord(e->ev0, e->stream));
+    if (timed) SG_HIP(hipEventRecord(e->ev0, e->stream));
     hipLaunchKernelGGL(k_bfs<BS>, dim3(pg->blocks), dim3(BS), 0, e->stream, args);
     SG_HIP(hipGetLastError());
-    SG_HIP(hipEventRecord(e->ev1, e->stream));
+    if (timed) SG_HIP(hipEventRecord(e->ev1, e->stream));
     return SG_OK;
 }
 
 }  // namespace
 
-extern "C" int sg_bfs(sg_engine *e, uint32_t source)
+namespace sg {
+
+// One BFS query from the caller's vertex `source` writing int32 levels, in the
+// caller's vertex order, to the device row `out` (n entries). Asynchronous;
+// with `traced`, the run also fills the engine's trace and kernel timing.
+int bfs_launch(sg_engine *e, uint32_t source, int32_t *out, bool traced)
 {
-    if (!e) SG_FAIL(SG_ERR_INVALID, "sg_bfs: null engine");
     const DevGraph &d = e->g->dev;
-    if (source >= d.n) SG_FAIL(SG_ERR_RANGE, "sg_bfs: source >= n");
+    if (source >= d.n || !out) SG_FAIL(SG_ERR_RANGE, "sg_bfs: source >= n");
     const sg_engine_options &o = e->opt;
     const int force = forced_direction(o);
     if (force < -1) SG_FAIL(SG_ERR_UNSUPPORTED, "sg_bfs: only data-driven push/pull variants apply to BFS");
     int rc = engine_ensure_bfs_state(e);
     if (rc) return rc;
-    if ((rc = engine_ensure_value(e, (uint64_t)d.n * 4))) return rc;
     BfsArgs a{};
     a.g = d;
-    a.level = (int32_t *)e->value;
+    a.level = out;
     a.perm = e->g->relabeled ? e->g->d_perm : nullptr;
     a.vis = e->bitmaps;
     a.fbit = e->bitmaps + d.words;
@@ -477,7 +480,7 @@ extern "C" int sg_bfs(sg_engine *e, uint32_t source)
     a.bar.spin_limit = 1u << 22;
     a.abort = e->abort_flag;
     a.stats = e->stats;
-    a.trace = e->trace;
+    a.trace = traced ? e->trace : nullptr;
     a.source = dev_vertex(e->g, source);
     a.source_id = source;
     a.force = force;
@@ -488,12 +491,24 @@ extern "C" int sg_bfs(sg_engine *e, uint32_t source)
     e->last_algo = 0;
     const int bs = o.block_size ? o.block_size : 1024;
     switch (bs) {
-    case 256: rc = launch_bfs<256>(e, a); break;
-    case 512: rc = launch_bfs<512>(e, a); break;
-    case 1024: rc = launch_bfs<1024>(e, a); break;
+    case 256: rc = launch_bfs<256>(e, a, traced); break;
+    case 512: rc = launch_bfs<512>(e, a, traced); break;
+    case 1024: rc = launch_bfs<1024>(e, a, traced); break;
     default: SG_FAIL(SG_ERR_UNSUPPORTED, "sg_bfs: block_size must be 256, 512 or 1024");
     }
     if (rc) return rc;
-    e->timed = true;
+    if (traced) e->timed = true;
     return SG_OK;
+}
+
+}  // namespace sg
+
+extern "C" int sg_bfs(sg_engine *e, uint32_t source)
+{
+    if (!e) SG_FAIL(SG_ERR_INVALID, "sg_bfs: null engine");
+    const DevGraph &d = e->g->dev;
+    if (source >= d.n) SG_FAIL(SG_ERR_RANGE, "sg_bfs: source >= n");
+    int rc = engine_ensure_value(e, (uint64_t)d.n * 4);
+    if (rc) return rc;
+    return bfs_launch(e, source, (int32_t *)e->value, true);
 }

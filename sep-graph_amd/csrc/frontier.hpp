@@ -334,7 +334,7 @@ struct DirectionPolicy {
 __device__ __forceinline__ void write_trace(DevTrace *trace, uint32_t L, bool pull, unsigned long long count,
                                             unsigned long long scout, long long t_start)
 {
-    if (L < kTraceRows) {
+    if (trace && L < kTraceRows) {
         DevTrace t;
         t.round = L;
         t.variant = pull ? SG_ASYNC_PULL_DD : SG_ASYNC_PUSH_DD;

@@ -575,6 +575,18 @@ int engine_gather_rows(sg_engine *e, const void *src, void *dst, uint64_t rows)
     return SG_OK;
 }
 
+int engine_run_status(sg_engine *e)
+{
+    SG_HIP(hipStreamSynchronize(e->stream));
+    int32_t st = 0;
+    SG_HIP(hipMemcpy(&st, (const char *)e->stats + offsetof(DevStats, status), 4, hipMemcpyDeviceToHost));
+    if (st != SG_OK) {
+        e->needs_bar_reset = true;
+        SG_FAIL(st, "engine run reported an error status");
+    }
+    return SG_OK;
+}
+
 int engine_finish(sg_engine *e, uint64_t rows)
 {
     return engine_gather_rows(e, e->work, e->value, rows);

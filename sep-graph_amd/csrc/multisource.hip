@@ -728,7 +728,7 @@ struct MsRelax {
 
 // Edge steps a group of lanes keeps in flight in the batched relaxation.
 #ifndef SG_MS_RELAX_STEPS
-#define SG_MS_RELAX_STEPS 4
+#define SG_MS_RELAX_STEPS 16
 #endif
 constexpr int kMsRelaxSteps = SG_MS_RELAX_STEPS;
 
@@ -765,36 +765,45 @@ __device__ __forceinline__ void ms_relax_range(MsRelax &R, uint32_t u, uint32_t 
         }
 #pragma unroll
         for (int s = 0; s < U; s++) cur[s] = act[s] ? a.dist[(size_t)v[s] * 64 + q] : 0u;
-        bool imp[U];
+        // every atomic of the step is issued before any result is read (a
+        // result consumed inside its own branch makes each one wait in turn)
+        uint32_t old[U];
 #pragma unroll
         for (int s = 0; s < U; s++) {
-            imp[s] = false;
-            if (act[s] && nd[s] < cur[s]) imp[s] = nd[s] < atomicMin(&a.dist[(size_t)v[s] * 64 + q], nd[s]);
+            old[s] = 0xFFFFFFFFu;
+            if (act[s] && nd[s] < cur[s]) old[s] = atomicMin(&a.dist[(size_t)v[s] * 64 + q], nd[s]);
         }
-        bool first[U];
+        unsigned long long fo[U];
 #pragma unroll
         for (int s = 0; s < U; s++) {
-            const uint64_t im = ballot(imp[s]);
+            const bool imp = act[s] && nd[s] < cur[s] && nd[s] < old[s];
+            const uint64_t im = ballot(imp);
             const uint64_t gm = (im >> (grp << qb)) & gmask;
-            first[s] = false;
-            if (q == 0 && gm)
-                first[s] = atomicOr((unsigned long long *)&R.Fn[v[s]], (unsigned long long)gm) == 0;
+            fo[s] = ~0ull;
+            if (q == 0 && gm) fo[s] = atomicOr((unsigned long long *)&R.Fn[v[s]], (unsigned long long)gm);
         }
+        // heads whose mask this wave set first: one append for the whole step
+        uint64_t fm[U];
+        uint32_t tot = 0;
 #pragma unroll
         for (int s = 0; s < U; s++) {
-            const uint64_t fm = ballot(first[s]);
-            if (!fm) continue;
-            const int l0 = first_lane(fm);
-            uint32_t base = 0;
-            if (ln == l0) base = atomicAdd(R.cnt_next, (uint32_t)__popcll(fm));
-            base = shfl(base, l0);
-            if (first[s]) {
-                const uint32_t pos = base + (uint32_t)rank_in(fm);
+            fm[s] = ballot(fo[s] == 0);
+            tot += (uint32_t)__popcll(fm[s]);
+        }
+        if (!tot) continue;
+        uint32_t base = 0;
+        if (ln == 0) base = atomicAdd(R.cnt_next, tot);
+        base = shfl(base, 0);
+#pragma unroll
+        for (int s = 0; s < U; s++) {
+            if (fo[s] == 0) {
+                const uint32_t pos = base + (uint32_t)rank_in(fm[s]);
                 if (pos < g.n) R.ring_next[pos] = v[s];
                 else a.ctl[1] = 1;
                 R.scout += g.row[v[s] + 1] - g.row[v[s]];
                 R.count++;
             }
+            base += (uint32_t)__popcll(fm[s]);
         }
     }
 }

@@ -12,11 +12,11 @@
 //     oldest request has waited longest, up to that kind's batch limit (<= 64)
 //     -- so requests that arrive while a batch runs join the next one;
 //   * few launches per batch, each writing every query's result straight into
-//     its row of the pool: BFS and BC-dependency batches run the bit-packed
-//     multi-source kernels (up to 64 sources per launch); SSSP batches run
-//     their queries one after another, each over the whole grid, which on
-//     RMAT-22 is faster per query than a bit-packed batch (whose
-//     Bellman-Ford rounds repeat relaxations) or one query per XCD.
+//     its row of the pool: BC-dependency batches, BFS batches of at least
+//     kMsBfsMin and SSSP batches of at least kMsSsspMin run the bit-packed
+//     multi-source kernels (up to 64 sources per launch); smaller BFS and
+//     SSSP batches run their queries one after another over the whole grid,
+//     which is faster per query than a small bit-packed launch.
 // Batches are serialised on the engine's stream: a persistent kernel assumes
 // all of its blocks are co-resident, which two concurrent launches would not
 // be. Host work (queueing, result hand-off) overlaps the running batch.
@@ -49,6 +49,16 @@ int kind_to_ms(int kind)
 
 enum { kQueued = 0, kRunning = 1, kDone = 2, kFailed = 3 };
 
+// Smallest SSSP batch run as one bit-packed launch: below it, whole-GPU
+// single queries are faster per query (RMAT-22: 2.2 ms alone against 1.8
+// ms per query for 8 sources per launch and 1.2 ms for 16).
+constexpr uint32_t kMsSsspMin = 8;
+// How long a short SSSP queue may yield to other kinds' batches.
+constexpr double kSsspHoldMs = 10.0;
+// Smallest BFS batch run as one bit-packed launch (RMAT-22: a lone BFS takes
+// 0.17 ms; bit-packed, 16 sources take 0.19 ms per source, 64 take 0.08).
+constexpr uint32_t kMsBfsMin = 16;
+
 }  // namespace
 
 struct sg_server {
@@ -60,10 +70,12 @@ struct sg_server {
     void *pool = nullptr;
     uint32_t *d_src = nullptr, *d_rows = nullptr;  // 64 entries each
     uint32_t *h_src = nullptr, *h_rows = nullptr;  // pinned staging
+    uint32_t h_qsrc[64] = {};                      // the batch's sources, caller's ids
 
     struct Req {
         int kind;
-        uint32_t source;
+        uint32_t source;     // device vertex id
+        uint32_t source_id;  // the caller's id
         int32_t row;
         int state;
         int status;
@@ -112,14 +124,24 @@ struct sg_server {
             std::unique_lock<std::mutex> lk(mu);
             cv_work.wait(lk, [&] { return stop || has_work(); });
             if (stop) return;
+            // the kind whose oldest query has waited longest; an SSSP queue
+            // too short for a bit-packed launch yields to the other kinds for
+            // up to kSsspHoldMs, so that under load SSSP batches grow into
+            // the cheaper bit-packed form instead of running one by one
+            const double tnow = now_s();
             int kind = -1;
             double oldest = 0.0;
-            for (int k = 0; k < SG_QUERY_KINDS; k++) {
-                if (queue[k].empty()) continue;
-                const double t = reqs[queue[k].front()].t_arrival;
-                if (kind < 0 || t < oldest) {
-                    kind = k;
-                    oldest = t;
+            for (int pass = 0; pass < 2 && kind < 0; pass++) {
+                for (int k = 0; k < SG_QUERY_KINDS; k++) {
+                    if (queue[k].empty()) continue;
+                    const double t = reqs[queue[k].front()].t_arrival;
+                    if (pass == 0 && k == SG_QUERY_SSSP && queue[k].size() < kMsSsspMin &&
+                        (tnow - t) * 1e3 < kSsspHoldMs)
+                        continue;
+                    if (kind < 0 || t < oldest) {
+                        kind = k;
+                        oldest = t;
+                    }
                 }
             }
             const uint32_t limit = (uint32_t)std::max(1, std::min(64, cfg.max_batch[kind] ? cfg.max_batch[kind] : 64));
@@ -135,6 +157,7 @@ struct sg_server {
                 r.state = kRunning;
                 r.t_start = t0;
                 h_src[i] = r.source;
+                h_qsrc[i] = r.source_id;
                 h_rows[i] = (uint32_t)r.row;
                 tickets.push_back(t);
             }
@@ -146,15 +169,21 @@ struct sg_server {
             const bool via_work = g->relabeled;
             const size_t n = g->dev.n;
             uint32_t *work = (uint32_t *)e->work;
-            bool ok;
-            if (kind == SG_QUERY_SSSP) {
+            bool ok = true;
+            if (kind == SG_QUERY_BFS && b < kMsBfsMin) {
+                // one whole-grid query after another, each writing its row in
+                // the caller's order
+                for (uint32_t i = 0; ok && i < b; i++) {
+                    int32_t *row = (int32_t *)pool + (size_t)h_rows[i] * n;
+                    ok = bfs_launch(e, h_qsrc[i], row, false) == SG_OK && engine_run_status(e) == SG_OK;
+                }
+            } else if (kind == SG_QUERY_SSSP && b < kMsSsspMin) {
                 // one whole-grid query after another, each into its row
-                ok = true;
                 // (a relabelled graph's kernel gathers into the pool row)
                 for (uint32_t i = 0; ok && i < b; i++) {
                     uint32_t *row = (uint32_t *)pool + (size_t)h_rows[i] * n;
                     ok = sssp_single_launch(e, h_src[i], via_work ? work : row, row) == SG_OK &&
-                         sssp_single_status(e) == SG_OK;
+                         engine_run_status(e) == SG_OK;
                 }
             } else {
                 ok = hipMemcpyAsync(d_src, h_src, b * 4, hipMemcpyHostToDevice, e->stream) == hipSuccess &&
@@ -162,10 +191,10 @@ struct sg_server {
                      hipMemsetAsync(e->stats, 0, sizeof(DevStats), e->stream) == hipSuccess;
                 ok = ok && ms_launch(e, kind_to_ms(kind), d_src, b, via_work ? (void *)work : pool,
                                      via_work ? nullptr : d_rows, true, true) == SG_OK;
+                for (uint32_t j = 0; ok && via_work && j < b; j++)
+                    ok = engine_gather_rows(e, work + j * n, (uint32_t *)pool + (size_t)h_rows[j] * n, 1) == SG_OK;
+                ok = ok && engine_run_status(e) == SG_OK;
             }
-            for (uint32_t j = 0; ok && via_work && kind != SG_QUERY_SSSP && j < b; j++)
-                ok = engine_gather_rows(e, work + j * n, (uint32_t *)pool + (size_t)h_rows[j] * n, 1) == SG_OK;
-            ok = ok && sg_engine_sync(e) == SG_OK;
             const double t1 = now_s();
 
             lk.lock();
@@ -221,9 +250,10 @@ extern "C" int sg_server_create(sg_graph *g, const sg_server_config *cfg, const 
         return code;
     };
     // working arenas of every kind (the largest wins) and the result pool;
-    // SSSP runs single-source queries on the engine's own frontier state,
-    // BFS and BC-dependency batches use the bit-packed kernels
+    // small SSSP batches run single-source queries on the engine's own
+    // frontier state, everything else the bit-packed kernels
     if ((rc = ms_prepare(s->e, kMsBfs))) return fail(rc);
+    if ((rc = ms_prepare(s->e, kMsSssp))) return fail(rc);
     if ((rc = ms_prepare(s->e, kMsBcDep))) return fail(rc);
     if (g->relabeled && (rc = engine_ensure_work(s->e, 64ull * g->dev.n * 4))) return fail(rc);
     if ((rc = engine_alloc(&s->pool, (uint64_t)s->nrows * row_bytes))) return fail(rc);
@@ -274,6 +304,7 @@ int submit_locked(sg_server *s, int32_t kind, uint32_t source, double t_arrival,
     sg_server::Req r{};
     r.kind = kind;
     r.source = dev_vertex(s->g, source);  // device vertex order
+    r.source_id = source;
     r.row = -1;
     r.state = kQueued;
     r.status = SG_OK;

@@ -228,9 +228,15 @@ int sssp_group_status(sg_engine *e);
 // One SSSP query over the whole grid (device source id) computing in the
 // device row `dist` (device order); a relabelled graph's result is gathered
 // into the row `out` (caller's order) by the kernel. Asynchronous;
-// sssp_single_status synchronises and reports it.
+// engine_run_status synchronises and reports it.
 int sssp_single_launch(sg_engine *e, uint32_t source, uint32_t *dist, uint32_t *out);
-int sssp_single_status(sg_engine *e);
+// One BFS query from the caller's vertex `source` writing int32 levels in the
+// caller's vertex order to the device row `out`; asynchronous. `traced` also
+// fills the engine's round trace and kernel timing (sg_bfs).
+int bfs_launch(sg_engine *e, uint32_t source, int32_t *out, bool traced);
+// Synchronise the engine's stream and return the status the last run wrote
+// to its device stats (marking the barriers for a re-probe after a failure).
+int engine_run_status(sg_engine *e);
 
 // Device allocation owned by the engine (freed with it); reports SG_ERR_NOMEM.
 int engine_alloc(void **p, uint64_t bytes);

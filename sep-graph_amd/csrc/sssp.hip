@@ -704,7 +704,7 @@ int sssp_group_launch(sg_engine *e, const uint32_t *sources, int nq, uint32_t *c
 // distances in the device row `dist` (device vertex order) and, for a
 // relabelled graph, gathered into the row `out` in the caller's order.
 // Asynchronous; its status lands in the engine's stats, overwritten by the
-// next run (see sssp_single_status).
+// next run (see engine_run_status).
 int sssp_single_launch(sg_engine *e, uint32_t source, uint32_t *dist, uint32_t *out)
 {
     if (source >= e->g->dev.n || !dist || (e->g->relabeled && !out))
@@ -728,19 +728,6 @@ int sssp_single_launch(sg_engine *e, uint32_t source, uint32_t *dist, uint32_t *
     a.source = source;
     e->last_algo = 1;
     return launch_sssp_bs(e, a, nullptr);
-}
-
-// Status of the last single-query launch (synchronises the stream).
-int sssp_single_status(sg_engine *e)
-{
-    SG_HIP(hipStreamSynchronize(e->stream));
-    int32_t st = 0;
-    SG_HIP(hipMemcpy(&st, (const char *)e->stats + offsetof(DevStats, status), 4, hipMemcpyDeviceToHost));
-    if (st != SG_OK) {
-        e->needs_bar_reset = true;
-        SG_FAIL(st, "SSSP: the query reported an error");
-    }
-    return SG_OK;
 }
 
 // Worst status of the last grouped launch (synchronises the stream).

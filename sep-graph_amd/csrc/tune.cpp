@@ -157,23 +157,37 @@ extern "C" int sg_autotune(sg_engine *e, const uint32_t *sources, int32_t k, con
         for (int kind = 0; kind < SG_QUERY_KINDS; kind++) {
             double best_qps = -1.0;
             if (kind == SG_QUERY_SSSP) {
-                // the server runs an SSSP batch as whole-grid queries one
-                // after another: throughput is one query's, so the limit only
-                // bounds how long a batch holds the GPU (half the budget)
+                // the server runs an SSSP batch below 8 queries as whole-grid
+                // queries one after another (throughput: one query's; the
+                // limit bounds how long the batch holds the GPU), and from 8
+                // queries as one bit-packed launch; the best queries/s within
+                // the budget wins
                 const double t = mean_ms(e, sources, std::min(k, 4), [&](uint32_t s) { return sg_sssp(e, s); });
                 if (t < 0) return SG_ERR_HIP;
                 int b = 1;
-                while (b < 8 && 2 * b * t <= 0.5 * c.latency_budget_ms) b *= 2;
+                while (b < 4 && 2 * b * t <= c.latency_budget_ms) b *= 2;
                 out->max_batch[kind] = b;
                 out->batch_ms[kind] = b * t;
                 out->batch_qps[kind] = t > 0 ? 1e3 / t : 0.0;
+                for (int bb : {8, 16, 32, 64}) {
+                    if ((rc = sg_sssp_multi(e, src.data(), bb, bb))) return rc;  // warm
+                    if ((rc = sg_sssp_multi(e, src.data(), bb, bb))) return rc;
+                    sg_stats st;
+                    if ((rc = sg_engine_get_stats(e, &st))) return rc;
+                    if (st.kernel_ms > c.latency_budget_ms) break;
+                    const double qps = bb / (st.kernel_ms / 1e3);
+                    if (qps > out->batch_qps[kind]) {
+                        out->max_batch[kind] = bb;
+                        out->batch_ms[kind] = st.kernel_ms;
+                        out->batch_qps[kind] = qps;
+                    }
+                }
                 continue;
             }
             for (int b : sizes) {
                 auto run = [&]() {
                     return kind == SG_QUERY_BFS ? sg_bfs_multi(e, src.data(), b, b)
-                           : kind == SG_QUERY_SSSP ? sg_sssp_multi(e, src.data(), b, b)
-                                                   : sg_bc_dependency(e, src.data(), b, b);
+                                                : sg_bc_dependency(e, src.data(), b, b);
                 };
                 if ((rc = run())) return rc;  // warm (grid probe, arena)
                 if ((rc = run())) return rc;

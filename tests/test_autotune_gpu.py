@@ -19,10 +19,10 @@ def test_autotune_rmat16(sg, orc, gpu, rmat16_sym):
     assert 0 < r["sssp_ms"] <= r["sssp_default_ms"] * 1.0001
     for k in (sg.QUERY_BFS, sg.QUERY_BC_DEP):
         assert r["max_batch"][k] in (4, 8, 16, 32, 64) and r["batch_qps"][k] > 0
-    # SSSP batches run whole-grid queries back to back: the limit keeps a
-    # batch within half the latency budget
-    assert r["max_batch"][sg.QUERY_SSSP] in (1, 2, 4, 8) and r["batch_qps"][sg.QUERY_SSSP] > 0
-    assert r["batch_ms"][sg.QUERY_SSSP] <= 5.0 or r["max_batch"][sg.QUERY_SSSP] == 1
+    # SSSP: whole-grid queries back to back (1, 2, 4) or one bit-packed
+    # launch (8 and up), within the latency budget
+    assert r["max_batch"][sg.QUERY_SSSP] in (1, 2, 4, 8, 16, 32, 64) and r["batch_qps"][sg.QUERY_SSSP] > 0
+    assert r["batch_ms"][sg.QUERY_SSSP] <= 10.0 or r["max_batch"][sg.QUERY_SSSP] == 1
     # the engine is left on the tuned BFS options; levels unchanged
     for s in srcs[:3]:
         e.bfs(int(s))
