@@ -306,7 +306,10 @@ typedef struct sg_server_config {
     uint64_t memory_budget;  /* device bytes for result rows */
     int32_t max_rows;        /* cap on result rows (0: the budget alone) */
     int32_t max_batch[3];    /* per kind: queries per batch, 1..64 (0: 64) */
-    int32_t reserved[8];
+    /* smallest BFS / SSSP batch run as one bit-packed launch; smaller ones run
+     * whole-GPU single queries back to back (0: 16 for BFS, 8 for SSSP) */
+    int32_t bfs_packed_min, sssp_packed_min;
+    int32_t reserved[6];
 } sg_server_config;
 
 typedef struct sg_server_stats {

@@ -94,8 +94,11 @@ __device__ __forceinline__ uint64_t seg_or(uint64_t s, int k)
 }
 
 // OR over in-edges [b, e) of each lane's vertex of Fc[csrc[edge]], the wave
-// walking the concatenated lists 64 edges per step.
-__device__ __forceinline__ uint64_t wave_gather_or(const DevGraph &g, const uint64_t *Fc, uint32_t b, uint32_t e)
+// walking the concatenated lists 64 edges per step. A lane stops once its OR
+// holds every bit of `need` (the reference's early break, generalised to a
+// set of sources: with one source it stops at the first parent).
+__device__ __forceinline__ uint64_t wave_gather_or(const DevGraph &g, const uint64_t *Fc, uint32_t b, uint32_t e,
+                                                   uint64_t need)
 {
     const int ln = lane();
     uint64_t acc = 0;
@@ -116,6 +119,7 @@ __device__ __forceinline__ uint64_t wave_gather_or(const DevGraph &g, const uint
         if (cov) {
             acc |= seg;
             b += cov;
+            if ((acc & need) == need) b = e;
         }
     }
     return acc;
@@ -250,7 +254,7 @@ __device__ __forceinline__ void ms_discover(const DevGraph &g, const uint64_t *s
                 e = g.cstart[v + 1];
                 if (!unseen || e - b > kPullHubDegree) b = e;
             }
-            const uint64_t o = wave_gather_or(g, Fc, b, e) & unseen;
+            const uint64_t o = wave_gather_or(g, Fc, b, e, unseen) & unseen;
             if (o) Fn[v] = o;
         }
     }

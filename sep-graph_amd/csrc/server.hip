@@ -12,11 +12,12 @@
 //     oldest request has waited longest, up to that kind's batch limit (<= 64)
 //     -- so requests that arrive while a batch runs join the next one;
 //   * few launches per batch, each writing every query's result straight into
-//     its row of the pool: BC-dependency batches, BFS batches of at least
-//     kMsBfsMin and SSSP batches of at least kMsSsspMin run the bit-packed
-//     multi-source kernels (up to 64 sources per launch); smaller BFS and
-//     SSSP batches run their queries one after another over the whole grid,
-//     which is faster per query than a small bit-packed launch.
+//     its row of the pool: BC-dependency batches, and BFS / SSSP batches of
+//     at least the configured bfs_packed_min / sssp_packed_min (defaults
+//     kMsBfsMin, kMsSsspMin) run the bit-packed multi-source kernels (up to
+//     64 sources per launch); smaller BFS and SSSP batches run their queries
+//     one after another over the whole grid, which is faster per query than a
+//     small bit-packed launch.
 // Batches are serialised on the engine's stream: a persistent kernel assumes
 // all of its blocks are co-resident, which two concurrent launches would not
 // be. Host work (queueing, result hand-off) overlaps the running batch.
@@ -49,14 +50,15 @@ int kind_to_ms(int kind)
 
 enum { kQueued = 0, kRunning = 1, kDone = 2, kFailed = 3 };
 
-// Smallest SSSP batch run as one bit-packed launch: below it, whole-GPU
-// single queries are faster per query (RMAT-22: 2.2 ms alone against 1.8
-// ms per query for 8 sources per launch and 1.2 ms for 16).
+// Default smallest SSSP batch run as one bit-packed launch: below it,
+// whole-GPU single queries are faster per query (RMAT-22: 2.2 ms alone
+// against 1.8 ms per query for 8 sources per launch and 1.2 ms for 16).
 constexpr uint32_t kMsSsspMin = 8;
 // How long a short SSSP queue may yield to other kinds' batches.
 constexpr double kSsspHoldMs = 10.0;
-// Smallest BFS batch run as one bit-packed launch (RMAT-22: a lone BFS takes
-// 0.17 ms; bit-packed, 16 sources take 0.19 ms per source, 64 take 0.08).
+// Default smallest BFS batch run as one bit-packed launch (RMAT-22: a lone
+// BFS takes 0.17 ms; bit-packed, 16 sources take 0.19 ms per source, 64 take
+// 0.08).
 constexpr uint32_t kMsBfsMin = 16;
 
 }  // namespace
@@ -135,7 +137,7 @@ struct sg_server {
                 for (int k = 0; k < SG_QUERY_KINDS; k++) {
                     if (queue[k].empty()) continue;
                     const double t = reqs[queue[k].front()].t_arrival;
-                    if (pass == 0 && k == SG_QUERY_SSSP && queue[k].size() < kMsSsspMin &&
+                    if (pass == 0 && k == SG_QUERY_SSSP && queue[k].size() < (size_t)cfg.sssp_packed_min &&
                         (tnow - t) * 1e3 < kSsspHoldMs)
                         continue;
                     if (kind < 0 || t < oldest) {
@@ -170,14 +172,14 @@ struct sg_server {
             const size_t n = g->dev.n;
             uint32_t *work = (uint32_t *)e->work;
             bool ok = true;
-            if (kind == SG_QUERY_BFS && b < kMsBfsMin) {
+            if (kind == SG_QUERY_BFS && b < (uint32_t)cfg.bfs_packed_min) {
                 // one whole-grid query after another, each writing its row in
                 // the caller's order
                 for (uint32_t i = 0; ok && i < b; i++) {
                     int32_t *row = (int32_t *)pool + (size_t)h_rows[i] * n;
                     ok = bfs_launch(e, h_qsrc[i], row, false) == SG_OK && engine_run_status(e) == SG_OK;
                 }
-            } else if (kind == SG_QUERY_SSSP && b < kMsSsspMin) {
+            } else if (kind == SG_QUERY_SSSP && b < (uint32_t)cfg.sssp_packed_min) {
                 // one whole-grid query after another, each into its row
                 // (a relabelled graph's kernel gathers into the pool row)
                 for (uint32_t i = 0; ok && i < b; i++) {
@@ -228,6 +230,10 @@ extern "C" int sg_server_create(sg_graph *g, const sg_server_config *cfg, const 
     else sg_server_config_default(&c);
     for (int k = 0; k < SG_QUERY_KINDS; k++)
         if (c.max_batch[k] < 0 || c.max_batch[k] > 64) SG_FAIL(SG_ERR_RANGE, "sg_server_create: max_batch in [0, 64]");
+    if (c.bfs_packed_min < 0 || c.bfs_packed_min > 65 || c.sssp_packed_min < 0 || c.sssp_packed_min > 65)
+        SG_FAIL(SG_ERR_RANGE, "sg_server_create: packed_min in [0, 65]");
+    if (!c.bfs_packed_min) c.bfs_packed_min = (int32_t)kMsBfsMin;
+    if (!c.sssp_packed_min) c.sssp_packed_min = (int32_t)kMsSsspMin;
     const uint64_t n = g->dev.n;
     const uint64_t row_bytes = n * 4;
     uint64_t rows = c.memory_budget / row_bytes;

@@ -428,7 +428,9 @@ class ServerConfig(C.Structure):
         ("memory_budget", C.c_uint64),
         ("max_rows", C.c_int32),
         ("max_batch", C.c_int32 * 3),
-        ("reserved", C.c_int32 * 8),
+        ("bfs_packed_min", C.c_int32),
+        ("sssp_packed_min", C.c_int32),
+        ("reserved", C.c_int32 * 6),
     ]
 
 
@@ -482,7 +484,8 @@ def _server_lib():
 class Server:
     """sg_server: batched BFS / SSSP / BC-dependency queries on a resident graph."""
 
-    def __init__(self, graph: Graph, memory_budget: int = 4 << 30, max_rows: int = 4096, max_batch=(64, 64, 64), **opts):
+    def __init__(self, graph: Graph, memory_budget: int = 4 << 30, max_rows: int = 4096, max_batch=(64, 64, 64),
+                 bfs_packed_min: int = 0, sssp_packed_min: int = 0, **opts):
         L = _server_lib()
         self.graph = graph
         self.n = graph.host.n
@@ -492,6 +495,8 @@ class Server:
         cfg.max_rows = max_rows
         for i, b in enumerate(max_batch):
             cfg.max_batch[i] = b
+        cfg.bfs_packed_min = bfs_packed_min
+        cfg.sssp_packed_min = sssp_packed_min
         o = default_options(**opts)
         h = _P()
         _check(L.sg_server_create(graph.h, C.byref(cfg), C.byref(o), C.byref(h)))

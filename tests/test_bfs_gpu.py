@@ -68,6 +68,32 @@ def test_bfs_small_graphs(sg, orc, gpu, name, variant):
         assert st["reached_edges"] == int(deg[ref >= 0].sum())
 
 
+@pytest.mark.parametrize("relabel", [False, True])
+def test_bfs_sssp_multi_edges_unsorted(sg, orc, gpu, relabel):
+    """A raw CSR with repeated edges, self-loops and unsorted neighbour lists:
+    a wave's lanes then hit one frontier word in several separate runs and
+    with the same head twice, which the bit claims must handle (each vertex
+    claimed once, none lost)."""
+    rng = np.random.default_rng(21)
+    n = 3000
+    deg = rng.integers(0, 40, size=n)
+    deg[:8] = 600  # hubs, so some rows span several chunks
+    row = np.zeros(n + 1, np.uint32)
+    row[1:] = np.cumsum(deg)
+    # heads drawn from a narrow window around the tail: many land in the
+    # same 64-vertex word, repeats and self-loops included
+    tails = np.repeat(np.arange(n), deg)
+    col = ((tails + rng.integers(-70, 70, size=len(tails))) % n).astype(np.uint32)
+    w = rng.integers(1, 50, size=len(col)).astype(np.uint32)
+    hg = sg.HostGraph(n, row, col, w)
+    e = sg.Engine(sg.Graph(hg, relabel=relabel))
+    for s in (0, 5, 1234, n - 1):
+        e.bfs(s)
+        assert np.array_equal(e.gather(np.int32), orc.bfs(row, col, s)), s
+        e.sssp(s)
+        assert np.array_equal(e.gather(np.uint32), orc.sssp_dijkstra(row, col, w, s)), s
+
+
 def test_bfs_single_vertex_and_isolated_source(sg, orc, gpu):
     hg = sg.HostGraph(1, np.array([0, 0], np.uint32), np.zeros(0, np.uint32))
     (lv, st), = run_bfs(sg, hg, [0])

@@ -93,6 +93,28 @@ def test_server_admission_by_rows(sg, gpu, wgraph):
     srv.close()
 
 
+@pytest.mark.parametrize("packed_min", [1, 65])
+def test_server_batch_forms(sg, gpu, wgraph, packed_min):
+    """packed_min 1: every BFS / SSSP batch runs as one bit-packed launch;
+    65: every batch runs as single whole-GPU queries. Results equal the lone
+    entry points either way."""
+    g = sg.Graph(wgraph)
+    srv = sg.Server(g, max_rows=256, bfs_packed_min=packed_min, sssp_packed_min=packed_min)
+    srcs = sg.pick_sources(wgraph.row, 48, seed=11)
+    kinds = [sg.QUERY_BFS, sg.QUERY_SSSP] * 24
+    tickets = [srv.submit(k, int(s)) for k, s in zip(kinds, srcs)]
+    e = sg.Engine(g)
+    for t, k, s in zip(tickets, kinds, srcs):
+        srv.wait(t, 60000)
+        got = srv.result(t, k)
+        assert same(sg, k, got, alone(sg, e, k, s)), (packed_min, k, int(s))
+        srv.release(t)
+    assert srv.stats()["failed"] == 0
+    with pytest.raises(sg.SepGraphError):
+        sg.Server(g, bfs_packed_min=66)
+    srv.close()
+
+
 def test_server_replay(sg, gpu, wgraph):
     g = sg.Graph(wgraph)
     srv = sg.Server(g, max_rows=1024)
