@@ -48,7 +48,8 @@ enum {
     SG_ERR_CAPACITY = -5,    /* a fixed-capacity buffer would overflow */
     SG_ERR_TIMEOUT = -6,     /* a device-side bounded wait expired */
     SG_ERR_STATE = -7,       /* call not valid in the handle's current state */
-    SG_ERR_UNSUPPORTED = -8  /* option combination not implemented */
+    SG_ERR_UNSUPPORTED = -8, /* option combination not implemented */
+    SG_ERR_IO = -9           /* a file could not be opened, read or written */
 };
 
 /* Thread-local message describing the last failing call on this thread. */
@@ -103,6 +104,69 @@ int sg_csc_from_csr(uint32_t n, const uint32_t *row_start, const uint32_t *edge_
 /* k distinct vertices with out-degree >= 1, drawn as draw(seed, i) mod n with
  * rejection. Fails with SG_ERR_RANGE if fewer than k such vertices exist. */
 int sg_pick_sources(uint32_t n, const uint32_t *row_start, int k, uint64_t seed, uint32_t *out);
+
+/* Missing edge weights generated as the reference's drivers do with
+ * --gen_weights (include/utils/graphs/traversal.h:174-186): weight i is the
+ * i-th draw of std::uniform_int_distribution<int>(1, range) over a
+ * default-constructed std::default_random_engine, in edge order. Compiled
+ * against the same C++ standard library, the sequence is identical. */
+int sg_gen_weights_std(uint64_t m, int32_t range, uint32_t *weights);
+
+/* ------------------------------------------------------------- graph files
+ * The reference's three input formats, chosen by --format
+ * (src/utils/utils.cpp:39-58), read into a host CSR, plus the cleanups and
+ * outputs of its gtools converter (samples/tools/gtools.cpp:56-346). The
+ * result goes to sg_graph_create like any other host CSR. Readers validate
+ * what the reference trusts (row ends, vertex ids), so a malformed file is an
+ * error rather than an out-of-range access later. */
+typedef struct sg_host_csr sg_host_csr;
+enum { SG_FORMAT_GR = 0, SG_FORMAT_METIS = 1, SG_FORMAT_MARKET = 2 };
+
+/* Galois .gr version 1 (ReadGraphGR, src/utils/parser.cpp:409-505): u64 header
+ * {1, edge data size, n, m}, u64 row ends, u32 heads, 4 pad bytes when m is
+ * odd and edge data follows, u32 edge data. Edge data of a size other than 0
+ * or 4 bytes is SG_ERR_UNSUPPORTED (the reference reads 4-byte words
+ * regardless).
+ * METIS text (ReadGraph, parser.cpp:230-395): '%' comment lines, header
+ * "n m [fmt [ncon]]" with m undirected edges (every edge listed from both
+ * ends, so 2m entries), fmt digits vertex-size / vertex-weight / edge-weight,
+ * 1-based neighbour ids; vertex sizes and weights are parsed and dropped.
+ * MatrixMarket coordinate (ReadGraphMarket, parser.cpp:508-665): '%' lines,
+ * header "rows cols nnz" (n = cols), nnz 1-based "row col [weight]" lines;
+ * integer weights when the first entry has a third field; each row's entries
+ * deduplicated and sorted as a std::set (of heads, or of (head, weight)
+ * pairs when weighted); symmetric files are not expanded. */
+int sg_file_read(const char *path, int format, sg_host_csr **out);
+/* Wrap a caller CSR (copied) so it can be cleaned or written. */
+int sg_file_from_csr(uint32_t n, const uint32_t *row_start, const uint32_t *edge_dst,
+                     const uint32_t *weights, sg_host_csr **out);
+int sg_file_free(sg_host_csr *g);
+/* n; m = row_start[n]; whether edge weights are present; and the edge count
+ * the reference's reader records (header_edges): 2 x the METIS header's m,
+ * the MatrixMarket header's nnz (not lowered by the deduplication,
+ * parser.cpp:541-665), the .gr header's m. */
+int sg_file_info(const sg_host_csr *g, uint32_t *n, uint64_t *m, int *weighted, uint64_t *header_edges);
+/* Copy out row_start[n+1], edge_dst[m] and, when present and weights != NULL,
+ * weights[m]. */
+int sg_file_copy(const sg_host_csr *g, uint32_t *row_start, uint32_t *edge_dst, uint32_t *weights);
+
+/* gtools cleanups, applied in its order: self-loops dropped (--deself_cycle),
+ * duplicates dropped keeping each row's first occurrence and its weight, row
+ * order otherwise unchanged (--deduplicate), then directed to undirected
+ * (--d2ud): both directions of every edge, each row sorted by head; a pair's
+ * weight, both ways, is that of the first edge between the two vertices in
+ * row-major order. */
+enum { SG_CLEAN_SELF_LOOPS = 1, SG_CLEAN_DEDUP = 2, SG_CLEAN_TO_UNDIRECTED = 4 };
+int sg_file_clean(sg_host_csr *g, int flags);
+
+/* Write in one of the three formats, so that sg_file_read gives back the
+ * same CSR (MatrixMarket: when rows are sorted and duplicate-free; METIS: m
+ * must be even, as every edge is listed from both ends). */
+int sg_file_write(const sg_host_csr *g, const char *path, int format);
+/* gtools --output: one "src<sep>dst[<sep>weight]\n" line per edge, 0-based.
+ * gtools --out_degree: one "v<sep>out-degree\n" line per vertex. */
+int sg_file_write_edgelist(const sg_host_csr *g, const char *path, const char *separator);
+int sg_file_write_out_degree(const sg_host_csr *g, const char *path, const char *separator);
 
 /* ------------------------------------------------------------ device graph */
 typedef struct sg_graph sg_graph;

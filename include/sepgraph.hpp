@@ -156,6 +156,42 @@ struct CSRGraph {
     std::vector<uint32_t> row_start, edge_dst, edge_weights;
     bool undirected = false;
 };
+
+// --graphfile / --format loading (GetCachedGraph, src/utils/utils.cpp:39-58,
+// bound as in include/utils/graphs/traversal.h:147-158): format "gr",
+// "metis" or "market"; `clean` takes SG_CLEAN_* flags (the gtools cleanups).
+// nedges is row_start[nnodes], the count the engine traverses.
+inline CSRGraph ReadGraph(const std::string &path, const std::string &format, int clean = 0,
+                          bool undirected = false)
+{
+    int fmt = format == "gr" ? SG_FORMAT_GR : format == "metis" ? SG_FORMAT_METIS : format == "market" ? SG_FORMAT_MARKET : -1;
+    if (fmt < 0) throw Error(SG_ERR_INVALID, "sepgraph: unsupported graph format " + format);
+    sg_host_csr *h = nullptr;
+    check(sg_file_read(path.c_str(), fmt, &h));
+    struct Free {
+        sg_host_csr *h;
+        ~Free() { sg_file_free(h); }
+    } guard{h};
+    if (clean) check(sg_file_clean(h, clean));
+    CSRGraph g;
+    int weighted = 0;
+    check(sg_file_info(h, &g.nnodes, &g.nedges, &weighted, nullptr));
+    g.row_start.resize((size_t)g.nnodes + 1);
+    g.edge_dst.resize(g.nedges);
+    if (weighted) g.edge_weights.resize(g.nedges);
+    check(sg_file_copy(h, g.row_start.data(), g.edge_dst.data(), weighted ? g.edge_weights.data() : nullptr));
+    g.undirected = undirected;
+    return g;
+}
+
+// --gen_weights: fill missing weights with the reference drivers' sequence
+// (include/utils/graphs/traversal.h:174-186); a graph with weights is kept.
+inline void GenWeights(CSRGraph &g, int32_t weight_range = 100)
+{
+    if (!g.edge_weights.empty()) return;
+    g.edge_weights.resize(g.nedges);
+    check(sg_gen_weights_std(g.nedges, weight_range, g.edge_weights.data()));
+}
 }  // namespace graphs
 
 namespace apps {

@@ -30,10 +30,13 @@ ERRORS = {
     -6: "SG_ERR_TIMEOUT",
     -7: "SG_ERR_STATE",
     -8: "SG_ERR_UNSUPPORTED",
+    -9: "SG_ERR_IO",
 }
 CSR_SYMMETRIZE, CSR_DROP_SELF_LOOPS, CSR_DEDUP = 1, 2, 4
 PR_AUTO, PR_PULL_TD, PR_PUSH_DD = 0, 1, 2
 GRAPH_UNDIRECTED, GRAPH_RELABEL, GRAPH_NO_RELABEL = 1, 2, 4
+FORMATS = {"gr": 0, "metis": 1, "market": 2}
+CLEAN_SELF_LOOPS, CLEAN_DEDUP, CLEAN_TO_UNDIRECTED = 1, 2, 4
 
 VARIANTS = {
     "auto": -1,
@@ -129,6 +132,16 @@ def lib() -> C.CDLL:
             "sg_gen_weights": ([C.c_uint32, _P, _P, C.c_uint32, C.c_uint32, C.c_uint64, C.c_int, _P], C.c_int),
             "sg_csc_from_csr": ([C.c_uint32, _P, _P, _P, _P, _P, _P], C.c_int),
             "sg_pick_sources": ([C.c_uint32, _P, C.c_int, C.c_uint64, _P], C.c_int),
+            "sg_gen_weights_std": ([C.c_uint64, C.c_int32, _P], C.c_int),
+            "sg_file_read": ([C.c_char_p, C.c_int, C.POINTER(_P)], C.c_int),
+            "sg_file_from_csr": ([C.c_uint32, _P, _P, _P, C.POINTER(_P)], C.c_int),
+            "sg_file_free": ([_P], C.c_int),
+            "sg_file_info": ([_P, _U32P, C.POINTER(C.c_uint64), C.POINTER(C.c_int), C.POINTER(C.c_uint64)], C.c_int),
+            "sg_file_copy": ([_P, _P, _P, _P], C.c_int),
+            "sg_file_clean": ([_P, C.c_int], C.c_int),
+            "sg_file_write": ([_P, C.c_char_p, C.c_int], C.c_int),
+            "sg_file_write_edgelist": ([_P, C.c_char_p, C.c_char_p], C.c_int),
+            "sg_file_write_out_degree": ([_P, C.c_char_p, C.c_char_p], C.c_int),
             "sg_graph_create": ([C.c_uint32, C.c_uint64, _P, _P, _P, _P, _P, _P, C.c_int, C.POINTER(_P)], C.c_int),
             "sg_graph_create_ex": ([C.c_uint32, C.c_uint64, _P, _P, _P, _P, _P, _P, C.c_uint32, C.POINTER(_P)], C.c_int),
             "sg_graph_destroy": ([_P], C.c_int),
@@ -247,6 +260,8 @@ class HostGraph:
     w: np.ndarray | None = None
     undirected: bool = False
     name: str = ""
+    # edge count a reference graph-file reader records (None: not from a file)
+    header_edges: int | None = None
 
     @property
     def m(self) -> int:
@@ -273,6 +288,88 @@ def grid_graph(rows=2048, cols=2048, p_delete=0.1, seed=1, weights=(1, 1000), we
     row, col = build_csr(n, s, d, symmetrize=True)
     w = None if weights is None else gen_weights(row, col, weights[0], weights[1], seed=weight_seed, symmetric=True)
     return HostGraph(n, row, col, w, undirected=True, name=f"grid{rows}x{cols}-p{p_delete}")
+
+
+# ----------------------------------------------------------------- graph files
+def gen_weights_std(m: int, weight_range: int = 100) -> np.ndarray:
+    """The reference drivers' --gen_weights sequence (traversal.h:174-186)."""
+    w = np.empty(max(m, 1), np.uint32)
+    _check(lib().sg_gen_weights_std(m, weight_range, _ptr(w)))
+    return w[:m]
+
+
+def _fmt(fmt: str) -> int:
+    if fmt not in FORMATS:
+        raise ValueError(f"unknown graph format {fmt!r} (gr, metis, market)")
+    return FORMATS[fmt]
+
+
+def _host_csr_out(h) -> tuple[np.ndarray, np.ndarray, np.ndarray | None, int]:
+    L = lib()
+    n, m, wt, hdr = C.c_uint32(), C.c_uint64(), C.c_int(), C.c_uint64()
+    _check(L.sg_file_info(h, C.byref(n), C.byref(m), C.byref(wt), C.byref(hdr)))
+    row = np.empty(n.value + 1, np.uint32)
+    col = np.empty(max(m.value, 1), np.uint32)
+    w = np.empty(max(m.value, 1), np.uint32) if wt.value else None
+    _check(L.sg_file_copy(h, _ptr(row), _ptr(col), _ptr(w)))
+    return row, col[: m.value], (None if w is None else w[: m.value]), hdr.value
+
+
+def _host_csr_in(hg: HostGraph):
+    h = _P()
+    _check(lib().sg_file_from_csr(hg.n, _ptr(_u32(hg.row)), _ptr(_u32(hg.col)),
+                                  _ptr(None if hg.w is None else _u32(hg.w)), C.byref(h)))
+    return h
+
+
+def read_graph(path: str, fmt: str = "gr", clean: int = 0, undirected=False) -> HostGraph:
+    """Load a graph file as the reference's --graphfile/--format do
+    (src/utils/utils.cpp:39-58), then apply the gtools cleanups in `clean`
+    (CLEAN_* flags). `header_edges` records the edge count the reference's
+    reader keeps."""
+    L = lib()
+    h = _P()
+    _check(L.sg_file_read(os.fsencode(path), _fmt(fmt), C.byref(h)))
+    try:
+        if clean:
+            _check(L.sg_file_clean(h, clean))
+        row, col, w, hdr = _host_csr_out(h)
+    finally:
+        L.sg_file_free(h)
+    return HostGraph(len(row) - 1, row, col, w, undirected=undirected, name=os.path.basename(path), header_edges=hdr)
+
+
+def clean_graph(hg: HostGraph, clean: int) -> HostGraph:
+    """gtools cleanups (samples/tools/gtools.cpp:56-327) on a host CSR."""
+    L = lib()
+    h = _host_csr_in(hg)
+    try:
+        _check(L.sg_file_clean(h, clean))
+        row, col, w, _ = _host_csr_out(h)
+    finally:
+        L.sg_file_free(h)
+    return HostGraph(len(row) - 1, row, col, w, undirected=hg.undirected, name=hg.name)
+
+
+def write_graph(hg: HostGraph, path: str, fmt: str = "gr") -> None:
+    L = lib()
+    h = _host_csr_in(hg)
+    try:
+        _check(L.sg_file_write(h, os.fsencode(path), _fmt(fmt)))
+    finally:
+        L.sg_file_free(h)
+
+
+def write_edgelist(hg: HostGraph, path: str, separator: str = "\t", out_degree: str | None = None) -> None:
+    """gtools --output (edge list) and, if given, --out_degree files."""
+    L = lib()
+    h = _host_csr_in(hg)
+    try:
+        _check(L.sg_file_write_edgelist(h, os.fsencode(path), separator.encode()))
+        if out_degree:
+            _check(L.sg_file_write_out_degree(h, os.fsencode(out_degree), separator.encode()))
+    finally:
+        L.sg_file_free(h)
 
 
 # --------------------------------------------------------------- device side

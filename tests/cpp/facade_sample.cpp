@@ -1,10 +1,9 @@
 // A hybrid_* style driver written against the C++ facade (include/sepgraph.hpp):
-// loads a binary CSR, runs BFS, SSSP, PageRank and single-source BC the way
-// samples/hybrid_{bfs,sssp,pr,bc}/*.cu drive the reference Engine, and writes
-// each result vector to <out>.<app>.bin.
+// loads a graph file the way --graphfile/--format do, runs BFS, SSSP, PageRank
+// and single-source BC the way samples/hybrid_{bfs,sssp,pr,bc}/*.cu drive the
+// reference Engine, and writes each result vector to <out>.<app>.bin.
 //
-// graph file: u32 n, u64 m, u32 undirected, u32 weighted, u32 row_start[n+1],
-//             u32 edge_dst[m], u32 weights[m] (when weighted)
+// usage: facade_sample graphfile format out_prefix source undirected(0|1) [gen_weight_range]
 #include <cstdio>
 #include <cstdlib>
 #include <string>
@@ -24,29 +23,20 @@ static void dump(const std::string &path, const std::vector<T> &v)
 
 int main(int argc, char **argv)
 {
-    if (argc != 4) {
-        std::fprintf(stderr, "usage: %s graph.bin out_prefix source\n", argv[0]);
+    if (argc != 6 && argc != 7) {
+        std::fprintf(stderr, "usage: %s graphfile format out_prefix source undirected [gen_weight_range]\n", argv[0]);
         return 2;
     }
     graphs::CSRGraph g;
-    FILE *f = std::fopen(argv[1], "rb");
-    if (!f) return 2;
-    uint32_t und = 0, weighted = 0;
-    bool ok = std::fread(&g.nnodes, 4, 1, f) == 1 && std::fread(&g.nedges, 8, 1, f) == 1 &&
-              std::fread(&und, 4, 1, f) == 1 && std::fread(&weighted, 4, 1, f) == 1;
-    g.undirected = und != 0;
-    g.row_start.resize(g.nnodes + 1);
-    g.edge_dst.resize(g.nedges);
-    ok = ok && std::fread(g.row_start.data(), 4, g.nnodes + 1, f) == g.nnodes + 1 &&
-         std::fread(g.edge_dst.data(), 4, g.nedges, f) == g.nedges;
-    if (weighted) {
-        g.edge_weights.resize(g.nedges);
-        ok = ok && std::fread(g.edge_weights.data(), 4, g.nedges, f) == g.nedges;
+    try {
+        g = graphs::ReadGraph(argv[1], argv[2], 0, std::atoi(argv[5]) != 0);
+        if (argc == 7) graphs::GenWeights(g, std::atoi(argv[6]));
+    } catch (const Error &e) {
+        std::fprintf(stderr, "%s (code %d)\n", e.what(), e.code());
+        return 2;
     }
-    std::fclose(f);
-    if (!ok) return 2;
-    const uint32_t source = (uint32_t)std::strtoul(argv[3], nullptr, 10);
-    const std::string out = argv[2];
+    const uint32_t source = (uint32_t)std::strtoul(argv[4], nullptr, 10);
+    const std::string out = argv[3];
     try {
         {
             typedef engine::Engine<int32_t, int32_t, graphs::NoWeight, apps::BFS, uint32_t> HybridEngine;
