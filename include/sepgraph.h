@@ -290,16 +290,31 @@ int sg_sssp(sg_engine *e, uint32_t source);
 
 /* PageRank schedules. SG_PR_PULL_TD is the reference's SYNC_PULL_TD
  * (topology-driven pull, RunSyncPullTD, include/framework/algo_variants.cuh:753-823);
- * SG_PR_PUSH_DD is its ASYNC_PUSH_DD residual push (hybrid_pr.cu:66-110). */
-enum { SG_PR_AUTO = 0, SG_PR_PULL_TD = 1, SG_PR_PUSH_DD = 2 };
+ * SG_PR_PUSH_DD is its ASYNC_PUSH_DD residual push (hybrid_pr.cu:66-110),
+ * each round pushing or pulling by the alpha rule on the active out-degree.
+ * SG_PR_ITERATIVE is the reference's ITERATIVE_SCHEME on the same residual
+ * system (PolicyDecisionMaker, include/framework/hybrid_policy.h:130-264):
+ * rounds 0, 1, 2 run SYNC_PULL_TD, ASYNC_PUSH_TD, ASYNC_PUSH_DD (the order of
+ * its variant set); every later round runs the variant with the least
+ * predicted time -- a topology-driven variant's last measured round time, or
+ * for ASYNC_PUSH_DD the time per input edge of its last round times the
+ * out-degree sum of the current worklist (ties to the earlier variant).
+ * Round times are taken on the device between grid barriers. A
+ * topology-driven round finds its active vertices by scanning every residual
+ * (RebuildArrayWorklist), a data-driven one from the queue the previous round
+ * filled. options.force_variant SYNC_PULL_TD / ASYNC_PUSH_TD / ASYNC_PUSH_DD
+ * runs that variant every round (ForceVariant); other forced variants are
+ * SG_ERR_UNSUPPORTED. The trace rows give each round's variant, active
+ * vertices and out-degree, and the barrier clock at the round's end. */
+enum { SG_PR_AUTO = 0, SG_PR_PULL_TD = 1, SG_PR_PUSH_DD = 2, SG_PR_ITERATIVE = 3 };
 
 /* PageRank with damping `damping`, uniform teleport and uniform redistribution
  * of dangling mass, ranks summing to 1 (FP32 per vertex, the engine's value
  * array). SG_PR_PULL_TD iterates x' = (1-d)/n + d (P^T x + dangling(x)/n)
  * from x = 1/n and stops after the first iteration whose L1 change (reduced in
  * FP64 on the device) is below tol, or after max_iter iterations.
- * SG_PR_PUSH_DD pushes residuals until the outstanding residual mass is below
- * tol * (1 - d) and normalises. Per-vertex sums are accumulated in FP64.
+ * SG_PR_PUSH_DD and SG_PR_ITERATIVE push residuals until every residual is
+ * below tol * (1 - d) / n and normalise. Per-vertex sums are accumulated in FP64.
  * sg_stats.iterations / .residual report the run. Asynchronous.
  * The reference's PR app solves the unnormalised system without dangling
  * redistribution (samples/hybrid_pr/hybrid_pr.cu:26-125); for uniform

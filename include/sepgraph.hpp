@@ -246,7 +246,7 @@ public:
             check(sg_sssp(e_, (uint32_t)std::get<0>(init_)));
         } else if constexpr (std::is_same<TApp, apps::PageRank>::value) {
             check(sg_pagerank(e_, 0.85, (double)std::get<0>(init_), 0,
-                              algo_type_ == policy::AlgoType::ITERATIVE_SCHEME ? SG_PR_PUSH_DD : SG_PR_PULL_TD));
+                              algo_type_ == policy::AlgoType::ITERATIVE_SCHEME ? SG_PR_ITERATIVE : SG_PR_PULL_TD));
         } else {
             const uint32_t s = (uint32_t)std::get<0>(init_);
             check(sg_bc_dependency(e_, &s, 1, 1));

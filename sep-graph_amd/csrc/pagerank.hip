@@ -304,6 +304,45 @@ struct PrDdArgs {
     int32_t force;     // -1 auto, 0 push, 1 pull
     uint32_t max_rounds;
     uint32_t probe;
+    uint32_t iterative;  // rounds chosen by the ITERATIVE scheme's predicted times
+    int32_t it_force;    // ITERATIVE: -1, or the variant every round runs (kItPullTd...)
+    float ticks_per_us;  // wall clock rate
+};
+
+// The ITERATIVE scheme's variant set in its std::set order
+// (include/framework/hybrid_policy.h:172-186; AlgoVariant::operator< compares
+// model, message passing, scheduling with SYNC < ASYNC, PUSH < PULL, TD < DD).
+enum { kItPullTd = 0, kItPushTd = 1, kItPushDd = 2 };
+
+// Replicated in every block: each computes the same decisions from the same
+// barrier stamps and counters (PolicyDecisionMaker::GetNextPolicy and
+// PredicateRunningTime, hybrid_policy.h:130-169,246-264; times are float as
+// in its RunningInfo).
+struct IterativePolicy {
+    float time[3] = {0.f, 0.f, 0.f};  // last measured round time of each variant (us)
+    float time_per_work = 0.f;
+    float last_in_wl = 0.f;  // input workload (out-degree sum) of the last round
+    int last = -1;
+
+    // Record the round just finished (variant `v`, `us` long, `in_wl` input edges).
+    __device__ void record(int v, float us, float in_wl)
+    {
+        time[v] = us;
+        last = v;
+        last_in_wl = in_wl;
+    }
+
+    // Variant for round k, `out_wl` = out-degree sum of the current worklist.
+    __device__ int next(uint32_t k, float out_wl)
+    {
+        if (k < 3) return (int)k;  // inspection
+        if (last == kItPushDd) time_per_work = time[kItPushDd] / last_in_wl;
+        const float pred[3] = {time[kItPullTd], time[kItPushTd], time_per_work * out_wl};
+        int best = 0;
+        for (int v = 1; v < 3; v++)
+            if (pred[v] < pred[best]) best = v;  // std::min_element: first minimum
+        return best;
+    }
 };
 
 struct DdRound {
@@ -311,6 +350,8 @@ struct DdRound {
     double theta;
     Stager S;
     unsigned long long count;
+    unsigned long long qscout;  // out-degree sum of queued vertices (ITERATIVE)
+    bool qdeg;
 };
 
 // Add `add` to r[v] (atomically when `atomic`), queueing v if this addition
@@ -328,6 +369,7 @@ __device__ __forceinline__ void dd_accumulate(DdRound &R, bool act, uint32_t v, 
         }
         crossed = old < R.theta && old + add >= R.theta;
         R.count += crossed;
+        if (crossed && R.qdeg) R.qscout += R.a->g.row[v + 1] - R.a->g.row[v];
     }
     stage(R.S, crossed, v);
 }
@@ -401,17 +443,40 @@ __global__ __launch_bounds__(BS) void k_pr_dd(PrDdArgs a)
     if (blockIdx.x == 0) {
         for (int s = 0; s < 5; s++) clear_ctr(&a.ctr[s]);
     }
-    bool ok = grid_sync(a.bar, xcc, gen, a.abort);
+    // stamped barriers: the block completing each one records the wall clock,
+    // so every block derives the same round times for the ITERATIVE scheme
+    bool ok = grid_sync<true>(a.bar, xcc, gen, a.abort);
+    __shared__ unsigned long long s_stamp;
+    IterativePolicy ipol;
+    unsigned long long stamp0 = 0, stamp_prev = 0;
+    float in_wl_prev = 0.f;
+    int v_prev = -1;
 
     uint32_t k = 0, pushes = 0, pulls = 0;
     unsigned long long ptrace = 0;
     while (ok) {
         if (a.max_rounds && k >= a.max_rounds) break;
         // ---- activation: move residual >= theta into y, set p and the bit
+        if (threadIdx.x == 0) s_stamp = barrier_stamp(a.bar);
         read_counts(&q_ctr[k % 3], s_lvl);
         __syncthreads();
         const RoundCounts qc = unpack_counts(s_lvl, a.qcap);
+        const unsigned long long stamp = s_stamp;
         __syncthreads();
+        // ITERATIVE: time the round that just ended, then pick this round's
+        // variant (topology-driven ones rebuild the active set by a scan)
+        int var = -1;
+        if (a.iterative) {
+            if (k == 0) {
+                stamp0 = stamp;
+            } else {
+                const float us = (float)((double)(stamp - stamp_prev) / (double)a.ticks_per_us);
+                ipol.record(v_prev, us, in_wl_prev);
+                if (gtid == 0 && k - 1 < kTraceRows) a.trace[k - 1].ticks = stamp - stamp0;
+            }
+            stamp_prev = stamp;
+            var = a.it_force >= 0 ? a.it_force : ipol.next(k, (float)qc.scout);
+        }
         uint64_t *ab = a.abit + (size_t)(k & 1) * W;
         unsigned long long scout = 0, cnt = 0;
         auto activate = [&](uint32_t u, double ru) {
@@ -421,7 +486,7 @@ __global__ __launch_bounds__(BS) void k_pr_dd(PrDdArgs a)
             scout += dout;
             cnt++;
         };
-        if (k == 0 || qc.overflow) {
+        if (k == 0 || qc.overflow || var == kItPullTd || var == kItPushTd) {
             for (uint32_t w = gwave; w < W; w += nwaves) {
                 const uint32_t v = w * 64 + ln;
                 bool act = false;
@@ -462,7 +527,8 @@ __global__ __launch_bounds__(BS) void k_pr_dd(PrDdArgs a)
         __syncthreads();
         if (ac.count == 0) break;  // every residual is below theta
         bool pull;
-        if (a.force >= 0) pull = a.force == 1;
+        if (var >= 0) pull = var == kItPullTd;
+        else if (a.force >= 0) pull = a.force == 1;
         else pull = (double)ac.scout > a.total_wl / a.alpha;
         if (pull) {
             pulls++;
@@ -470,7 +536,22 @@ __global__ __launch_bounds__(BS) void k_pr_dd(PrDdArgs a)
         } else {
             pushes++;
         }
-        if (gtid == 0) write_trace(a.trace, k, pull, ac.count, ac.scout, t_start);
+        if (var >= 0) {
+            // the reference's input workload: edges for a topology-driven
+            // round, the worklist's out-degree sum for a data-driven one
+            in_wl_prev = var == kItPushDd ? (float)ac.scout : (float)g.m;
+            v_prev = var;
+            if (gtid == 0 && k < kTraceRows) {
+                DevTrace t{};
+                t.round = k;
+                t.variant = var == kItPullTd ? SG_SYNC_PULL_TD : var == kItPushTd ? SG_ASYNC_PUSH_TD : SG_ASYNC_PUSH_DD;
+                t.in_nodes = (uint32_t)ac.count;
+                t.in_edges = ac.scout;
+                a.trace[k] = t;  // ticks filled in when the round's end is known
+            }
+        } else if (gtid == 0) {
+            write_trace(a.trace, k, pull, ac.count, ac.scout, t_start);
+        }
         if (blockIdx.x == 0) {
             clear_ctr(&act_ctr[(k + 1) & 1]);
             clear_ctr(&q_ctr[(k + 2) % 3]);
@@ -483,6 +564,8 @@ __global__ __launch_bounds__(BS) void k_pr_dd(PrDdArgs a)
         R.theta = theta;
         R.S = Stager{a.queue + (size_t)((k + 1) & 1) * a.qcap, a.qcap, &q_ctr[(k + 1) % 3], s_q, &s_qn, kLdsQ};
         R.count = 0;
+        R.qscout = 0;
+        R.qdeg = a.iterative != 0;
         if (!pull) {
             for (uint32_t c0 = 0; c0 < g.nhuge; c0 += nwaves * kWave) {
                 const uint32_t c = c0 + (uint32_t)ln * nwaves + gwave;
@@ -543,9 +626,9 @@ __global__ __launch_bounds__(BS) void k_pr_dd(PrDdArgs a)
             }
         }
         flush_stage(R.S, s_flush);
-        publish_counts<WPB>(&q_ctr[(k + 1) % 3], 0ull, R.count, false, s_red);
+        publish_counts<WPB>(&q_ctr[(k + 1) % 3], R.qscout, R.count, false, s_red);
         k++;
-        ok = grid_sync(a.bar, xcc, gen, a.abort);
+        ok = grid_sync<true>(a.bar, xcc, gen, a.abort);
     }
 
     // ---- absorb what residual is left, then normalise
@@ -628,7 +711,20 @@ extern "C" int sg_pagerank(sg_engine *e, double damping, double tol, int32_t max
                    ? SG_PR_PUSH_DD
                    : SG_PR_PULL_TD;
     }
-    if (mode != SG_PR_PULL_TD && mode != SG_PR_PUSH_DD) SG_FAIL(SG_ERR_INVALID, "sg_pagerank: unknown mode");
+    if (mode != SG_PR_PULL_TD && mode != SG_PR_PUSH_DD && mode != SG_PR_ITERATIVE)
+        SG_FAIL(SG_ERR_INVALID, "sg_pagerank: unknown mode");
+    int32_t it_force = -1;
+    if (mode == SG_PR_ITERATIVE) {
+        switch (o.force_variant) {
+        case SG_VARIANT_AUTO: break;
+        case SG_SYNC_PULL_TD: it_force = kItPullTd; break;
+        case SG_ASYNC_PUSH_TD: it_force = kItPushTd; break;
+        case SG_ASYNC_PUSH_DD: it_force = kItPushDd; break;
+        default:
+            SG_FAIL(SG_ERR_UNSUPPORTED,
+                    "sg_pagerank: the ITERATIVE scheme runs SYNC_PULL_TD, ASYNC_PUSH_TD or ASYNC_PUSH_DD only");
+        }
+    }
     const DevGraph &d = e->g->dev;
     int rc = engine_ensure_bfs_state(e);
     if (rc) return rc;
@@ -664,7 +760,7 @@ extern "C" int sg_pagerank(sg_engine *e, double damping, double tol, int32_t max
         default: rc = launch_persistent<PrArgs, k_pr_pull<1024>>(e, a, 1024); break;
         }
     } else {
-        const int force = forced_direction(o);
+        const int force = mode == SG_PR_ITERATIVE ? -1 : forced_direction(o);
         if (force < -1) SG_FAIL(SG_ERR_UNSUPPORTED, "sg_pagerank: data-driven mode takes push/pull DD variants only");
         // y | r | p (n doubles each) | partials
         const uint64_t off_part = 3 * n * 8;
@@ -691,6 +787,9 @@ extern "C" int sg_pagerank(sg_engine *e, double damping, double tol, int32_t max
         a.total_wl = (double)d.m * o.edge_factor;
         a.force = force;
         a.max_rounds = max_iter ? (uint32_t)max_iter : 100000u;
+        a.iterative = mode == SG_PR_ITERATIVE;
+        a.it_force = it_force;
+        a.ticks_per_us = (float)(e->wall_khz / 1000.0);
         switch (bs) {
         case 256: rc = launch_persistent<PrDdArgs, k_pr_dd<256>>(e, a, 256); break;
         case 512: rc = launch_persistent<PrDdArgs, k_pr_dd<512>>(e, a, 512); break;

@@ -95,6 +95,7 @@ constexpr int kBarWords = kBarLine * (2 + 3 * kMaxXcd);
 
 // Word offsets inside the barrier area.
 constexpr int kBarGen = 0;
+constexpr int kBarStamp = 2;  // 64-bit: wall clock when a stamped barrier completed
 constexpr int kBarTop = kBarLine;
 __host__ __device__ constexpr int bar_arrive(int x) { return kBarLine * (2 + x); }
 __host__ __device__ constexpr int bar_release(int x) { return kBarLine * (2 + kMaxXcd + x); }
@@ -190,6 +191,9 @@ __device__ __forceinline__ uint32_t grid_gen(const GridBarrier &gb) { return pol
 // Returns false if the wait timed out or another block aborted; the caller
 // must then leave its loop. `gen` is this block's generation (grid_gen at
 // kernel start), advanced on success; only thread 0's copy is used.
+// kStamp: the block that completes the barrier also records the wall clock
+// (barrier_stamp), so every block can time the grid's rounds identically.
+template <bool kStamp = false>
 __device__ __forceinline__ bool grid_sync(const GridBarrier &gb, uint32_t xcc, uint32_t &gen, uint32_t *abort)
 {
     __shared__ uint32_t s_ok;
@@ -213,6 +217,9 @@ __device__ __forceinline__ bool grid_sync(const GridBarrier &gb, uint32_t xcc, u
                 // the reset must complete before the generation flips, or a
                 // leader's next-barrier arrival could be overwritten by it
                 flag_store(w + kBarTop, 0u);
+                if (kStamp)
+                    __hip_atomic_store((uint64_t *)(w + kBarStamp), (uint64_t)wall_clock64(), __ATOMIC_RELAXED,
+                                       __HIP_MEMORY_SCOPE_SYSTEM);
                 __builtin_amdgcn_s_waitcnt(0);
                 flag_store(w + kBarGen, gen0 + 1);
             }
@@ -226,6 +233,12 @@ __device__ __forceinline__ bool grid_sync(const GridBarrier &gb, uint32_t xcc, u
     }
     __syncthreads();
     return s_ok != 0;
+}
+
+// Wall clock recorded by the last stamped barrier (read after it).
+__device__ __forceinline__ uint64_t barrier_stamp(const GridBarrier &gb)
+{
+    return __hip_atomic_load((const uint64_t *)(gb.words + kBarStamp), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
 }
 
 }  // namespace sg

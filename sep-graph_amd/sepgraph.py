@@ -33,7 +33,7 @@ ERRORS = {
     -9: "SG_ERR_IO",
 }
 CSR_SYMMETRIZE, CSR_DROP_SELF_LOOPS, CSR_DEDUP = 1, 2, 4
-PR_AUTO, PR_PULL_TD, PR_PUSH_DD = 0, 1, 2
+PR_AUTO, PR_PULL_TD, PR_PUSH_DD, PR_ITERATIVE = 0, 1, 2, 3
 GRAPH_UNDIRECTED, GRAPH_RELABEL, GRAPH_NO_RELABEL = 1, 2, 4
 FORMATS = {"gr": 0, "metis": 1, "market": 2}
 CLEAN_SELF_LOOPS, CLEAN_DEDUP, CLEAN_TO_UNDIRECTED = 1, 2, 4
@@ -438,7 +438,8 @@ class Engine:
         _check(lib().sg_sssp(self.h, int(source)))
 
     def pagerank(self, damping: float = 0.85, tol: float = 1e-6, max_iter: int = 0, mode: int = 0):
-        """Normalised PageRank into the value array (FP32); mode PR_PULL_TD or PR_PUSH_DD."""
+        """Normalised PageRank into the value array (FP32); mode PR_PULL_TD,
+        PR_PUSH_DD or PR_ITERATIVE (the reference's ITERATIVE_SCHEME)."""
         _check(lib().sg_pagerank(self.h, float(damping), float(tol), int(max_iter), int(mode)))
 
     def bc(self, sources, batch: int = 64):

@@ -177,3 +177,87 @@ def test_pr_rmat23_directed_full_size(sg, gpu):
     assert np.abs(power_step(hg.row, hg.col, xd) - xd.astype(np.float64)).sum() < 1e-6
     # the pull result is within its stopping error of the fixed point
     assert rel_l1(x, xd.astype(np.float64)) < 1e-5
+
+
+# ---------------------------------------------------- ITERATIVE scheme
+ITER_ORDER = ["sync_pull_td", "async_push_td", "async_push_dd"]
+
+
+def iterative_decisions_ok(trace):
+    """Re-derive every round's variant from the trace with the reference's
+    rule (hybrid_policy.h:130-264): rounds 0..2 inspect the variant set in
+    order; later rounds take the first minimum of the predicted times -- the
+    last measured time of each topology-driven variant, and for
+    ASYNC_PUSH_DD its last round's time per input edge times the current
+    worklist's out-degree sum (the next round's input edges). Times are the
+    barrier-clock round durations in the trace. A choice that differs from
+    the restatement must be a tie within float rounding."""
+    rows = [r for r in trace if r["variant"] in ITER_ORDER]
+    t_prev = 0.0
+    times, last, last_in, tpw = {}, None, 0.0, 0.0
+    for k, r in enumerate(rows):
+        if k < 3:
+            assert r["variant"] == ITER_ORDER[k], (k, r)
+        else:
+            if last == "async_push_dd":
+                tpw = np.float32(times[last]) / np.float32(last_in)
+            pred = [np.float32(times["sync_pull_td"]), np.float32(times["async_push_td"]),
+                    np.float32(tpw) * np.float32(r["in_edges"])]
+            best = int(np.argmin(pred))
+            got = ITER_ORDER.index(r["variant"])
+            if got != best:
+                assert abs(pred[got] - pred[best]) <= 1e-5 * max(abs(pred[best]), 1e-30), (k, pred, r)
+        dur = r["time_us"] - t_prev
+        t_prev = r["time_us"]
+        assert dur > 0, (k, r)
+        times[r["variant"]] = dur
+        last = r["variant"]
+        last_in = r["in_edges"] if last == "async_push_dd" else None
+    return rows
+
+
+@pytest.mark.parametrize("name", sorted(SMALL))
+def test_pr_iterative_small(sg, orc, gpu, name):
+    row, col = SMALL[name]()
+    hg = sg.HostGraph(len(row) - 1, row, col, None, undirected=False)
+    x, st, e = run_pr(sg, hg, sg.PR_ITERATIVE, tol=1e-9)
+    ref, _, _ = orc.pagerank(hg.row, hg.col, D, 1e-13, 100000)
+    assert rel_l1(x, ref) <= 1e-6, (name, st)
+    rows = e.trace(4096)
+    for k, r in enumerate(rows[: min(3, st["rounds"])]):
+        assert r["variant"] == ITER_ORDER[k]
+
+
+@pytest.mark.parametrize("directed", [True, False])
+def test_pr_iterative_rmat18_policy(sg, orc, gpu, directed):
+    """Each round's variant follows the ITERATIVE rule given the round times
+    the device measured, and the ranks match the FP64 oracle."""
+    hg = sg.rmat_graph(18, 16, symmetrize=not directed, seed=2)
+    x, st, e = run_pr(sg, hg, sg.PR_ITERATIVE, tol=1e-8)
+    ref, _, _ = orc.pagerank(hg.row, hg.col, D, 1e-13, 100000)
+    assert rel_l1(x, ref) <= 1e-6, st
+    rows = iterative_decisions_ok(e.trace(4096))
+    assert len(rows) == st["rounds"] >= 4
+    assert st["pull_rounds"] == sum(r["variant"] == "sync_pull_td" for r in rows)
+    # identical inputs, identical results whatever the timings chose
+    e.pagerank(D, 1e-8, 0, sg.PR_ITERATIVE)
+    assert rel_l1(e.gather(np.float32), ref) <= 1e-6
+
+
+@pytest.mark.parametrize("variant", ITER_ORDER)
+def test_pr_iterative_forced(sg, orc, gpu, variant):
+    hg = sg.rmat_graph(16, 16, symmetrize=False, seed=1)
+    x, st, e = run_pr(sg, hg, sg.PR_ITERATIVE, tol=1e-8, force_variant=variant)
+    ref, _, _ = orc.pagerank(hg.row, hg.col, D, 1e-13, 100000)
+    assert rel_l1(x, ref) <= 1e-6, st
+    rows = [r for r in e.trace(4096) if r["variant"] != "auto"]  # the closing row has no variant
+    assert len(rows) == st["rounds"] and all(r["variant"] == variant for r in rows)
+
+
+def test_pr_iterative_rejects_other_variants(sg, gpu):
+    row, col = G.random_graph(500, 3000, 3)
+    hg = sg.HostGraph(len(row) - 1, row, col, None)
+    e = sg.Engine(sg.Graph(hg), force_variant="async_pull_dd")
+    with pytest.raises(sg.SepGraphError) as ei:
+        e.pagerank(D, 1e-6, 0, sg.PR_ITERATIVE)
+    assert ei.value.code == -8
