@@ -62,6 +62,7 @@ struct BfsArgs {
     int32_t *level;
     const uint32_t *perm;
     uint64_t *vis;    // W words: visited filter (set bit => visited or in-degree 0)
+    uint64_t *vis_idle;  // the other filter, reset here for the next query
     uint64_t *fbit;   // 3 * W words: frontier L's bitmap in slot L % 3
     uint32_t *queue;  // 3 * qcap entries
     uint32_t qcap;
@@ -77,7 +78,12 @@ struct BfsArgs {
     uint32_t total_workload0;
     uint32_t max_rounds;
     uint32_t probe;   // launch is a residency probe of the grid
+    uint32_t full_init;  // bitmaps and counters not known to be clean
+    uint32_t launch_id;  // value of the round-0 release flag (never 0)
 };
+
+// Largest source out-degree whose expansion (round 0) block 0 does alone.
+constexpr uint32_t kLocalRound0 = 4096;
 
 __device__ __forceinline__ uint32_t caller_id(const BfsArgs &a, uint32_t v) { return a.perm ? a.perm[v] : v; }
 
@@ -89,6 +95,7 @@ struct Round {
     Stager S;
     unsigned long long scout, count;  // this lane's share of the next frontier
     bool huge;                        // this lane discovered a vertex above kHugeDegree
+    bool levels = true;               // write discovered levels now (else the next round does)
 };
 
 __device__ __forceinline__ bool in_frontier(const Round &R, uint32_t u) { return test_bit(R.fcur, u); }
@@ -107,7 +114,7 @@ __device__ __forceinline__ void push_visit(Round &R, bool act, uint32_t v)
     const bool cand = act && !test_bit(a.vis, v);
     const bool found = wave_claim_bits(a.vis, cand, v, R.fnext);
     if (found) {
-        a.level[caller_id(a, v)] = R.L + 1;
+        if (R.levels) a.level[caller_id(a, v)] = R.L + 1;
         const uint32_t dv = a.g.row[v + 1] - a.g.row[v];
         R.scout += dv;
         R.count++;
@@ -264,42 +271,90 @@ __global__ SG_BFS_BOUNDS(BS) void k_bfs(BfsArgs a)
     __shared__ uint32_t s_q[kLdsQ];
     __shared__ uint32_t s_qn, s_flush[2];
 
-    // ---- prologue: reset per-query state, seed the source (one barrier)
+    // ---- prologue: reset the levels (caller order) and seed the source.
+    // The previous query left the frontier bitmaps and counters zeroed, and
+    // visited filters alternate between two buffers (the idle one is reset
+    // here for the next query), so with a narrow source block 0 expands it
+    // (round 0) while the other blocks reset, and one barrier ends both.
     const uint32_t src = a.source;
+    const uint32_t sbeg = g.row[src], sdeg = g.row[src + 1] - sbeg;
+    const uint64_t sbit = 1ull << (src & 63);
     for (uint32_t i = gtid; i < n; i += nthreads) a.level[i] = (i == a.source_id) ? 0 : -1;
-    for (uint32_t w = gtid; w < W; w += nthreads) {
-        const uint64_t sb = (w == (src >> 6)) ? (1ull << (src & 63)) : 0ull;
-        a.vis[w] = (g.inert ? g.inert[w] : 0ull) | sb;
-        a.fbit[w] = sb;              // frontier 0
-        a.fbit[(size_t)W + w] = 0;   // frontier 1, built by round 0
-    }
-    if (blockIdx.x == 0) {
-        clear_ctr(&a.ctr[0]);
-        clear_ctr(&a.ctr[1]);
-        clear_ctr(&a.ctr[2]);
-        block_sync_global();
-        if (threadIdx.x == 0) {
-            const uint32_t deg = g.row[src + 1] - g.row[src];
-            a.queue[0] = src;
-            a.ctr[0].qlen = 1;
-            a.ctr[0].has_huge = deg > kHugeDegree;
-            a.ctr[0].stripe[0].scout = deg;
-            a.ctr[0].stripe[0].count = 1;
-        }
-    }
-    bool ok = grid_sync(a.bar, xcc, gen, a.abort);
-
-    // ---- replicated policy state (identical in every block)
-    DirectionPolicy pol{a.alpha, a.beta, a.total_workload0, n, a.force};
+    for (uint32_t w = gtid; w < W; w += nthreads) a.vis_idle[w] = g.inert ? g.inert[w] : 0ull;
+    DirectionPolicy pol{a.alpha, a.beta, a.total_workload0, n, a.force};  // replicated in every block
     unsigned long long ptrace = 0;
     uint32_t pushes = 0, pulls = 0;
     uint32_t L = 0;
+    const bool local0 = !a.full_init && a.force != 1 && sdeg <= kLocalRound0;
+    if (!local0) {
+        // state of unknown content (first query, other work since), a wide
+        // source or forced pull: reset everything, round 0 over the grid
+        for (uint32_t w = gtid; w < W; w += nthreads) {
+            const uint64_t sb = (w == (src >> 6)) ? sbit : 0ull;
+            a.vis[w] = (g.inert ? g.inert[w] : 0ull) | sb;
+            a.fbit[w] = sb;                    // frontier 0
+            a.fbit[(size_t)W + w] = 0;         // frontier 1, built by round 0
+            a.fbit[2 * (size_t)W + w] = 0;
+        }
+        if (blockIdx.x == 0) {
+            clear_ctr(&a.ctr[0]);
+            clear_ctr(&a.ctr[1]);
+            clear_ctr(&a.ctr[2]);
+            block_sync_global();
+            if (threadIdx.x == 0) {
+                a.queue[0] = src;
+                a.ctr[0].qlen = 1;
+                a.ctr[0].has_huge = sdeg > kHugeDegree;
+                a.ctr[0].stripe[0].scout = sdeg;
+                a.ctr[0].stripe[0].count = 1;
+            }
+        }
+    } else if (blockIdx.x == 0) {
+        // round 0 without its levels: other blocks are still resetting the
+        // level array, so round 1 writes level 1 for the queue it reads
+        if (threadIdx.x == 0) {
+            atomicOr((unsigned long long *)&a.vis[src >> 6], (unsigned long long)sbit);
+            s_qn = 0;
+        }
+        block_sync_global();
+        Round R;
+        R.a = &a;
+        R.L = 0;
+        R.fcur = nullptr;
+        R.fnext = a.fbit + (size_t)W;
+        R.S = Stager{a.queue + (size_t)a.qcap, a.qcap, &a.ctr[1], s_q, &s_qn, kLdsQ};
+        R.scout = 0;
+        R.count = 0;
+        R.huge = false;
+        R.levels = false;
+        for (uint32_t e0 = (threadIdx.x / kWave) * kWave; e0 < sdeg; e0 += WPB * kWave) {
+            const uint32_t e = e0 + (uint32_t)ln;
+            const bool act = e < sdeg;
+            push_visit(R, act, act ? g.col[sbeg + e] : 0u);
+        }
+        flush_stage(R.S, s_flush);
+        publish_counts<WPB>(&a.ctr[1], R.scout, R.count, R.huge, s_red);
+    }
+    bool ok = grid_sync(a.bar, xcc, gen, a.abort);
+    if (local0) {
+        pol.next(0, 1, sdeg);  // push, as every round 0
+        pushes = 1;
+        if (gtid == 0) write_trace(a.trace, 0, false, 1, sdeg, t_start);
+        L = 1;
+    }
+
     while (ok) {
         read_counts(&a.ctr[L % 3], s_lvl);
         if (threadIdx.x == 0) s_qn = 0;
         __syncthreads();
         const RoundCounts rc = unpack_counts(s_lvl, a.qcap);
         __syncthreads();
+        if (L == 1 && local0) {
+            // the source's out-neighbours found by block 0 (at most
+            // kLocalRound0, so the queue holds them all)
+            const uint32_t *q1 = a.queue + (size_t)a.qcap;
+            for (uint32_t i = gtid; i < rc.qlen; i += nthreads) a.level[caller_id(a, q1[i])] = 1;
+        }
         if (rc.count == 0) break;
         if (a.max_rounds && L >= a.max_rounds) break;
 
@@ -423,8 +478,17 @@ __global__ SG_BFS_BOUNDS(BS) void k_bfs(BfsArgs a)
         ok = grid_sync(a.bar, xcc, gen, a.abort);
     }
 
-    // levels in the caller's vertex order: every block left the loop after
-    // the same barrier, which made the last round's levels visible
+    // ---- epilogue: every block left the loop after the same barrier, which
+    // made the last round's levels visible. Zero the frontier bitmaps and
+    // counters for the next query.
+    if (ok) {
+        for (uint32_t w = gtid; w < 3 * W; w += nthreads) a.fbit[w] = 0;
+        if (blockIdx.x == 0) {
+            clear_ctr(&a.ctr[0]);
+            clear_ctr(&a.ctr[1]);
+            clear_ctr(&a.ctr[2]);
+        }
+    }
     if (gtid == 0) write_final(a.trace, a.stats, L, pushes, pulls, ptrace, ok, t_start);
 }
 
@@ -472,8 +536,15 @@ int bfs_launch(sg_engine *e, uint32_t source, int32_t *out, bool traced)
     a.g = d;
     a.level = out;
     a.perm = e->g->relabeled ? e->g->d_perm : nullptr;
-    a.vis = e->bitmaps;
-    a.fbit = e->bitmaps + d.words;
+    // state left clean by the previous BFS unless another kernel ran since
+    // (each persistent launch advances launch_seq) or an abort reset it
+    const bool clean = e->bfs_token != 0 && e->bfs_token == e->launch_seq && !e->needs_bar_reset;
+    a.full_init = clean ? 0u : 1u;
+    a.vis = e->bitmaps + (size_t)(e->bfs_parity & 1) * d.words;
+    a.vis_idle = e->bitmaps + (size_t)((e->bfs_parity & 1) ^ 1) * d.words;
+    a.fbit = e->bitmaps + 2 * (size_t)d.words;
+    if (++e->bfs_launch_id == 0) e->bfs_launch_id = 1;
+    a.launch_id = e->bfs_launch_id;
     a.queue = e->queues;
     a.qcap = e->qcap;
     a.ctr = (FrontierCtr *)e->ctr;
@@ -497,6 +568,8 @@ int bfs_launch(sg_engine *e, uint32_t source, int32_t *out, bool traced)
     default: SG_FAIL(SG_ERR_UNSUPPORTED, "sg_bfs: block_size must be 256, 512 or 1024");
     }
     if (rc) return rc;
+    e->bfs_token = e->launch_seq;
+    e->bfs_parity ^= 1;
     if (traced) e->timed = true;
     return SG_OK;
 }

@@ -475,7 +475,9 @@ static int size_grid(sg_engine *e, sg_engine::PGrid &pg, int per_cu)
 int engine_persistent_grid(sg_engine *e, const void *fn, int bs, const std::function<int(int, uint32_t *)> &probe,
                            sg_engine::PGrid **out)
 {
+    e->launch_seq++;  // any persistent kernel may overwrite the BFS state
     if (e->needs_bar_reset) {
+        e->bfs_token = 0;
         // an aborted run leaves counters mid-way: zero and re-probe every grid
         for (int i = 0; i < e->ngrids; i++) {
             sg_engine::PGrid &pg = e->grids[i];
@@ -654,8 +656,9 @@ int engine_ensure_bfs_state(sg_engine *e)
     };
     int rc;
     if ((rc = engine_ensure_value(e, (uint64_t)d.n * 4))) return rc;
-    // BFS: visited filter + 3 rotating frontier bitmaps (SSSP uses the first 3)
-    if ((rc = alloc((void **)&e->bitmaps, (uint64_t)d.words * 8 * 4))) return rc;
+    // BFS: 2 visited filters + 3 rotating frontier bitmaps (SSSP uses the
+    // first 3 words-sized areas, PageRank the first 2)
+    if ((rc = alloc((void **)&e->bitmaps, (uint64_t)d.words * 8 * 5))) return rc;
     // Frontier queues may hold a vertex more than once (non-atomic discovery
     // across XCDs); past qcap a round falls back to scanning the level array.
     e->qcap = (uint32_t)std::min<uint64_t>(2ull * d.n + 4096, 0xFFFFFFF0ull);

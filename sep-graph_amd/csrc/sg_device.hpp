@@ -85,23 +85,31 @@ __device__ __forceinline__ void wave_append(bool pred, uint32_t v, uint32_t *q, 
 // flag set by the XCD's leader measured slower on MI355X (one more store/poll
 // hop on the critical path).
 //
+// Counters come in two sets used by alternate barriers (generation parity).
+// A set's counters are zeroed by the blocks that finished with them -- the
+// XCD leader after its top arrival, the last XCD after the flip -- and those
+// stores drain before the same blocks arrive at the next barrier, so they
+// land before the set is used again two barriers on. No store has to
+// complete between the last arrival and the flip.
+//
 // The number of blocks each XCD receives for a given grid is measured once per
 // engine by a census kernel (dispatch is deterministic); a launch whose blocks
 // land differently cannot complete a group, runs into the bounded spin and
 // reports a timeout instead of hanging.
 constexpr int kBarLine = 32;      // uint32 per 128-byte line
 constexpr int kMaxXcd = 16;       // XCC_ID is 4 bits wide
-constexpr int kBarWords = kBarLine * (2 + 3 * kMaxXcd);
+constexpr int kBarWords = kBarLine * (5 + 2 * kMaxXcd);
 
 // Word offsets inside the barrier area.
 constexpr int kBarGen = 0;
 constexpr int kBarStamp = 2;  // 64-bit: wall clock when a stamped barrier completed
-constexpr int kBarTop = kBarLine;
-__host__ __device__ constexpr int bar_arrive(int x) { return kBarLine * (2 + x); }
-__host__ __device__ constexpr int bar_release(int x) { return kBarLine * (2 + kMaxXcd + x); }
-__host__ __device__ constexpr int bar_count(int x) { return kBarLine * (2 + 2 * kMaxXcd) + x; }
+__host__ __device__ constexpr int bar_top(int set) { return kBarLine * (1 + set); }
+__host__ __device__ constexpr int bar_arrive(int set, int x) { return kBarLine * (3 + set * kMaxXcd + x); }
+__host__ __device__ constexpr int bar_count(int x) { return kBarLine * (3 + 2 * kMaxXcd) + x; }
 // residency probe: blocks registered / blocks whose wait for the others expired
-constexpr int kBarProbeTotal = kBarLine * (2 + 2 * kMaxXcd) + kMaxXcd;
+constexpr int kBarProbeTotal = kBarLine * (3 + 2 * kMaxXcd) + kMaxXcd;
+// one-to-all release flag (a query's id), in a line of its own
+constexpr int kBarFlag = kBarLine * (4 + 2 * kMaxXcd);
 constexpr int kBarProbeFail = kBarProbeTotal + 1;
 
 struct GridBarrier {
@@ -207,21 +215,24 @@ __device__ __forceinline__ bool grid_sync(const GridBarrier &gb, uint32_t xcc, u
         const uint32_t gen0 = gen;
         gen = gen0 + 1;
         const uint32_t cnt = gb.cnt[xcc];
+        const int set = (int)(gen0 & 1u);
         uint32_t ok = 1;
-        const uint32_t a = atomic_add_relaxed(w + bar_arrive(xcc), 1u);
+        uint32_t *arr = w + bar_arrive(set, (int)xcc);
+        uint32_t *top = w + bar_top(set);
+        const uint32_t a = atomic_add_relaxed(arr, 1u);
         if (a == cnt - 1) {
-            flag_store(w + bar_arrive(xcc), 0u);
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-            const uint32_t t = atomic_add_relaxed(w + kBarTop, 1u);
+            const uint32_t t = atomic_add_relaxed(top, 1u);
+            flag_store(arr, 0u);  // this set is used again two barriers on
             if (t == gb.nxcd - 1) {
-                // the reset must complete before the generation flips, or a
-                // leader's next-barrier arrival could be overwritten by it
-                flag_store(w + kBarTop, 0u);
-                if (kStamp)
+                if (kStamp) {
+                    // the stamp must be visible before the flip that publishes it
                     __hip_atomic_store((uint64_t *)(w + kBarStamp), (uint64_t)wall_clock64(), __ATOMIC_RELAXED,
                                        __HIP_MEMORY_SCOPE_SYSTEM);
-                __builtin_amdgcn_s_waitcnt(0);
+                    __builtin_amdgcn_s_waitcnt(0);
+                }
                 flag_store(w + kBarGen, gen0 + 1);
+                flag_store(top, 0u);
             }
         }
         // every block polls the generation word itself (a relay through a

@@ -69,7 +69,7 @@ struct DevGraph {
 #define SG_HUGE_DEGREE 256
 #endif
 #ifndef SG_HUGE_CHUNK
-#define SG_HUGE_CHUNK 128
+#define SG_HUGE_CHUNK 64
 #endif
 constexpr uint32_t kHugeDegree = SG_HUGE_DEGREE;
 constexpr uint32_t kHugeChunk = SG_HUGE_CHUNK;
@@ -144,7 +144,13 @@ struct sg_engine {
     void *value = nullptr;        // result of the last run (levels / distances / ranks / BC)
     uint64_t value_bytes = 0;     // bytes of that result
     uint64_t value_cap = 0;       // allocated bytes
-    uint64_t *bitmaps = nullptr;  // 4 * words: visited + 3 rotating frontier bitmaps
+    uint64_t *bitmaps = nullptr;  // 5 * words: 2 visited filters + 3 rotating frontier bitmaps
+    // BFS state left clean by the last BFS (frontier bitmaps and counters
+    // zeroed, the idle visited filter reset): valid while no other kernel ran
+    uint64_t launch_seq = 0;      // persistent-kernel launches on this engine
+    uint64_t bfs_token = 0;       // launch_seq right after the last BFS launch
+    uint32_t bfs_parity = 0;      // visited filter the next BFS uses
+    uint32_t bfs_launch_id = 0;   // id the next BFS releases its first round with
     uint32_t *queues = nullptr;   // 3 frontier queues of qcap entries
     uint32_t qcap = 0;
     void *ctr = nullptr;          // kCtrBytes of per-round counters

@@ -498,7 +498,7 @@ class Engine:
             for r in rows[: nrow.value]
         ]
 
-    def debug_grid(self, index: int = 0, nwords: int = 1600):
+    def debug_grid(self, index: int = 0, nwords: int = 1152):
         info = np.zeros(19, np.uint32)
         words = np.zeros(nwords, np.uint32)
         _check(lib().sg_engine_debug_grid(self.h, index, _ptr(info), _ptr(words), nwords))

@@ -181,3 +181,47 @@ def test_bfs_engine_reuse_and_two_engines(sg, orc, gpu, rmat16_sym):
         e2.bfs(s + 1)
         assert np.array_equal(e1.gather(), orc.bfs(hg.row, hg.col, s))
         assert np.array_equal(e2.gather(), orc.bfs(hg.row, hg.col, s + 1))
+
+
+def test_bfs_state_reuse_across_kinds(sg, orc, gpu):
+    """A BFS that follows another BFS reuses the state that query left
+    (frontier bitmaps and counters zeroed, the idle visited filter reset) and
+    expands a narrow source in one block; one that follows other kernels, or
+    starts from a wide source, resets everything. Every order gives the
+    oracle's levels."""
+    hg = sg.rmat_graph(15, 16, symmetrize=True, seed=5, weights=(1, 64))
+    deg = np.diff(hg.row.astype(np.int64))
+    wide = int(np.argmax(deg))
+    assert deg[wide] > 4096  # above what one block expands alone
+    narrow = [int(s) for s in sg.pick_sources(hg.row, 6, seed=8)]
+    for relabel in (False, True):
+        e = sg.Engine(sg.Graph(hg, relabel=relabel))
+        steps = [("bfs", narrow[0]), ("bfs", narrow[1]), ("sssp", narrow[2]), ("bfs", narrow[3]),
+                 ("pr", 0), ("bfs", narrow[4]), ("bfs", wide), ("bfs", narrow[5]), ("multi", 0),
+                 ("bfs", narrow[0]), ("bfs", narrow[0])]
+        for kind, s in steps:
+            if kind == "bfs":
+                e.bfs(s)
+                assert np.array_equal(e.gather(np.int32), orc.bfs(hg.row, hg.col, s)), (relabel, s)
+                assert e.stats()["status"] == 0
+            elif kind == "sssp":
+                e.sssp(s)
+            elif kind == "pr":
+                e.pagerank(0.85, 1e-4, 3, sg.PR_PULL_TD)
+            else:
+                e.bfs_multi(narrow[:3], 64)
+            e.sync()
+
+
+def test_bfs_round_limit_after_reuse(sg, orc, gpu):
+    """max_iteration = 1 stops after round 0: the source at 0, its
+    out-neighbours at 1 (written at the start of the next round when block 0
+    expanded the source alone), every other vertex -1."""
+    hg = sg.rmat_graph(13, 16, symmetrize=False, seed=2)
+    srcs = [int(s) for s in sg.pick_sources(hg.row, 4, seed=1)]
+    e = sg.Engine(sg.Graph(hg), max_iteration=1)
+    for s in srcs + srcs:
+        e.bfs(s)
+        ref = orc.bfs(hg.row, hg.col, s)
+        want = np.where((ref >= 0) & (ref <= 1), ref, -1)
+        assert np.array_equal(e.gather(np.int32), want), s
