@@ -28,6 +28,12 @@
  *     takes a hipStream_t as void*); gather/stats calls synchronise it.
  *   - a handle is not safe for concurrent use from several threads; distinct
  *     handles are independent.
+ *   - the traversal kernels are persistent (every block resident, grid
+ *     barriers between rounds) and each fills the GPU: runs of two engines
+ *     on different streams of one device must not overlap in time (share a
+ *     stream through sg_engine_set_stream, or synchronise one before starting
+ *     the other). Overlapping runs can leave both waiting for blocks that
+ *     cannot be scheduled; the bounded waits then end them with SG_ERR_TIMEOUT.
  */
 #ifndef SEPGRAPH_H
 #define SEPGRAPH_H

@@ -103,16 +103,20 @@ def test_server_batch_forms(sg, gpu, wgraph, packed_min):
     srcs = sg.pick_sources(wgraph.row, 48, seed=11)
     kinds = [sg.QUERY_BFS, sg.QUERY_SSSP] * 24
     tickets = [srv.submit(k, int(s)) for k, s in zip(kinds, srcs)]
-    e = sg.Engine(g)
-    for t, k, s in zip(tickets, kinds, srcs):
+    results = []
+    for t, k in zip(tickets, kinds):
         srv.wait(t, 60000)
-        got = srv.result(t, k)
-        assert same(sg, k, got, alone(sg, e, k, s)), (packed_min, k, int(s))
+        results.append(srv.result(t, k))
         srv.release(t)
     assert srv.stats()["failed"] == 0
+    srv.close()
+    # lone runs only once the server is idle: persistent kernels of two
+    # engines on different streams must not overlap on one GPU
+    e = sg.Engine(g)
+    for got, k, s in zip(results, kinds, srcs):
+        assert same(sg, k, got, alone(sg, e, k, s)), (packed_min, k, int(s))
     with pytest.raises(sg.SepGraphError):
         sg.Server(g, bfs_packed_min=66)
-    srv.close()
 
 
 def test_server_replay(sg, gpu, wgraph):
