@@ -79,7 +79,6 @@ struct BfsArgs {
     uint32_t max_rounds;
     uint32_t probe;   // launch is a residency probe of the grid
     uint32_t full_init;  // bitmaps and counters not known to be clean
-    uint32_t launch_id;  // value of the round-0 release flag (never 0)
 };
 
 // Largest source out-degree whose expansion (round 0) block 0 does alone.
@@ -509,11 +508,13 @@ int launch_bfs(sg_engine *e, BfsArgs &args, bool timed)
     args.bar.words = pg->words;
     args.bar.nxcd = pg->nxcd;
     for (int x = 0; x < kMaxXcd; x++) args.bar.cnt[x] = pg->cnt[x];
-    if (timed) SG_HIP(hipEventRecord(e->ev0, e->stream));
-    hipLaunchKernelGGL(k_bfs<BS>, dim3(pg->blocks), dim3(BS), 0, e->stream, args);
-    SG_HIP(hipGetLastError());
-    if (timed) SG_HIP(hipEventRecord(e->ev1, e->stream));
-    return SG_OK;
+    return engine_launch_persistent(e, [&]() -> int {
+        if (timed) SG_HIP(hipEventRecord(e->ev0, e->stream));
+        hipLaunchKernelGGL(k_bfs<BS>, dim3(pg->blocks), dim3(BS), 0, e->stream, args);
+        SG_HIP(hipGetLastError());
+        if (timed) SG_HIP(hipEventRecord(e->ev1, e->stream));
+        return SG_OK;
+    });
 }
 
 }  // namespace
@@ -543,8 +544,6 @@ int bfs_launch(sg_engine *e, uint32_t source, int32_t *out, bool traced)
     a.vis = e->bitmaps + (size_t)(e->bfs_parity & 1) * d.words;
     a.vis_idle = e->bitmaps + (size_t)((e->bfs_parity & 1) ^ 1) * d.words;
     a.fbit = e->bitmaps + 2 * (size_t)d.words;
-    if (++e->bfs_launch_id == 0) e->bfs_launch_id = 1;
-    a.launch_id = e->bfs_launch_id;
     a.queue = e->queues;
     a.qcap = e->qcap;
     a.ctr = (FrontierCtr *)e->ctr;

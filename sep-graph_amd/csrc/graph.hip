@@ -7,6 +7,7 @@
 #include "sg_internal.hpp"
 
 #include <cstring>
+#include <mutex>
 #include <vector>
 
 using namespace sg;
@@ -429,12 +430,31 @@ extern "C" int sg_engine_set_options(sg_engine *e, const sg_engine_options *o)
 
 namespace sg {
 
+static std::mutex g_pk_mu;
+static hipEvent_t g_pk_last = nullptr;      // after the last persistent kernel enqueued
+static hipStream_t g_pk_stream = nullptr;   // ... and the stream it went to
+
+int engine_launch_persistent(sg_engine *e, const std::function<int()> &launch)
+{
+    std::lock_guard<std::mutex> lk(g_pk_mu);
+    if (!g_pk_last) {
+        SG_HIP(hipEventCreateWithFlags(&g_pk_last, hipEventDisableTiming));
+    } else if (g_pk_stream != e->stream) {
+        SG_HIP(hipStreamWaitEvent(e->stream, g_pk_last, 0));
+    }
+    int rc = launch();
+    if (rc) return rc;
+    SG_HIP(hipEventRecord(g_pk_last, e->stream));
+    g_pk_stream = e->stream;
+    return SG_OK;
+}
+
 // Probe one grid size: true when every block was resident at once; fills the
 // per-XCD block counts the barrier needs.
 static int probe_grid(sg_engine *e, sg_engine::PGrid &pg, bool *resident)
 {
     SG_HIP(hipMemsetAsync(pg.words, 0, kBarWords * 4, e->stream));
-    int rc = pg.probe(pg.blocks, pg.words);
+    int rc = engine_launch_persistent(e, [&]() { return pg.probe(pg.blocks, pg.words); });
     if (rc) return rc;
     uint32_t counts[kMaxXcd + 2];
     SG_HIP(hipMemcpyAsync(counts, pg.words + bar_count(0), sizeof(counts), hipMemcpyDeviceToHost, e->stream));

@@ -1014,9 +1014,11 @@ int launch_grid(sg_engine *e, const void *fn, int bs, Args &a)
     a.bar.words = pg->words;
     a.bar.nxcd = pg->nxcd;
     for (int x = 0; x < kMaxXcd; x++) a.bar.cnt[x] = pg->cnt[x];
-    void *params[] = {&a};
-    SG_HIP(hipLaunchKernel(fn, dim3(pg->blocks), dim3(bs), params, 0, e->stream));
-    return SG_OK;
+    return engine_launch_persistent(e, [&]() -> int {
+        void *params[] = {&a};
+        SG_HIP(hipLaunchKernel(fn, dim3(pg->blocks), dim3(bs), params, 0, e->stream));
+        return SG_OK;
+    });
 }
 
 }  // namespace

@@ -689,11 +689,13 @@ int launch_persistent(sg_engine *e, Args &args, int bs)
     args.bar.words = pg->words;
     args.bar.nxcd = pg->nxcd;
     for (int x = 0; x < kMaxXcd; x++) args.bar.cnt[x] = pg->cnt[x];
-    SG_HIP(hipEventRecord(e->ev0, e->stream));
-    void *params[] = {&args};
-    SG_HIP(hipLaunchKernel((const void *)K, dim3(pg->blocks), dim3(bs), params, 0, e->stream));
-    SG_HIP(hipEventRecord(e->ev1, e->stream));
-    return SG_OK;
+    return engine_launch_persistent(e, [&]() -> int {
+        SG_HIP(hipEventRecord(e->ev0, e->stream));
+        void *params[] = {&args};
+        SG_HIP(hipLaunchKernel((const void *)K, dim3(pg->blocks), dim3(bs), params, 0, e->stream));
+        SG_HIP(hipEventRecord(e->ev1, e->stream));
+        return SG_OK;
+    });
 }
 
 }  // namespace

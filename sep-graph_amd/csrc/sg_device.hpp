@@ -98,7 +98,7 @@ __device__ __forceinline__ void wave_append(bool pred, uint32_t v, uint32_t *q, 
 // reports a timeout instead of hanging.
 constexpr int kBarLine = 32;      // uint32 per 128-byte line
 constexpr int kMaxXcd = 16;       // XCC_ID is 4 bits wide
-constexpr int kBarWords = kBarLine * (5 + 2 * kMaxXcd);
+constexpr int kBarWords = kBarLine * (4 + 2 * kMaxXcd);
 
 // Word offsets inside the barrier area.
 constexpr int kBarGen = 0;
@@ -108,8 +108,6 @@ __host__ __device__ constexpr int bar_arrive(int set, int x) { return kBarLine *
 __host__ __device__ constexpr int bar_count(int x) { return kBarLine * (3 + 2 * kMaxXcd) + x; }
 // residency probe: blocks registered / blocks whose wait for the others expired
 constexpr int kBarProbeTotal = kBarLine * (3 + 2 * kMaxXcd) + kMaxXcd;
-// one-to-all release flag (a query's id), in a line of its own
-constexpr int kBarFlag = kBarLine * (4 + 2 * kMaxXcd);
 constexpr int kBarProbeFail = kBarProbeTotal + 1;
 
 struct GridBarrier {

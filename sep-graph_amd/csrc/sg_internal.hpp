@@ -150,7 +150,6 @@ struct sg_engine {
     uint64_t launch_seq = 0;      // persistent-kernel launches on this engine
     uint64_t bfs_token = 0;       // launch_seq right after the last BFS launch
     uint32_t bfs_parity = 0;      // visited filter the next BFS uses
-    uint32_t bfs_launch_id = 0;   // id the next BFS releases its first round with
     uint32_t *queues = nullptr;   // 3 frontier queues of qcap entries
     uint32_t qcap = 0;
     void *ctr = nullptr;          // kCtrBytes of per-round counters
@@ -252,4 +251,10 @@ int engine_alloc(void **p, uint64_t bytes);
 // counts from that probe. Barriers are re-zeroed after an aborted run.
 int engine_persistent_grid(sg_engine *e, const void *fn, int bs,
                            const std::function<int(int, uint32_t *)> &probe, sg_engine::PGrid **out);
+// Enqueue one persistent kernel (or residency probe) through `launch`, after
+// the device's previous persistent kernel when that was enqueued on another
+// stream. Persistent grids fill the GPU: two that overlap can each hold CUs
+// the other's blocks need and wait at a grid barrier until their bounded
+// waits expire, so persistent kernels of all engines run one at a time.
+int engine_launch_persistent(sg_engine *e, const std::function<int()> &launch);
 }  // namespace sg

@@ -593,9 +593,11 @@ int launch_sssp(sg_engine *e, SsspArgs &args, uint32_t *group_words)
     } else {
         args.bar.words = pg->words;
     }
-    hipLaunchKernelGGL(k_sssp<BS>, dim3(pg->blocks), dim3(BS), 0, e->stream, args);
-    SG_HIP(hipGetLastError());
-    return SG_OK;
+    return engine_launch_persistent(e, [&]() -> int {
+        hipLaunchKernelGGL(k_sssp<BS>, dim3(pg->blocks), dim3(BS), 0, e->stream, args);
+        SG_HIP(hipGetLastError());
+        return SG_OK;
+    });
 }
 
 int launch_sssp_bs(sg_engine *e, SsspArgs &a, uint32_t *group_words)

@@ -225,3 +225,23 @@ def test_bfs_round_limit_after_reuse(sg, orc, gpu):
         ref = orc.bfs(hg.row, hg.col, s)
         want = np.where((ref >= 0) & (ref <= 1), ref, -1)
         assert np.array_equal(e.gather(np.int32), want), s
+
+
+def test_two_engines_interleaved_without_sync(sg, orc, gpu, rmat16_sym):
+    """Two engines (two streams) launching persistent kernels back to back
+    without host synchronisation: the launches are ordered device-wide, so
+    no run waits at a barrier for blocks the other holds."""
+    hg = rmat16_sym
+    g = sg.Graph(hg)
+    e1, e2 = sg.Engine(g), sg.Engine(g)
+    srcs = [int(s) for s in sg.pick_sources(hg.row, 12, seed=7)]
+    for i, s in enumerate(srcs):
+        e1.bfs(s)
+        if i % 3 == 2:
+            e2.pagerank(0.85, 1e-4, 5, sg.PR_PULL_TD)
+        else:
+            e2.bfs(srcs[-1 - i])
+    assert e1.stats()["status"] == 0 and e2.stats()["status"] == 0
+    assert np.array_equal(e1.gather(np.int32), orc.bfs(hg.row, hg.col, srcs[-1]))
+    e2.bfs(srcs[0])
+    assert np.array_equal(e2.gather(np.int32), orc.bfs(hg.row, hg.col, srcs[0]))
