@@ -29,11 +29,11 @@
  *   - a handle is not safe for concurrent use from several threads; distinct
  *     handles are independent.
  *   - the traversal kernels are persistent (every block resident, grid
- *     barriers between rounds) and each fills the GPU: runs of two engines
- *     on different streams of one device must not overlap in time (share a
- *     stream through sg_engine_set_stream, or synchronise one before starting
- *     the other). Overlapping runs can leave both waiting for blocks that
- *     cannot be scheduled; the bounded waits then end them with SG_ERR_TIMEOUT.
+ *     barriers between rounds) and each fills the GPU, so two of them must
+ *     not overlap: the library makes each engine's stream wait for the
+ *     device's previous persistent kernel when that went to another stream.
+ *     Runs of different engines therefore execute one after another even
+ *     when their streams are otherwise independent.
  */
 #ifndef SEPGRAPH_H
 #define SEPGRAPH_H
