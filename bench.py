@@ -150,7 +150,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--sssp-sources", type=int, default=16)
     ap.add_argument("--no-serving", action="store_true")
-    ap.add_argument("--serve-rates", type=str, default="400,800,1200,1600,2000,2600",
+    ap.add_argument("--serve-rates", type=str, default="400,800,1000,1200,1600,2000,2600",
                     help="Poisson arrival rates (queries/s) for the serving run")
     ap.add_argument("--serve-count", type=int, default=1200, help="queries per arrival rate")
     ap.add_argument("--serve-plain", type=int, default=100, help="queries timed one at a time (plain baseline)")
