@@ -469,9 +469,10 @@ __global__ SG_BFS_BOUNDS(BS) void k_bfs(BfsArgs a)
                 }
             }
         }
-        flush_stage(R.S, s_flush);
-
+        // counters first: their atomics are then in flight while the queue
+        // flush waits for its position
         publish_counts<WPB>(R.S.nxt, R.scout, R.count, R.huge, s_red);
+        flush_stage(R.S, s_flush);
         pol.pull_prev = pull;
         L++;
         ok = grid_sync(a.bar, xcc, gen, a.abort);

@@ -625,8 +625,8 @@ __global__ __launch_bounds__(BS) void k_pr_dd(PrDdArgs a)
                 dd_accumulate(R, own, v, s, false);
             }
         }
-        flush_stage(R.S, s_flush);
         publish_counts<WPB>(&q_ctr[(k + 1) % 3], R.qscout, R.count, false, s_red);
+        flush_stage(R.S, s_flush);
         k++;
         ok = grid_sync<true>(a.bar, xcc, gen, a.abort);
     }

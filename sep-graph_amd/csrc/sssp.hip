@@ -491,8 +491,8 @@ __device__ __forceinline__ void sssp_run(const SsspArgs &a, uint32_t brank, uint
                 pull_word<kPullSteps>(R, w);
             }
         }
-        flush_stage(R.S, s_flush);
-
+        // counters first: their atomics are then in flight while the queue
+        // flush waits for its position
         publish_counts<WPB>(R.S.nxt, R.scout, R.count, R.huge, s_red);
         if (a.delta) {
             // block minimum of the queued distances, one atomic per block
@@ -505,6 +505,7 @@ __device__ __forceinline__ void sssp_run(const SsspArgs &a, uint32_t brank, uint
             __syncthreads();
             if (threadIdx.x == 0 && s_bmin != kInf) atomicMax(&slot_cmin(R.S.nxt), ~s_bmin);
         }
+        flush_stage(R.S, s_flush);
         pol.pull_prev = pull;
         L++;
         ok = grid_sync(a.bar, xcc, gen, a.abort);
