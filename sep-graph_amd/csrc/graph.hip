@@ -391,6 +391,7 @@ extern "C" int sg_engine_destroy(sg_engine *e)
 {
     if (!e) return SG_OK;
     if (e->stream) (void)hipStreamSynchronize(e->stream);
+    engine_forget_stream(e);
     void *ptrs[] = {e->value, e->bitmaps, e->queues, e->ctr,
                     e->abort_flag, e->stats, e->trace, e->exits, e->aux, e->srcbuf, e->garena, e->work};
     for (void *p : ptrs)
@@ -407,6 +408,7 @@ extern "C" int sg_engine_set_stream(sg_engine *e, void *hip_stream)
 {
     if (!e) SG_FAIL(SG_ERR_INVALID, "sg_engine_set_stream: null engine");
     SG_HIP(hipStreamSynchronize(e->stream));
+    engine_forget_stream(e);
     if (e->own_stream) (void)hipStreamDestroy(e->stream);
     e->stream = (hipStream_t)hip_stream;
     e->own_stream = false;
@@ -431,22 +433,31 @@ extern "C" int sg_engine_set_options(sg_engine *e, const sg_engine_options *o)
 namespace sg {
 
 static std::mutex g_pk_mu;
-static hipEvent_t g_pk_last = nullptr;      // after the last persistent kernel enqueued
-static hipStream_t g_pk_stream = nullptr;   // ... and the stream it went to
+static hipEvent_t g_pk_last = nullptr;     // recorded on demand, see below
+static hipStream_t g_pk_stream = nullptr;  // stream of the last persistent launch
 
 int engine_launch_persistent(sg_engine *e, const std::function<int()> &launch)
 {
     std::lock_guard<std::mutex> lk(g_pk_mu);
-    if (!g_pk_last) {
-        SG_HIP(hipEventCreateWithFlags(&g_pk_last, hipEventDisableTiming));
-    } else if (g_pk_stream != e->stream) {
+    if (g_pk_stream && g_pk_stream != e->stream) {
+        // recorded now on the other stream, the event completes no earlier
+        // than that stream's last persistent kernel; launches that stay on
+        // one stream record nothing (a record costs ~5 us between
+        // back-to-back kernels)
+        if (!g_pk_last) SG_HIP(hipEventCreateWithFlags(&g_pk_last, hipEventDisableTiming));
+        SG_HIP(hipEventRecord(g_pk_last, g_pk_stream));
         SG_HIP(hipStreamWaitEvent(e->stream, g_pk_last, 0));
     }
     int rc = launch();
     if (rc) return rc;
-    SG_HIP(hipEventRecord(g_pk_last, e->stream));
     g_pk_stream = e->stream;
     return SG_OK;
+}
+
+void engine_forget_stream(sg_engine *e)
+{
+    std::lock_guard<std::mutex> lk(g_pk_mu);
+    if (g_pk_stream == e->stream) g_pk_stream = nullptr;
 }
 
 // Probe one grid size: true when every block was resident at once; fills the

@@ -257,4 +257,7 @@ int engine_persistent_grid(sg_engine *e, const void *fn, int bs,
 // the other's blocks need and wait at a grid barrier until their bounded
 // waits expire, so persistent kernels of all engines run one at a time.
 int engine_launch_persistent(sg_engine *e, const std::function<int()> &launch);
+// Drop the engine's stream from that ordering (the stream was synchronised:
+// its persistent kernels are done), before the stream is destroyed or replaced.
+void engine_forget_stream(sg_engine *e);
 }  // namespace sg
