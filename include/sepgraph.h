@@ -310,7 +310,11 @@ int sg_sssp(sg_engine *e, uint32_t source);
  * (RebuildArrayWorklist), a data-driven one from the queue the previous round
  * filled. options.force_variant SYNC_PULL_TD / ASYNC_PUSH_TD / ASYNC_PUSH_DD
  * runs that variant every round (ForceVariant); other forced variants are
- * SG_ERR_UNSUPPORTED. The trace rows give each round's variant, active
+ * SG_ERR_UNSUPPORTED. With options.priority SG_PRIO_SAMPLING
+ * (SetSampleBasedPriority, 0 < cut_threshold < 1) an ASYNC_PUSH_TD round
+ * pushes only residuals at or above the value at index 1000 * (1 - cut) of
+ * 1000 sampled residuals in ascending order (GetPriorityThreshold,
+ * include/framework/algo_variants.cuh:98-131). The trace rows give each round's variant, active
  * vertices and out-degree, and the barrier clock at the round's end. */
 enum { SG_PR_AUTO = 0, SG_PR_PULL_TD = 1, SG_PR_PUSH_DD = 2, SG_PR_ITERATIVE = 3 };
 

@@ -307,7 +307,56 @@ struct PrDdArgs {
     uint32_t iterative;  // rounds chosen by the ITERATIVE scheme's predicted times
     int32_t it_force;    // ITERATIVE: -1, or the variant every round runs (kItPullTd...)
     float ticks_per_us;  // wall clock rate
+    float cut;           // sample-based priority for ASYNC_PUSH_TD rounds (0: off)
 };
+
+// Sample-based priority (GetPriorityThreshold, include/framework/algo_variants.cuh:98-131;
+// PRIORITY_SAMPLE_SIZE, graph_datum.cuh:18): a round's threshold is the
+// residual at index size * (1 - cut) of size sampled residuals in ascending
+// order, and only vertices at or above it push (IsHighPriority,
+// samples/hybrid_pr/hybrid_pr.cu:117-121).
+constexpr uint32_t kPrioSample = 1000;
+constexpr uint32_t kPrioSort = 1024;  // sample padded to a power of two
+
+__device__ __forceinline__ uint64_t sample_hash(uint64_t z)
+{
+    // SplitMix64 finaliser (Steele, Lea & Flood, OOPSLA 2014)
+    z += 0x9E3779B97F4A7C15ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+
+// Every block draws the same vertices (a hash of round and index) and sorts
+// their residuals in LDS (bitonic); residuals other blocks have already moved
+// out this round read as 0, which can only lower a block's threshold.
+template <int BS>
+__device__ double sampled_threshold(const PrDdArgs &a, uint32_t k, double *s_smp)
+{
+    const uint32_t n = a.g.n, size = min(kPrioSample, n);
+    for (uint32_t i = threadIdx.x; i < kPrioSort; i += BS)
+        s_smp[i] = i < size ? a.r[(uint32_t)(sample_hash(((uint64_t)k << 32) | i) % n)] : __builtin_inf();
+    __syncthreads();
+    for (uint32_t len = 2; len <= kPrioSort; len <<= 1) {
+        for (uint32_t j = len >> 1; j > 0; j >>= 1) {
+            for (uint32_t i = threadIdx.x; i < kPrioSort; i += BS) {
+                const uint32_t p = i ^ j;
+                if (p > i) {
+                    const double x = s_smp[i], y = s_smp[p];
+                    if ((x > y) == ((i & len) == 0)) {
+                        s_smp[i] = y;
+                        s_smp[p] = x;
+                    }
+                }
+            }
+            __syncthreads();
+        }
+    }
+    const uint32_t at = min((uint32_t)((float)size * (1.0f - a.cut)), size - 1);
+    const double t = s_smp[at];
+    __syncthreads();
+    return t;
+}
 
 // The ITERATIVE scheme's variant set in its std::set order
 // (include/framework/hybrid_policy.h:172-186; AlgoVariant::operator< compares
@@ -427,6 +476,7 @@ __global__ __launch_bounds__(BS) void k_pr_dd(PrDdArgs a)
     FrontierCtr *act_ctr = a.ctr;       // 2 slots
     FrontierCtr *q_ctr = a.ctr + 2;     // 3 slots
 
+    __shared__ double s_smp[kPrioSort];
     __shared__ double2 s_red2[WPB];
     __shared__ double2 s_glob;
     __shared__ unsigned long long s_red[2][WPB];
@@ -451,6 +501,7 @@ __global__ __launch_bounds__(BS) void k_pr_dd(PrDdArgs a)
     unsigned long long stamp0 = 0, stamp_prev = 0;
     float in_wl_prev = 0.f;
     int v_prev = -1;
+    bool held = false;  // the last activation held residuals >= theta back (priority)
 
     uint32_t k = 0, pushes = 0, pulls = 0;
     unsigned long long ptrace = 0;
@@ -486,13 +537,19 @@ __global__ __launch_bounds__(BS) void k_pr_dd(PrDdArgs a)
             scout += dout;
             cnt++;
         };
-        if (k == 0 || qc.overflow || var == kItPullTd || var == kItPushTd) {
+        bool waiting = false;  // this lane saw a residual >= theta left below the priority threshold
+        // a scan also after a round that held residuals back: they did not
+        // cross theta again, so no queue holds them
+        if (k == 0 || qc.overflow || held || var == kItPullTd || var == kItPushTd) {
+            double thr = theta;
+            if (var == kItPushTd && a.cut > 0.f) thr = fmax(theta, sampled_threshold<BS>(a, k, s_smp));
             for (uint32_t w = gwave; w < W; w += nwaves) {
                 const uint32_t v = w * 64 + ln;
                 bool act = false;
                 if (v < n) {
                     const double rv = a.r[v];
-                    act = rv >= theta;
+                    act = rv >= thr;
+                    waiting |= !act && rv >= theta;
                     if (act) {
                         a.r[v] = 0.0;
                         activate(v, rv);
@@ -515,7 +572,8 @@ __global__ __launch_bounds__(BS) void k_pr_dd(PrDdArgs a)
                 }
             }
         }
-        publish_counts<WPB>(&act_ctr[k & 1], scout, cnt, false, s_red);
+        // the slot's has_huge flag carries "residuals >= theta were held back"
+        publish_counts<WPB>(&act_ctr[k & 1], scout, cnt, waiting, s_red);
         ok = grid_sync(a.bar, xcc, gen, a.abort);
         if (!ok) break;
 
@@ -525,7 +583,8 @@ __global__ __launch_bounds__(BS) void k_pr_dd(PrDdArgs a)
         __syncthreads();
         const RoundCounts ac = unpack_counts(s_lvl, a.qcap);
         __syncthreads();
-        if (ac.count == 0) break;  // every residual is below theta
+        if (ac.count == 0 && !ac.has_huge) break;  // every residual is below theta
+        held = ac.has_huge;
         bool pull;
         if (var >= 0) pull = var == kItPullTd;
         else if (a.force >= 0) pull = a.force == 1;
@@ -792,6 +851,12 @@ extern "C" int sg_pagerank(sg_engine *e, double damping, double tol, int32_t max
         a.iterative = mode == SG_PR_ITERATIVE;
         a.it_force = it_force;
         a.ticks_per_us = (float)(e->wall_khz / 1000.0);
+        a.cut = 0.f;
+        if (mode == SG_PR_ITERATIVE && o.priority == SG_PRIO_SAMPLING) {
+            if (!(o.cut_threshold > 0.f && o.cut_threshold < 1.f))
+                SG_FAIL(SG_ERR_RANGE, "sg_pagerank: sample-based priority needs 0 < cut_threshold < 1");
+            a.cut = o.cut_threshold;
+        }
         switch (bs) {
         case 256: rc = launch_persistent<PrDdArgs, k_pr_dd<256>>(e, a, 256); break;
         case 512: rc = launch_persistent<PrDdArgs, k_pr_dd<512>>(e, a, 512); break;

@@ -261,3 +261,46 @@ def test_pr_iterative_rejects_other_variants(sg, gpu):
     with pytest.raises(sg.SepGraphError) as ei:
         e.pagerank(D, 1e-6, 0, sg.PR_ITERATIVE)
     assert ei.value.code == -8
+
+
+@pytest.mark.parametrize("name", ["dir_random", "sparse_dir", "star_out", "grid", "rmat16"])
+@pytest.mark.parametrize("cut", [0.2, 0.6])
+def test_pr_iterative_sampling_priority(sg, orc, gpu, name, cut):
+    """Sample-based priority (SetSampleBasedPriority): every ASYNC_PUSH_TD
+    round pushes only residuals at or above a sampled quantile; residuals held
+    back are picked up by later rounds, so the ranks still converge to the
+    oracle's."""
+    if name == "rmat16":
+        hg = sg.rmat_graph(16, 16, symmetrize=False, seed=1)
+    else:
+        row, col = SMALL[name]()
+        hg = sg.HostGraph(len(row) - 1, row, col, None, undirected=False)
+    ref, _, _ = orc.pagerank(hg.row, hg.col, D, 1e-13, 100000)
+    for variant in ("async_push_td", "auto"):
+        x, st, _ = run_pr(sg, hg, sg.PR_ITERATIVE, tol=1e-9, force_variant=variant, priority=2, cut_threshold=cut)
+        assert rel_l1(x, ref) <= 1e-6, (name, cut, variant, st)
+        assert st["status"] == 0
+
+
+def test_pr_sampling_priority_rejects_bad_cut(sg, gpu):
+    row, col = G.random_graph(500, 3000, 3)
+    hg = sg.HostGraph(len(row) - 1, row, col, None)
+    for cut in (0.0, 1.0):
+        e = sg.Engine(sg.Graph(hg), priority=2, cut_threshold=cut)
+        with pytest.raises(sg.SepGraphError) as ei:
+            e.pagerank(D, 1e-6, 0, sg.PR_ITERATIVE)
+        assert ei.value.code == -2
+
+
+def test_pr_sampling_priority_holds_back(sg, orc, gpu):
+    """The priority actually filters: with the top 40 % of sampled residuals
+    pushed per round, forced ASYNC_PUSH_TD activates fewer vertices in its
+    second round than without priority, and needs more rounds to converge."""
+    hg = sg.rmat_graph(16, 16, symmetrize=False, seed=1)
+    _, plain, e0 = run_pr(sg, hg, sg.PR_ITERATIVE, tol=1e-9, force_variant="async_push_td")
+    t0 = [r for r in e0.trace(4096) if r["variant"] == "async_push_td"]
+    _, prio, e1 = run_pr(sg, hg, sg.PR_ITERATIVE, tol=1e-9, force_variant="async_push_td", priority=2,
+                         cut_threshold=0.4)
+    t1 = [r for r in e1.trace(4096) if r["variant"] == "async_push_td"]
+    assert t1[1]["in_nodes"] < t0[1]["in_nodes"], (t0[:3], t1[:3])
+    assert prio["rounds"] > plain["rounds"], (plain, prio)
