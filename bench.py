@@ -255,7 +255,7 @@ def main():
         log(f"cpu baseline {cpu['value']:.4f} GTEPS ({cpu['sample']}), exact {cpu['bit_exact_vs_gpu']}")
 
     out = {
-        "metric": "BFS GTEPS, RMAT-22 directed, 64 sources (bit-exact levels)",
+        "metric": f"BFS GTEPS, RMAT-{args.scale} directed, {nq} sources (bit-exact levels)",
         "value": round(gteps, 3),
         "unit": "GTEPS",
         "n_gpus": 1,
