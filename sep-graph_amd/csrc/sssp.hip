@@ -81,6 +81,8 @@ struct Round {
     const SsspArgs *a;
     uint32_t L;
     const uint64_t *fcur;
+    const uint64_t *sfb;  // LDS copy of fcur's first sfw words (pull rounds), else unused
+    uint32_t sfw;
     uint64_t *fnext;
     Stager S;
     unsigned long long scout, count;
@@ -95,7 +97,15 @@ __device__ __forceinline__ uint32_t &slot_cmin(FrontierCtr *c) { return c->pad[0
 
 __device__ __forceinline__ uint32_t sat_add(uint32_t d, uint32_t w) { return d > kInf - w ? kInf : d + w; }
 
-__device__ __forceinline__ bool in_frontier(const Round &R, uint32_t u) { return test_bit(R.fcur, u); }
+// Frontier membership; pull rounds read the low-id words (in a degree-ordered
+// graph the sources of most in-edges) from the block's LDS copy instead of
+// sending one L2 request per in-edge.
+__device__ __forceinline__ bool in_frontier(const Round &R, uint32_t u)
+{
+    const uint32_t w = u >> 6;
+    const uint64_t x = w < R.sfw ? R.sfb[w] : R.fcur[w];
+    return (x >> (u & 63)) & 1;
+}
 
 __device__ __forceinline__ uint32_t edge_w(const uint32_t *w, uint32_t e) { return w ? w[e] : 1u; }
 
@@ -324,6 +334,11 @@ __device__ __forceinline__ void sssp_run(const SsspArgs &a, uint32_t brank, uint
     __shared__ unsigned long long s_lvl[5];
     __shared__ uint32_t s_q[kLdsQ];
     __shared__ uint32_t s_qn, s_flush[2], s_cmin, s_bmin;
+#ifndef SG_FRONT_LDS
+#define SG_FRONT_LDS 12
+#endif
+    constexpr uint32_t kFrontLds = SG_FRONT_LDS * BS;  // frontier words cached in LDS by pull rounds
+    __shared__ uint64_t s_fb[kFrontLds];
 
     const uint32_t src = a.source;
     for (uint32_t i = gtid; i < n; i += nthreads) a.dist[i] = (i == src) ? 0u : kInf;
@@ -394,6 +409,8 @@ __device__ __forceinline__ void sssp_run(const SsspArgs &a, uint32_t brank, uint
         // every round leaves its next frontier in fnext (pull rounds by whole
         // words, push rounds one atomicOr per run of queued vertices)
         R.fcur = a.fbit + (size_t)(L % 3) * W;
+        R.sfb = s_fb;
+        R.sfw = 0;
         R.fnext = a.fbit + (size_t)((L + 1) % 3) * W;
         // the bitmap for round L+2 is free (read in round L-1): clear it
         uint64_t *fclr = a.fbit + (size_t)((L + 2) % 3) * W;
@@ -469,6 +486,9 @@ __device__ __forceinline__ void sssp_run(const SsspArgs &a, uint32_t brank, uint
                 }
             }
         } else {
+            R.sfw = min(W, kFrontLds);
+            for (uint32_t i = threadIdx.x; i < R.sfw; i += BS) s_fb[i] = R.fcur[i];
+            __syncthreads();
             // Pull: (1) hubs' in-edges through the static chunk list, spread
             // over all waves, kHubChunks chunks per wave at a time; (2) every
             // other vertex with in-edges, a word at a time.
