@@ -41,7 +41,7 @@ using namespace sg;
 
 // Words of 64 vertices a wave works on at once in a pull round.
 #ifndef SG_PULL_GROUP
-#define SG_PULL_GROUP 4
+#define SG_PULL_GROUP 2
 #endif
 // Minimum waves per SIMD the compiler must allow for k_bfs (0: no bound);
 // 8 caps registers at 64 so two 1024-thread blocks fit on a CU.
