@@ -490,6 +490,13 @@ __global__ __launch_bounds__(BS) void k_pr_dd(PrDdArgs a)
     FrontierCtr *q_ctr = a.ctr + 2;     // 3 slots
 
     __shared__ double s_smp[kPrioSort];
+    // pull rounds test the active bits of the lowest ids (in a degree-ordered
+    // graph the sources of most in-edges) in an LDS copy of the bitmap
+#ifndef SG_PRDD_LDS
+#define SG_PRDD_LDS 8
+#endif
+    constexpr uint32_t kAbLds = SG_PRDD_LDS * BS;
+    __shared__ uint64_t s_ab[kAbLds];
     __shared__ double2 s_red2[WPB];
     __shared__ double2 s_glob;
     __shared__ unsigned long long s_red[2][WPB];
@@ -666,6 +673,13 @@ __global__ __launch_bounds__(BS) void k_pr_dd(PrDdArgs a)
                 if (word) dd_push_word(R, word, w);
             }
         } else {
+            const uint32_t nab = min(W, kAbLds);
+            for (uint32_t i = threadIdx.x; i < nab; i += BS) s_ab[i] = ab[i];
+            __syncthreads();
+            auto active = [&](uint32_t u) -> bool {
+                const uint32_t w = u >> 6;
+                return ((w < nab ? s_ab[w] : ab[w]) >> (u & 63)) & 1;
+            };
             for (uint32_t c = gwave; c < g.nhugein; c += nwaves) {
                 const uint2 dsc = g.hugein[c];
                 const uint32_t end = min(g.cstart[dsc.x + 1], dsc.y + kPullChunk);
@@ -674,7 +688,7 @@ __global__ __launch_bounds__(BS) void k_pr_dd(PrDdArgs a)
                     const uint32_t e = e0 + ln;
                     if (e < end) {
                         const uint32_t u = g.csrc[e];
-                        if (test_bit(ab, u)) s += a.p[u];
+                        if (active(u)) s += a.p[u];
                     }
                 }
                 s = wave_sum_f64(s);
@@ -692,7 +706,7 @@ __global__ __launch_bounds__(BS) void k_pr_dd(PrDdArgs a)
                 }
                 const double s = wave_gather_sum(b, e, [&](uint32_t ed) {
                     const uint32_t u = g.csrc[ed];
-                    return test_bit(ab, u) ? a.p[u] : 0.0;
+                    return active(u) ? a.p[u] : 0.0;
                 });
                 dd_accumulate(R, own, v, s, false);
             }
