@@ -143,6 +143,12 @@ def test_pr_rmat16_block_sizes(sg, orc, gpu, bs):
     xd = e.gather(np.float32)
     ref, _, _ = orc.pagerank(hg.row, hg.col, D, 1e-13, 100000)
     assert rel_l1(xd, ref) <= 1e-6
+    # the ITERATIVE scheme (LDS bitmap copy, sampled priority) at this block size
+    e.pagerank(D, 1e-8, 0, sg.PR_ITERATIVE)
+    assert rel_l1(e.gather(np.float32), ref) <= 1e-6
+    e2 = sg.Engine(sg.Graph(hg), block_size=bs, force_variant="async_push_td", priority=2, cut_threshold=0.3)
+    e2.pagerank(D, 1e-8, 0, sg.PR_ITERATIVE)
+    assert rel_l1(e2.gather(np.float32), ref) <= 1e-6
 
 
 def test_pr_rmat20_directed(sg, orc, gpu):
