@@ -478,12 +478,6 @@ int sg_engine_get_stats(sg_engine *e, sg_stats *out);
  * the last one (variant -1) marking the end; *n_out gets the rows written. */
 int sg_engine_get_trace(sg_engine *e, sg_round_trace *out, int max, int *n_out);
 
-/* Diagnostics: grid configuration and barrier words of the engine's
- * persistent kernel configuration `index` (0-based). info[0] = blocks,
- * info[1] = block size, info[2] = XCDs used, info[3..18] = blocks per XCD;
- * words gets up to `nwords` barrier words. Synchronises. */
-int sg_engine_debug_grid(sg_engine *e, int index, uint32_t *info, uint32_t *words, int nwords);
-
 #ifdef __cplusplus
 }
 #endif

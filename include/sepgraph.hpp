@@ -201,6 +201,12 @@ struct PageRank {};  // samples/hybrid_pr: InitGraph(tolerance) (damping 0.85)
 struct BC {};        // samples/hybrid_bc: single-source BC, InitGraph(source)
 }  // namespace apps
 
+namespace detail {
+template <typename T>
+struct always_false : std::false_type {
+};
+}  // namespace detail
+
 namespace engine {
 
 template <typename TValue, typename TBuffer, typename TWeight, typename TApp, typename... InitArgs>
@@ -223,6 +229,15 @@ public:
 
     void LoadGraph(const graphs::CSRGraph &csr)
     {
+        // a second LoadGraph replaces the first graph and its engine
+        if (e_) {
+            sg_engine_destroy(e_);
+            e_ = nullptr;
+        }
+        if (g_) {
+            sg_graph_destroy(g_);
+            g_ = nullptr;
+        }
         csr_ = csr;
         const uint32_t *w = (!std::is_same<TWeight, graphs::NoWeight>::value && !csr_.edge_weights.empty())
                                 ? csr_.edge_weights.data()
@@ -241,15 +256,18 @@ public:
     {
         if (!e_) throw Error(SG_ERR_STATE, "sepgraph: Start before LoadGraph");
         if constexpr (std::is_same<TApp, apps::BFS>::value) {
-            check(sg_bfs(e_, (uint32_t)std::get<0>(init_)));
+            check(sg_bfs(e_, source()));
         } else if constexpr (std::is_same<TApp, apps::SSSP>::value) {
-            check(sg_sssp(e_, (uint32_t)std::get<0>(init_)));
+            check(sg_sssp(e_, source()));
         } else if constexpr (std::is_same<TApp, apps::PageRank>::value) {
             check(sg_pagerank(e_, 0.85, (double)std::get<0>(init_), 0,
                               algo_type_ == policy::AlgoType::ITERATIVE_SCHEME ? SG_PR_ITERATIVE : SG_PR_PULL_TD));
-        } else {
-            const uint32_t s = (uint32_t)std::get<0>(init_);
+        } else if constexpr (std::is_same<TApp, apps::BC>::value) {
+            const uint32_t s = source();
             check(sg_bc_dependency(e_, &s, 1, 1));
+        } else {
+            static_assert(detail::always_false<TApp>::value,
+                          "sepgraph::engine::Engine: TApp must be apps::BFS, apps::SSSP, apps::PageRank or apps::BC");
         }
         check(sg_engine_sync(e_));
         check(sg_engine_get_stats(e_, &stats_));
@@ -274,6 +292,16 @@ public:
     sg_engine *Handle() const { return e_; }
 
 private:
+    // The first InitGraph argument as a vertex id, clamped to [0, n - 1] as
+    // the reference's drivers clamp --source_node (hybrid_bfs.cu:180,
+    // hybrid_sssp.cu:158).
+    uint32_t source() const
+    {
+        const long long s = (long long)std::get<0>(init_);
+        const long long hi = csr_.nnodes ? (long long)csr_.nnodes - 1 : 0;
+        return (uint32_t)(s < 0 ? 0 : s > hi ? hi : s);
+    }
+
     policy::AlgoType algo_type_;
     common::EngineOptions options_;
     graphs::CSRGraph csr_;

@@ -262,7 +262,7 @@ __global__ SG_BFS_BOUNDS(BS) void k_bfs(BfsArgs a)
     const uint32_t nwaves = gridDim.x * WPB;
     const int ln = lane();
     const uint32_t xcc = xcc_id();
-    uint32_t gen = threadIdx.x == 0 ? grid_gen(a.bar) : 0u;
+    BarState gen = bar_begin(a.bar, xcc);
     const long long t_start = wall_clock64();
 
     __shared__ unsigned long long s_red[2][WPB];
@@ -506,13 +506,12 @@ int launch_bfs(sg_engine *e, BfsArgs &args, bool timed)
     };
     int rc = engine_persistent_grid(e, (const void *)k_bfs<BS>, BS, probe, &pg);
     if (rc) return rc;
-    args.bar.words = pg->words;
-    args.bar.nxcd = pg->nxcd;
-    for (int x = 0; x < kMaxXcd; x++) args.bar.cnt[x] = pg->cnt[x];
+    bind_barrier(pg, args.bar);
     return engine_launch_persistent(e, [&]() -> int {
         if (timed) SG_HIP(hipEventRecord(e->ev0, e->stream));
         hipLaunchKernelGGL(k_bfs<BS>, dim3(pg->blocks), dim3(BS), 0, e->stream, args);
         SG_HIP(hipGetLastError());
+        pg->launches++;
         if (timed) SG_HIP(hipEventRecord(e->ev1, e->stream));
         return SG_OK;
     });

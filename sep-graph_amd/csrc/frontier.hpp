@@ -143,11 +143,13 @@ __device__ __forceinline__ void wave_set_bits(uint64_t *bits, bool p, uint32_t v
 
 // wave_set_bits on `bits`, returning for each lane with p whether this call
 // set its bit (the bit was clear before): each run's atomicOr returns the old
-// word, broadcast to the run's lanes; of adjacent lanes with the same v only
-// the first claims it. Racing waves therefore claim a vertex once (an
-// unclaimed copy would enter the next queue again, and duplicates compound
-// over the rounds of a high-diameter traversal). bits2 gets every p lane's
-// bit with a non-returning atomicOr. Wave-convergent.
+// word, broadcast to the run's lanes; of the lanes of one run with the same v
+// (adjacent or not: two frontier vertices' concatenated lists can give
+// v = 5, 6, 5) only the first claims it, found from the run's exclusive
+// prefix OR. Racing waves therefore claim a vertex once (an unclaimed copy
+// would enter the next queue again, and duplicates compound over the rounds
+// of a high-diameter traversal). bits2 gets every p lane's bit with a
+// non-returning atomicOr. Wave-convergent.
 __device__ __forceinline__ bool wave_claim_bits(uint64_t *bits, bool p, uint32_t v, uint64_t *bits2 = nullptr)
 {
     if (!ballot(p)) return false;
@@ -177,9 +179,11 @@ __device__ __forceinline__ bool wave_claim_bits(uint64_t *bits, bool p, uint32_t
     // a lane's run ends at the first lane at or above it flagged last
     const uint64_t lm = ballot(last) >> ln;
     old = __shfl(old, ln + __builtin_ctzll(lm), kWave);
-    const uint32_t vp = __shfl_up(v, 1, kWave), wp = __shfl_up(w, 1, kWave);
-    const bool dup = ln > 0 && wp == w && vp == v;
-    return p && !(old & mine) && !dup;
+    // bits of the earlier lanes of this lane's run
+    const uint64_t mp = __shfl_up(m, 1, kWave);
+    const uint32_t rp = __shfl_up(run, 1, kWave);
+    const uint64_t before = (ln > 0 && rp == run) ? mp : 0ull;
+    return p && !(old & mine) && !(before & mine);
 }
 
 // out[v] = src[inv[v]] for v = t, t + nt, ... < n: a kernel epilogue putting a

@@ -190,6 +190,21 @@ extern "C" int sg_graph_create_ex(uint32_t n, uint64_t m, const uint32_t *row_st
     bool have_csc = col_start != nullptr;
     if (!undirected && have_csc && (!edge_src || (weights && !csc_weights)))
         SG_FAIL(SG_ERR_INVALID, "sg_graph_create: incomplete CSC");
+    if (!undirected && have_csc) {
+        // a caller's CSC is checked like the CSR before anything reads it,
+        // and must give every vertex the in-degree the CSR gives it
+        if (col_start[0] != 0 || col_start[n] != m)
+            SG_FAIL(SG_ERR_INVALID, "sg_graph_create: col_start[0] != 0 or col_start[n] != m");
+        for (uint32_t v = 0; v < n; v++)
+            if (col_start[v + 1] < col_start[v]) SG_FAIL(SG_ERR_INVALID, "sg_graph_create: col_start not monotone");
+        for (uint64_t e = 0; e < m; e++)
+            if (edge_src[e] >= n) SG_FAIL(SG_ERR_RANGE, "sg_graph_create: edge_src entry >= n");
+        std::vector<uint32_t> indeg(n, 0u);
+        for (uint64_t e = 0; e < m; e++) indeg[edge_dst[e]]++;
+        for (uint32_t v = 0; v < n; v++)
+            if (indeg[v] != col_start[v + 1] - col_start[v])
+                SG_FAIL(SG_ERR_INVALID, "sg_graph_create: CSC in-degrees differ from the CSR's");
+    }
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) {
         (void)hipGetLastError();
@@ -393,7 +408,7 @@ extern "C" int sg_engine_destroy(sg_engine *e)
     if (e->stream) (void)hipStreamSynchronize(e->stream);
     engine_forget_stream(e);
     void *ptrs[] = {e->value, e->bitmaps, e->queues, e->ctr,
-                    e->abort_flag, e->stats, e->trace, e->exits, e->aux, e->srcbuf, e->garena, e->work};
+                    e->abort_flag, e->stats, e->trace, e->aux, e->srcbuf, e->garena, e->work};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     for (int i = 0; i < e->ngrids; i++) (void)hipFree(e->grids[i].words);
@@ -460,8 +475,9 @@ void engine_forget_stream(sg_engine *e)
     if (g_pk_stream == e->stream) g_pk_stream = nullptr;
 }
 
-// Probe one grid size: true when every block was resident at once; fills the
-// per-XCD block counts the barrier needs.
+// Probe one grid size: true when every block was resident at once. The
+// barrier words are zeroed first, so the grid's launches start from clean
+// counters (every launch takes its own per-XCD census).
 static int probe_grid(sg_engine *e, sg_engine::PGrid &pg, bool *resident)
 {
     SG_HIP(hipMemsetAsync(pg.words, 0, kBarWords * 4, e->stream));
@@ -470,15 +486,11 @@ static int probe_grid(sg_engine *e, sg_engine::PGrid &pg, bool *resident)
     uint32_t counts[kMaxXcd + 2];
     SG_HIP(hipMemcpyAsync(counts, pg.words + bar_count(0), sizeof(counts), hipMemcpyDeviceToHost, e->stream));
     SG_HIP(hipStreamSynchronize(e->stream));
-    uint32_t nx = 0, total = 0;
-    for (int x = 0; x < kMaxXcd; x++) {
-        nx += counts[x] > 0;
-        total += counts[x];
-        pg.cnt[x] = counts[x];
-    }
+    uint32_t total = 0;
+    for (int x = 0; x < kMaxXcd; x++) total += counts[x];
     *resident = counts[kMaxXcd + 1] == 0 && total == (uint32_t)pg.blocks;
-    pg.nxcd = nx;
     // the probe's words share none with the barrier's counters, which stay zero
+    pg.launches = 0;
     return SG_OK;
 }
 
@@ -509,11 +521,16 @@ int engine_persistent_grid(sg_engine *e, const void *fn, int bs, const std::func
     e->launch_seq++;  // any persistent kernel may overwrite the BFS state
     if (e->needs_bar_reset) {
         e->bfs_token = 0;
-        // an aborted run leaves counters mid-way: zero and re-probe every grid
+        // an aborted run leaves counters mid-way: zero and re-probe every
+        // grid, and zero the concurrent-SSSP barrier areas
         for (int i = 0; i < e->ngrids; i++) {
             sg_engine::PGrid &pg = e->grids[i];
             int rc = size_grid(e, pg, std::max(1, pg.blocks / e->g->num_cus));
             if (rc) return rc;
+        }
+        if (e->garena && e->garena_bar_bytes) {
+            SG_HIP(hipMemsetAsync((char *)e->garena + e->garena_bar_off, 0, (size_t)e->garena_bar_bytes, e->stream));
+            e->group_launches = 0;
         }
         SG_HIP(hipMemsetAsync(e->abort_flag, 0, 4, e->stream));
         e->needs_bar_reset = false;
@@ -699,8 +716,6 @@ int engine_ensure_bfs_state(sg_engine *e)
     SG_HIP(hipMemsetAsync(e->abort_flag, 0, 4, e->stream));
     if ((rc = alloc((void **)&e->stats, sizeof(DevStats)))) return rc;
     if ((rc = alloc((void **)&e->trace, sizeof(DevTrace) * kTraceRows))) return rc;
-    if ((rc = alloc((void **)&e->exits, (uint64_t)kMaxBlocks * 16))) return rc;
-    SG_HIP(hipMemsetAsync(e->exits, 0, (size_t)kMaxBlocks * 16, e->stream));
     SG_HIP(hipMemsetAsync(e->stats, 0, sizeof(DevStats), e->stream));
     e->needs_bar_reset = false;
     int khz = 0;
@@ -715,6 +730,7 @@ extern "C" int sg_engine_sync(sg_engine *e)
 {
     if (!e) SG_FAIL(SG_ERR_INVALID, "sg_engine_sync: null engine");
     SG_HIP(hipStreamSynchronize(e->stream));
+    if (e->last_algo == 10) return sssp_group_status(e);
     if (e->stats) {
         DevStats s;
         SG_HIP(hipMemcpy(&s, e->stats, sizeof(s), hipMemcpyDeviceToHost));
@@ -763,6 +779,12 @@ extern "C" int sg_engine_get_stats(sg_engine *e, sg_stats *out)
     out->policy_trace = s.trace;
     out->max_depth = s.max_depth;
     out->status = s.status;
+    if (e->last_algo == 10 && e->group_status) {
+        // concurrent SSSP: the worst status over the call's queries
+        int32_t gs = 0;
+        SG_HIP(hipMemcpy(&gs, e->group_status, 4, hipMemcpyDeviceToHost));
+        out->status = gs;
+    }
     out->iterations = s.iterations;
     out->residual = s.residual;
     if ((e->last_algo == 0 || e->last_algo == 1) && e->value) {
@@ -812,27 +834,5 @@ extern "C" int sg_engine_get_trace(sg_engine *e, sg_round_trace *out, int max, i
         out[i].time_us = (double)t[i].ticks * us_per_tick;
     }
     *n_out = (int)rows;
-    return SG_OK;
-}
-
-extern "C" int sg_engine_debug_grid(sg_engine *e, int index, uint32_t *info, uint32_t *words, int nwords)
-{
-    if (!e || !info || index < 0) SG_FAIL(SG_ERR_INVALID, "sg_engine_debug_grid: bad argument");
-    if (index >= e->ngrids) SG_FAIL(SG_ERR_RANGE, "sg_engine_debug_grid: no such grid");
-    SG_HIP(hipStreamSynchronize(e->stream));
-    const sg_engine::PGrid &pg = e->grids[index];
-    info[0] = (uint32_t)pg.blocks;
-    info[1] = (uint32_t)pg.bs;
-    info[2] = pg.nxcd;
-    for (int x = 0; x < kMaxXcd; x++) info[3 + x] = pg.cnt[x];
-    if (words && nwords > 0) {
-        // barrier words, then the per-block exit records (4 words per block)
-        const int k = std::min(nwords, kBarWords);
-        SG_HIP(hipMemcpy(words, pg.words, (size_t)k * 4, hipMemcpyDeviceToHost));
-        if (nwords > kBarWords && e->exits) {
-            const int r = std::min(nwords - kBarWords, (int)kMaxBlocks * 4);
-            SG_HIP(hipMemcpy(words + kBarWords, e->exits, (size_t)r * 4, hipMemcpyDeviceToHost));
-        }
-    }
     return SG_OK;
 }

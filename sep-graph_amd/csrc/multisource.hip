@@ -298,7 +298,7 @@ __global__ __launch_bounds__(BS) void k_bc(BcArgs a)
     const uint32_t nwaves = gridDim.x * WPB;
     const int ln = lane();
     const uint32_t xcc = xcc_id();
-    uint32_t gen = threadIdx.x == 0 ? grid_gen(a.bar) : 0u;
+    BarState gen = bar_begin(a.bar, xcc);
     const long long t_start = wall_clock64();
     const uint64_t bm = a.nsrc >= 64 ? ~0ull : ((1ull << a.nsrc) - 1);
 
@@ -562,7 +562,7 @@ __global__ __launch_bounds__(BS) void k_msbfs(MsBfsArgs a)
     const uint32_t nwaves = gridDim.x * WPB;
     const int ln = lane();
     const uint32_t xcc = xcc_id();
-    uint32_t gen = threadIdx.x == 0 ? grid_gen(a.bar) : 0u;
+    BarState gen = bar_begin(a.bar, xcc);
     const long long t_start = wall_clock64();
     const uint64_t bm = a.nsrc >= 64 ? ~0ull : ((1ull << a.nsrc) - 1);
 
@@ -828,7 +828,7 @@ __global__ __launch_bounds__(BS) void k_mssssp(MsSsspArgs a)
     const uint32_t nwaves = gridDim.x * WPB;
     const int ln = lane();
     const uint32_t xcc = xcc_id();
-    uint32_t gen = threadIdx.x == 0 ? grid_gen(a.bar) : 0u;
+    BarState gen = bar_begin(a.bar, xcc);
     const long long t_start = wall_clock64();
     uint32_t qbits = 0;
     while ((1u << qbits) < a.nsrc) qbits++;
@@ -1011,12 +1011,11 @@ int launch_grid(sg_engine *e, const void *fn, int bs, Args &a)
     };
     int rc = engine_persistent_grid(e, fn, bs, probe, &pg);
     if (rc) return rc;
-    a.bar.words = pg->words;
-    a.bar.nxcd = pg->nxcd;
-    for (int x = 0; x < kMaxXcd; x++) a.bar.cnt[x] = pg->cnt[x];
+    bind_barrier(pg, a.bar);
     return engine_launch_persistent(e, [&]() -> int {
         void *params[] = {&a};
         SG_HIP(hipLaunchKernel(fn, dim3(pg->blocks), dim3(bs), params, 0, e->stream));
+        pg->launches++;
         return SG_OK;
     });
 }

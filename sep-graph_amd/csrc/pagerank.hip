@@ -170,7 +170,7 @@ __global__ __launch_bounds__(BS) void k_pr_pull(PrArgs a)
     const uint32_t nwaves = gridDim.x * WPB;
     const int ln = lane();
     const uint32_t xcc = xcc_id();
-    uint32_t gen = threadIdx.x == 0 ? grid_gen(a.bar) : 0u;
+    BarState gen = bar_begin(a.bar, xcc);
     const long long t_start = wall_clock64();
     const double d = a.damping;
     const double inv_n = 1.0 / (double)n;
@@ -483,7 +483,7 @@ __global__ __launch_bounds__(BS) void k_pr_dd(PrDdArgs a)
     const uint32_t nwaves = gridDim.x * WPB;
     const int ln = lane();
     const uint32_t xcc = xcc_id();
-    uint32_t gen = threadIdx.x == 0 ? grid_gen(a.bar) : 0u;
+    BarState gen = bar_begin(a.bar, xcc);
     const long long t_start = wall_clock64();
     const double d = a.damping, theta = a.theta;
     FrontierCtr *act_ctr = a.ctr;       // 2 slots
@@ -772,13 +772,12 @@ int launch_persistent(sg_engine *e, Args &args, int bs)
     int rc = engine_persistent_grid(e, (const void *)K, bs, probe, &pg);
     if (rc) return rc;
     if (pg->blocks > (int)kMaxBlocks) SG_FAIL(SG_ERR_STATE, "sg_pagerank: grid exceeds the partial-sum area");
-    args.bar.words = pg->words;
-    args.bar.nxcd = pg->nxcd;
-    for (int x = 0; x < kMaxXcd; x++) args.bar.cnt[x] = pg->cnt[x];
+    bind_barrier(pg, args.bar);
     return engine_launch_persistent(e, [&]() -> int {
         SG_HIP(hipEventRecord(e->ev0, e->stream));
         void *params[] = {&args};
         SG_HIP(hipLaunchKernel((const void *)K, dim3(pg->blocks), dim3(bs), params, 0, e->stream));
+        pg->launches++;
         SG_HIP(hipEventRecord(e->ev1, e->stream));
         return SG_OK;
     });

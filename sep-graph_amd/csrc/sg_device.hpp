@@ -92,13 +92,16 @@ __device__ __forceinline__ void wave_append(bool pred, uint32_t v, uint32_t *q, 
 // land before the set is used again two barriers on. No store has to
 // complete between the last arrival and the flip.
 //
-// The number of blocks each XCD receives for a given grid is measured once per
-// engine by a census kernel (dispatch is deterministic); a launch whose blocks
-// land differently cannot complete a group, runs into the bounded spin and
-// reports a timeout instead of hanging.
+// How many blocks each XCD holds is counted on every launch: each block
+// registers on its XCD when it starts (bar_begin), and before its first
+// barrier a block waits until the whole grid has registered and reads the
+// per-XCD counts (the census). Registration counters come in two sets chosen
+// by the launch's parity (GridBarrier::rset); block 0 zeroes the other set,
+// which only the previous launch on these words used, when it starts. So no
+// launch depends on how blocks were dispatched in an earlier one.
 constexpr int kBarLine = 32;      // uint32 per 128-byte line
 constexpr int kMaxXcd = 16;       // XCC_ID is 4 bits wide
-constexpr int kBarWords = kBarLine * (4 + 2 * kMaxXcd);
+constexpr int kBarWords = kBarLine * (6 + 4 * kMaxXcd);
 
 // Word offsets inside the barrier area.
 constexpr int kBarGen = 0;
@@ -109,12 +112,23 @@ __host__ __device__ constexpr int bar_count(int x) { return kBarLine * (3 + 2 * 
 // residency probe: blocks registered / blocks whose wait for the others expired
 constexpr int kBarProbeTotal = kBarLine * (3 + 2 * kMaxXcd) + kMaxXcd;
 constexpr int kBarProbeFail = kBarProbeTotal + 1;
+// per-launch registration: blocks on XCD x, and blocks in all, per parity set
+__host__ __device__ constexpr int bar_reg(int set, int x) { return kBarLine * (4 + 2 * kMaxXcd + set * kMaxXcd + x); }
+__host__ __device__ constexpr int bar_regtotal(int set) { return kBarLine * (4 + 4 * kMaxXcd + set); }
+static_assert(bar_regtotal(1) + kBarLine <= kBarWords, "barrier layout");
 
 struct GridBarrier {
     uint32_t *words;          // kBarWords, zero-initialised counters
-    uint32_t nxcd;            // XCDs that hold at least one block of the grid
     uint32_t spin_limit;      // max polls before giving up (sets *abort)
-    uint32_t cnt[kMaxXcd];    // blocks of the grid on each XCD (passed by value)
+    uint32_t rset;            // registration set of this launch (launch parity)
+};
+
+// Barrier state kept by thread 0 of each block.
+struct BarState {
+    uint32_t gen;   // generation the next barrier completes from
+    uint32_t cnt;   // blocks of the grid on this block's XCD (0 before the census)
+    uint32_t nxcd;  // XCDs holding at least one block of the grid
+    uint32_t rank;  // this block's arrival order among its XCD's blocks
 };
 
 __device__ __forceinline__ uint32_t xcc_id() { return __builtin_amdgcn_s_getreg((3 << 11) | 20) & 15; }
@@ -165,6 +179,64 @@ __device__ __forceinline__ bool spin_wait(const uint32_t *p, uint32_t v, bool wa
     }
 }
 
+// Start of a persistent kernel (thread 0's copy is the one used): read the
+// generation, register this block on its XCD, and let block 0 zero the
+// registration set of the other parity for the next launch.
+__device__ __forceinline__ BarState bar_begin(const GridBarrier &gb, uint32_t xcc)
+{
+    BarState s{0u, 0u, 0u, 0u};
+    if (threadIdx.x == 0) {
+        uint32_t *w = gb.words;
+        const int set = (int)(gb.rset & 1u);
+        s.gen = poll_load(w + kBarGen);
+        s.rank = atomic_add_relaxed(w + bar_reg(set, (int)xcc), 1u);
+        atomic_add_relaxed(w + bar_regtotal(set), 1u);
+        if (blockIdx.x == 0) {
+            for (int x = 0; x < kMaxXcd; x++) flag_store(w + bar_reg(set ^ 1, x), 0u);
+            flag_store(w + bar_regtotal(set ^ 1), 0u);
+        }
+    }
+    return s;
+}
+
+// The census (thread 0): bounded wait until every block of the grid has
+// registered, then the blocks per XCD (all[x], when given) and this block's
+// share. The total and the per-XCD counts are read together and accepted only
+// when they agree, so a registration whose two atomics land in either order
+// is never half seen.
+__device__ inline bool bar_census(const GridBarrier &gb, uint32_t xcc, BarState &s, uint32_t *abort,
+                                  uint32_t *all = nullptr)
+{
+    const uint32_t *w = gb.words;
+    const int set = (int)(gb.rset & 1u);
+    uint32_t polls = 0;
+    for (;;) {
+        uint32_t c[kMaxXcd];
+        const uint32_t tot = poll_load(w + bar_regtotal(set));
+#pragma unroll
+        for (int x = 0; x < kMaxXcd; x++) c[x] = poll_load(w + bar_reg(set, x));
+        uint32_t sum = 0, nx = 0;
+#pragma unroll
+        for (int x = 0; x < kMaxXcd; x++) {
+            sum += c[x];
+            nx += c[x] != 0u;
+        }
+        if (tot == gridDim.x && sum == gridDim.x) {
+            s.cnt = c[xcc];
+            s.nxcd = nx;
+            if (all)
+                for (int x = 0; x < kMaxXcd; x++) all[x] = c[x];
+            return true;
+        }
+        if ((++polls & 63) == 0 && poll_load(abort)) return false;
+        if (polls > gb.spin_limit) {
+            flag_store(abort, 1u);
+            return false;
+        }
+        __builtin_amdgcn_s_sleep(1);
+    }
+}
+
 // Residency probe, run by a persistent kernel launched with its real grid and
 // resources before first use: every block counts itself on its XCD and then
 // waits (bounded) until all blocks of the grid have done so. A block whose
@@ -190,17 +262,14 @@ __device__ inline void residency_probe(uint32_t *words, uint32_t limit)
     __syncthreads();
 }
 
-// Generation the barrier words are at when a kernel starts (thread 0 keeps it
-// in a register and advances it per barrier, so no barrier re-reads it).
-__device__ __forceinline__ uint32_t grid_gen(const GridBarrier &gb) { return poll_load(gb.words + kBarGen); }
-
 // Returns false if the wait timed out or another block aborted; the caller
-// must then leave its loop. `gen` is this block's generation (grid_gen at
-// kernel start), advanced on success; only thread 0's copy is used.
+// must then leave its loop. `st` is this block's state from bar_begin (only
+// thread 0's copy is used): the first barrier of a launch takes the census,
+// and each barrier advances the generation.
 // kStamp: the block that completes the barrier also records the wall clock
 // (barrier_stamp), so every block can time the grid's rounds identically.
 template <bool kStamp = false>
-__device__ __forceinline__ bool grid_sync(const GridBarrier &gb, uint32_t xcc, uint32_t &gen, uint32_t *abort)
+__device__ __forceinline__ bool grid_sync(const GridBarrier &gb, uint32_t xcc, BarState &st, uint32_t *abort)
 {
     __shared__ uint32_t s_ok;
     // __syncthreads() alone only waits for LDS traffic: every wave must first
@@ -210,32 +279,33 @@ __device__ __forceinline__ bool grid_sync(const GridBarrier &gb, uint32_t xcc, u
     __syncthreads();
     if (threadIdx.x == 0) {
         uint32_t *w = gb.words;
-        const uint32_t gen0 = gen;
-        gen = gen0 + 1;
-        const uint32_t cnt = gb.cnt[xcc];
-        const int set = (int)(gen0 & 1u);
-        uint32_t ok = 1;
-        uint32_t *arr = w + bar_arrive(set, (int)xcc);
-        uint32_t *top = w + bar_top(set);
-        const uint32_t a = atomic_add_relaxed(arr, 1u);
-        if (a == cnt - 1) {
-            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-            const uint32_t t = atomic_add_relaxed(top, 1u);
-            flag_store(arr, 0u);  // this set is used again two barriers on
-            if (t == gb.nxcd - 1) {
-                if (kStamp) {
-                    // the stamp must be visible before the flip that publishes it
-                    __hip_atomic_store((uint64_t *)(w + kBarStamp), (uint64_t)wall_clock64(), __ATOMIC_RELAXED,
-                                       __HIP_MEMORY_SCOPE_SYSTEM);
-                    __builtin_amdgcn_s_waitcnt(0);
+        uint32_t ok = st.cnt != 0u || bar_census(gb, xcc, st, abort);
+        if (ok) {
+            const uint32_t gen0 = st.gen;
+            st.gen = gen0 + 1;
+            const int set = (int)(gen0 & 1u);
+            uint32_t *arr = w + bar_arrive(set, (int)xcc);
+            uint32_t *top = w + bar_top(set);
+            const uint32_t a = atomic_add_relaxed(arr, 1u);
+            if (a == st.cnt - 1) {
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+                const uint32_t t = atomic_add_relaxed(top, 1u);
+                flag_store(arr, 0u);  // this set is used again two barriers on
+                if (t == st.nxcd - 1) {
+                    if (kStamp) {
+                        // the stamp must be visible before the flip that publishes it
+                        __hip_atomic_store((uint64_t *)(w + kBarStamp), (uint64_t)wall_clock64(), __ATOMIC_RELAXED,
+                                           __HIP_MEMORY_SCOPE_SYSTEM);
+                        __builtin_amdgcn_s_waitcnt(0);
+                    }
+                    flag_store(w + kBarGen, gen0 + 1);
+                    flag_store(top, 0u);
                 }
-                flag_store(w + kBarGen, gen0 + 1);
-                flag_store(top, 0u);
             }
+            // every block polls the generation word itself (a relay through a
+            // per-XCD flag costs one more store/poll hop on the critical path)
+            ok = spin_wait(w + kBarGen, gen0, true, gb.spin_limit, abort);
         }
-        // every block polls the generation word itself (a relay through a
-        // per-XCD flag costs one more store/poll hop on the critical path)
-        ok = spin_wait(w + kBarGen, gen0, true, gb.spin_limit, abort);
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
         __builtin_amdgcn_s_waitcnt(0);
         s_ok = ok;

@@ -119,7 +119,7 @@ struct DevStats {
 
 // Rows of the per-round device trace kept per engine.
 constexpr uint32_t kTraceRows = 4096;
-// Upper bound on persistent-grid blocks covered by the per-block exit records.
+// Upper bound on the blocks of a persistent grid (sg_engine_options.grid_blocks).
 constexpr uint32_t kMaxBlocks = 8192;
 
 struct DevTrace {
@@ -164,19 +164,20 @@ struct sg_engine {
     uint64_t work_cap = 0;
     void *garena = nullptr;           // per-query state of concurrent (one per XCD) SSSP
     uint64_t garena_bytes = 0;
-    int32_t *group_status = nullptr;  // worst status of the last concurrent launch
-    uint32_t *exits = nullptr;        // per-block exit records (diagnostics), kMaxBlocks x 4
+    uint64_t garena_bar_off = 0;      // its barrier areas (re-zeroed after an aborted run)
+    uint64_t garena_bar_bytes = 0;
+    uint64_t group_launches = 0;      // launches on those barrier areas (registration parity)
+    int32_t *group_status = nullptr;  // worst status over the last concurrent-SSSP call
     uint32_t trace_rows = 0;          // rows written by the last run (host copy)
     double wall_khz = 100000.0;       // device wall clock rate
     // Persistent-grid configurations (one per kernel and block size): grid
-    // size from occupancy, blocks per XCD from a census, barrier words.
+    // size from occupancy checked by a residency probe, barrier words.
     struct PGrid {
         const void *fn = nullptr;
         int bs = 0;
         int cap = 0;       // sg_engine_options.grid_blocks it was sized for
         int blocks = 0;
-        uint32_t nxcd = 0;
-        uint32_t cnt[sg::kMaxXcd] = {};   // blocks per XCD from the probe
+        uint64_t launches = 0;  // kernels run on `words` (their parity picks the registration set)
         uint32_t *words = nullptr;
         // launches the kernel in residency-probe mode with a given grid
         std::function<int(int blocks, uint32_t *words)> probe;
@@ -228,7 +229,8 @@ int ms_launch(sg_engine *e, int kind, const uint32_t *dev_src, uint32_t nsrc, vo
 // Up to one SSSP query per XCD in one launch (the XCD's blocks form the
 // query's grid); query i writes its distances to the device row out[i].
 int sssp_group_launch(sg_engine *e, const uint32_t *sources, int nq, uint32_t *const *out);
-// Worst status of the last such launch (synchronises the stream).
+// Worst status over the queries of the last concurrent-SSSP call
+// (synchronises the stream).
 int sssp_group_status(sg_engine *e);
 // One SSSP query over the whole grid (device source id) computing in the
 // device row `dist` (device order); a relabelled graph's result is gathered
@@ -251,6 +253,17 @@ int engine_alloc(void **p, uint64_t bytes);
 // counts from that probe. Barriers are re-zeroed after an aborted run.
 int engine_persistent_grid(sg_engine *e, const void *fn, int bs,
                            const std::function<int(int, uint32_t *)> &probe, sg_engine::PGrid **out);
+// Point a launch's barrier at the grid's words, with the registration set of
+// the grid's next launch; the caller counts the launch (pg->launches++) once
+// it has been enqueued.
+inline void bind_barrier(const sg_engine::PGrid *pg, GridBarrier &bar)
+{
+    bar.words = pg->words;
+    bar.rset = (uint32_t)(pg->launches & 1u);
+}
+// Concurrent SSSP: size its arena and reset the call's status word
+// (asynchronous), before the call's first sssp_group_launch.
+int sssp_group_begin(sg_engine *e);
 // Enqueue one persistent kernel (or residency probe) through `launch`, after
 // the device's previous persistent kernel when that was enqueued on another
 // stream. Persistent grids fill the GPU: two that overlap can each hold CUs

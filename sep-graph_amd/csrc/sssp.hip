@@ -40,11 +40,10 @@ constexpr uint32_t kInf = 0xFFFFFFFFu;
 constexpr int kPullSteps = SG_SSSP_PULL_STEPS;
 constexpr int kHubChunks = SG_SSSP_HUB_CHUNKS;
 
-// Table of a launch with one query per XCD (passed by value).
+// Table of a launch with one query per XCD (passed by value). The XCDs that
+// hold blocks in this launch take queries 0, 1, ... in XCD order.
 struct SsspGroups {
-    uint32_t *ticket;          // kMaxXcd counters zeroed before the launch: block ranks
-    int32_t *status;           // worst status over the queries (atomicMin), SG_OK before
-    uint32_t slot[kMaxXcd];    // query run by each XCD (>= nq: none)
+    int32_t *status;           // worst status over a call's queries (atomicMin), SG_OK before it
     uint32_t source[kMaxXcd];
     uint32_t *out[kMaxXcd];    // distance row of each query
     uint32_t nq;
@@ -67,7 +66,6 @@ struct SsspArgs {
     uint32_t total_workload0;
     uint32_t max_rounds;
     uint32_t probe;   // launch is a residency probe of the grid
-    uint32_t *exits;  // per-block exit records (diagnostics)
     uint32_t delta;   // two-level priority bucket width (0: off)
     // degree-ordered graphs: dist is in device order, and the kernel ends by
     // writing out[v] = dist[inv[v]] (the caller's order); null otherwise
@@ -315,8 +313,11 @@ __device__ __forceinline__ void pull_hub_chunks(Round &R, const uint2 (&dsc)[NC]
     }
 }
 
+// `gen`: the block's barrier state (from bar_begin, or the census of a
+// grouped launch); `brank` of `nblocks` ranks the block in the query's grid.
 template <int BS>
-__device__ __forceinline__ void sssp_run(const SsspArgs &a, uint32_t brank, uint32_t nblocks, uint32_t xcc)
+__device__ __forceinline__ void sssp_run(const SsspArgs &a, uint32_t brank, uint32_t nblocks, uint32_t xcc,
+                                         BarState gen)
 {
     constexpr int WPB = BS / kWave;
     constexpr uint32_t kLdsQ = 8u * BS;
@@ -327,7 +328,6 @@ __device__ __forceinline__ void sssp_run(const SsspArgs &a, uint32_t brank, uint
     const uint32_t gwave = brank * WPB + threadIdx.x / kWave;
     const uint32_t nwaves = nblocks * WPB;
     const int ln = lane();
-    uint32_t gen = threadIdx.x == 0 ? grid_gen(a.bar) : 0u;
     const long long t_start = wall_clock64();
 
     __shared__ unsigned long long s_red[2][WPB];
@@ -366,8 +366,6 @@ __device__ __forceinline__ void sssp_run(const SsspArgs &a, uint32_t brank, uint
     unsigned long long ptrace = 0;
     uint32_t pushes = 0, pulls = 0;
     uint32_t L = 0;
-    uint32_t reason = 3;  // 1: empty frontier, 2: round cap, 3: barrier failed
-    unsigned long long last_count = ~0ull;
     while (ok) {
         read_counts(&a.ctr[L % 3], s_lvl);
         if (threadIdx.x == 0) {
@@ -383,15 +381,8 @@ __device__ __forceinline__ void sssp_run(const SsspArgs &a, uint32_t brank, uint
             thr = (uint32_t)min(t, (uint64_t)kInf);
         }
         __syncthreads();
-        last_count = rc.count;
-        if (rc.count == 0) {
-            reason = 1;
-            break;
-        }
-        if (a.max_rounds && L >= a.max_rounds) {
-            reason = 2;
-            break;
-        }
+        if (rc.count == 0) break;
+        if (a.max_rounds && L >= a.max_rounds) break;
 
         const bool pull = pol.next(L, rc.count, rc.scout);
         if (pull) {
@@ -535,17 +526,14 @@ __device__ __forceinline__ void sssp_run(const SsspArgs &a, uint32_t brank, uint
     // the same barrier, which made the last round's distances visible
     if (ok && a.out) gather_caller_order(a.out, a.dist, a.inv, n, gtid, nthreads);
     if (gtid == 0) write_final(a.trace, a.stats, L, pushes, pulls, ptrace, ok, t_start);
-    if (a.exits && threadIdx.x == 0 && brank < kMaxBlocks) {
-        uint4 rec = make_uint4(L, reason, (uint32_t)min(last_count, 0xFFFFFFFFull), xcc);
-        *(uint4 *)&a.exits[brank * 4] = rec;
-    }
 }
 
-// One query over the whole grid, or (a.groups) one query per XCD: the blocks
-// an XCD holds form that query's grid, ranked by an arrival ticket, with a
-// barrier of their own. Concurrent single-source queries each keep their
-// working set in one XCD's L2 and synchronise only within it, which suits
-// SSSP, whose rounds are latency-bound at far below the bandwidth of the chip.
+// One query over the whole grid, or (a.grouped) one query per XCD: the
+// blocks an XCD holds in this launch form that query's grid, ranked by their
+// registration order, with a barrier of their own. Concurrent single-source
+// queries each keep their working set in one XCD's L2 and synchronise only
+// within it, which suits SSSP, whose rounds are latency-bound at far below
+// the bandwidth of the chip.
 template <int BS>
 __global__ __launch_bounds__(BS) void k_sssp(SsspArgs a)
 {
@@ -554,15 +542,33 @@ __global__ __launch_bounds__(BS) void k_sssp(SsspArgs a)
         return;
     }
     const uint32_t xcc = xcc_id();
+    BarState st = bar_begin(a.bar, xcc);
     if (!a.grouped) {
-        sssp_run<BS>(a, blockIdx.x, gridDim.x, xcc);
+        sssp_run<BS>(a, blockIdx.x, gridDim.x, xcc, st);
         return;
     }
-    __shared__ uint32_t s_rank;
-    if (threadIdx.x == 0) s_rank = atomicAdd(&a.gr.ticket[xcc], 1u);
+    // Grouped: the census of this launch decides which XCD runs which query.
+    __shared__ uint32_t s_q, s_rank, s_cnt, s_ok;
+    if (threadIdx.x == 0) {
+        uint32_t all[kMaxXcd];
+        const bool ok = bar_census(a.bar, xcc, st, a.abort, all);
+        uint32_t q = 0, held = 0;
+        for (uint32_t x = 0; x < (uint32_t)kMaxXcd; x++) {
+            q += (x < xcc && all[x] != 0u) ? 1u : 0u;
+            held += all[x] != 0u ? 1u : 0u;
+        }
+        // queries no XCD holds fail the call; the first block of the
+        // first XCD holding blocks reports them
+        if (ok && held < a.gr.nq && q == 0 && st.rank == 0) atomicMin(a.gr.status, (int32_t)SG_ERR_CAPACITY);
+        if (!ok) atomicMin(a.gr.status, (int32_t)SG_ERR_TIMEOUT);
+        s_q = q;
+        s_rank = st.rank;
+        s_cnt = st.cnt;
+        s_ok = ok;
+    }
     __syncthreads();
-    const uint32_t q = a.gr.slot[xcc];
-    if (q >= a.gr.nq) return;  // an XCD without a query this launch
+    const uint32_t q = s_q;
+    if (!s_ok || q >= a.gr.nq) return;  // no query for this XCD in this launch
     const size_t W = a.g.words;
     SsspArgs b = a;
     b.dist = a.gr.out[q];
@@ -571,12 +577,13 @@ __global__ __launch_bounds__(BS) void k_sssp(SsspArgs a)
     b.ctr = a.ctr + (size_t)q * 3;
     b.stats = a.stats + q;
     b.trace = nullptr;
-    b.exits = nullptr;
     b.source = a.gr.source[q];
-    // the XCD's own barrier area (after the whole-grid one)
+    // the XCD's own barrier area (after the whole-grid one), whose census is
+    // the one just taken: all s_cnt blocks of the XCD, on one XCD
     b.bar.words = a.bar.words + (size_t)(xcc + 1) * kBarWords;
-    b.bar.nxcd = 1;
-    sssp_run<BS>(b, s_rank, a.bar.cnt[xcc], xcc);
+    BarState xs{0u, s_cnt, 1u, s_rank};
+    if (threadIdx.x == 0) xs.gen = poll_load(b.bar.words + kBarGen);
+    sssp_run<BS>(b, s_rank, s_cnt, xcc, xs);
     if (s_rank == 0 && threadIdx.x == 0) atomicMin(a.gr.status, b.stats->status);
 }
 
@@ -587,7 +594,7 @@ template <int BS>
 int launch_sssp(sg_engine *e, SsspArgs &args, uint32_t *group_words)
 {
     if (!args.abort || !args.stats || (!args.dist && !args.grouped) || !args.fbit || !args.queue ||
-        !args.ctr || (args.grouped && (!group_words || !args.gr.ticket || !args.gr.status)))
+        !args.ctr || (args.grouped && (!group_words || !args.gr.status)))
         SG_FAIL(SG_ERR_STATE, "sg_sssp: launch with unset device state");
     sg_engine::PGrid *pg = nullptr;
     auto probe = [e, args](int blocks, uint32_t *words) -> int {
@@ -601,22 +608,17 @@ int launch_sssp(sg_engine *e, SsspArgs &args, uint32_t *group_words)
     };
     int rc = engine_persistent_grid(e, (const void *)k_sssp<BS>, BS, probe, &pg);
     if (rc) return rc;
-    args.bar.nxcd = pg->nxcd;
-    for (int x = 0; x < kMaxXcd; x++) args.bar.cnt[x] = pg->cnt[x];
     if (args.grouped) {
         args.bar.words = group_words;
-        uint32_t q = 0;
-        for (int x = 0; x < kMaxXcd; x++) {
-            args.gr.slot[x] = kMaxXcd;
-            if (pg->cnt[x] && q < args.gr.nq) args.gr.slot[x] = q++;
-        }
-        if (q < args.gr.nq) SG_FAIL(SG_ERR_CAPACITY, "concurrent SSSP: more queries than XCDs holding blocks");
+        args.bar.rset = (uint32_t)(e->group_launches & 1u);
     } else {
-        args.bar.words = pg->words;
+        bind_barrier(pg, args.bar);
     }
     return engine_launch_persistent(e, [&]() -> int {
         hipLaunchKernelGGL(k_sssp<BS>, dim3(pg->blocks), dim3(BS), 0, e->stream, args);
         SG_HIP(hipGetLastError());
+        if (args.grouped) e->group_launches++;
+        else pg->launches++;
         return SG_OK;
     });
 }
@@ -665,62 +667,98 @@ int sssp_args(sg_engine *e, SsspArgs &a)
     return SG_OK;
 }
 
+// Working state of concurrent SSSP (one query per XCD): per-query bitmaps,
+// queues, counters and stats, the barrier areas (the whole grid's, then one
+// per XCD) and the call's status word. Offsets into the engine's garena.
+struct GroupLayout {
+    uint64_t fbit, queue, ctr, stats, bar, status, total;
+};
+
+GroupLayout group_layout(const sg_engine *e)
+{
+    const uint64_t W = e->g->dev.words, qc = e->qcap;
+    auto al = [](uint64_t x) { return (x + 255) / 256 * 256; };
+    GroupLayout L;
+    L.fbit = 0;
+    L.queue = L.fbit + al(kMaxXcd * 3 * W * 8);
+    L.ctr = L.queue + al(kMaxXcd * 3 * qc * 4);
+    L.stats = L.ctr + al(kMaxXcd * 3 * sizeof(FrontierCtr));
+    L.bar = L.stats + al(kMaxXcd * sizeof(DevStats));
+    L.status = L.bar + al((uint64_t)(1 + kMaxXcd) * kBarWords * 4);
+    L.total = L.status + 256;
+    return L;
+}
+
+__global__ void k_set_status(int32_t *status, int32_t v)
+{
+    if (threadIdx.x == 0 && blockIdx.x == 0) *status = v;
+}
+
 }  // namespace
 
 namespace sg {
 
-// Up to one SSSP query per XCD in one launch (nq <= XCDs holding blocks);
-// query i writes distances to out[i] (device rows of n). Asynchronous; the
-// worst status lands in the engine's group status word.
-int sssp_group_launch(sg_engine *e, const uint32_t *sources, int nq, uint32_t *const *out)
+// Size the concurrent-SSSP arena and set the call's status word to SG_OK
+// (by a kernel, in order on the stream). Every launch of the call folds the
+// status of each of its queries into that word, so a failure in any launch
+// of a multi-launch call stays visible until the next call.
+int sssp_group_begin(sg_engine *e)
 {
-    if (nq < 1 || nq > kMaxXcd) SG_FAIL(SG_ERR_RANGE, "concurrent SSSP: 1..16 queries per launch");
-    const DevGraph &d = e->g->dev;
-    for (int i = 0; i < nq; i++)
-        if (sources[i] >= d.n) SG_FAIL(SG_ERR_RANGE, "concurrent SSSP: source >= n");
     int rc = engine_ensure_bfs_state(e);
     if (rc) return rc;
-    const uint64_t W = d.words, qc = e->qcap;
-    auto al = [](uint64_t x) { return (x + 255) / 256 * 256; };
-    const uint64_t o_fbit = 0;
-    const uint64_t o_queue = o_fbit + al(kMaxXcd * 3 * W * 8);
-    const uint64_t o_ctr = o_queue + al(kMaxXcd * 3 * qc * 4);
-    const uint64_t o_stats = o_ctr + al(kMaxXcd * 3 * sizeof(FrontierCtr));
-    const uint64_t o_bar = o_stats + al(kMaxXcd * sizeof(DevStats));
-    const uint64_t o_tick = o_bar + al((uint64_t)(1 + kMaxXcd) * kBarWords * 4);
-    const uint64_t total = o_tick + 256;
-    if (!e->garena || e->garena_bytes < total) {
+    const GroupLayout L = group_layout(e);
+    if (!e->garena || e->garena_bytes < L.total) {
         if (e->garena) {
             SG_HIP(hipStreamSynchronize(e->stream));
             (void)hipFree(e->garena);
             e->garena = nullptr;
+            e->garena_bytes = 0;
         }
-        if ((rc = engine_alloc(&e->garena, total))) return rc;
-        e->garena_bytes = total;
-        // barrier counters start at zero and stay consistent across launches
-        SG_HIP(hipMemsetAsync((char *)e->garena + o_bar, 0, (size_t)(total - o_bar), e->stream));
+        if ((rc = engine_alloc(&e->garena, L.total))) return rc;
+        e->garena_bytes = L.total;
+        // barrier counters start at zero; the engine re-zeroes them after an
+        // aborted run (engine_persistent_grid)
+        SG_HIP(hipMemsetAsync((char *)e->garena + L.bar, 0, (size_t)(L.status - L.bar), e->stream));
+        e->garena_bar_off = L.bar;
+        e->garena_bar_bytes = L.status - L.bar;
+        e->group_launches = 0;
     }
+    e->group_status = (int32_t *)((char *)e->garena + L.status);
+    hipLaunchKernelGGL(k_set_status, dim3(1), dim3(64), 0, e->stream, e->group_status, (int32_t)SG_OK);
+    SG_HIP(hipGetLastError());
+    return SG_OK;
+}
+
+// Up to one SSSP query per XCD in one launch; query i writes distances to
+// out[i] (device rows of n). The XCDs holding blocks in the launch take the
+// queries in XCD order; queries beyond their number fail the call with
+// SG_ERR_CAPACITY. Asynchronous; sssp_group_begin must have run first.
+int sssp_group_launch(sg_engine *e, const uint32_t *sources, int nq, uint32_t *const *out)
+{
+    if (nq < 1 || nq > kMaxXcd) SG_FAIL(SG_ERR_RANGE, "concurrent SSSP: 1..16 queries per launch");
+    if (!e->garena || !e->group_status) SG_FAIL(SG_ERR_STATE, "concurrent SSSP: arena not prepared");
+    const DevGraph &d = e->g->dev;
+    for (int i = 0; i < nq; i++)
+        if (sources[i] >= d.n) SG_FAIL(SG_ERR_RANGE, "concurrent SSSP: source >= n");
+    int rc;
+    const GroupLayout L = group_layout(e);
     char *base = (char *)e->garena;
     SsspArgs a;
     if ((rc = sssp_args(e, a))) return rc;
-    a.fbit = (uint64_t *)(base + o_fbit);
-    a.queue = (uint32_t *)(base + o_queue);
+    a.fbit = (uint64_t *)(base + L.fbit);
+    a.queue = (uint32_t *)(base + L.queue);
     a.qcap = e->qcap;
-    a.ctr = (FrontierCtr *)(base + o_ctr);
-    a.stats = (DevStats *)(base + o_stats);
+    a.ctr = (FrontierCtr *)(base + L.ctr);
+    a.stats = (DevStats *)(base + L.stats);
     a.grouped = 1;
-    a.gr.ticket = (uint32_t *)(base + o_tick);
-    a.gr.status = (int32_t *)(base + o_tick + 128);
+    a.gr.status = e->group_status;
     a.gr.nq = (uint32_t)nq;
     for (int i = 0; i < nq; i++) {
         a.gr.source[i] = sources[i];
         a.gr.out[i] = out[i];
     }
-    // tickets and the status word are zeroed in order on the stream
-    SG_HIP(hipMemsetAsync(base + o_tick, 0, 256, e->stream));
-    e->last_algo = 6;
-    e->group_status = a.gr.status;
-    return launch_sssp_bs(e, a, (uint32_t *)(base + o_bar));
+    e->last_algo = 10;  // concurrent SSSP (status in group_status)
+    return launch_sssp_bs(e, a, (uint32_t *)(base + L.bar));
 }
 
 // One SSSP query over the whole grid from device vertex `source`, with its
@@ -747,13 +785,13 @@ int sssp_single_launch(sg_engine *e, uint32_t source, uint32_t *dist, uint32_t *
     a.ctr = (FrontierCtr *)e->ctr;
     a.stats = e->stats;
     a.trace = nullptr;
-    a.exits = nullptr;
     a.source = source;
     e->last_algo = 1;
     return launch_sssp_bs(e, a, nullptr);
 }
 
-// Worst status of the last grouped launch (synchronises the stream).
+// Worst status over the queries of the last concurrent-SSSP call
+// (synchronises the stream).
 int sssp_group_status(sg_engine *e)
 {
     if (!e->group_status) return SG_OK;
@@ -762,7 +800,8 @@ int sssp_group_status(sg_engine *e)
     SG_HIP(hipMemcpy(&st, e->group_status, 4, hipMemcpyDeviceToHost));
     if (st != SG_OK) {
         e->needs_bar_reset = true;
-        SG_FAIL(st, "concurrent SSSP: a query reported an error");
+        SG_FAIL(st, st == SG_ERR_CAPACITY ? "concurrent SSSP: more queries than XCDs holding blocks"
+                                          : "concurrent SSSP: a query reported an error");
     }
     return SG_OK;
 }
@@ -788,7 +827,6 @@ extern "C" int sg_sssp(sg_engine *e, uint32_t source)
     a.ctr = (FrontierCtr *)e->ctr;
     a.stats = e->stats;
     a.trace = e->trace;
-    a.exits = e->exits;
     a.source = dev_vertex(e->g, source);
     if (e->g->relabeled) {
         // the kernel gathers the result into the value array itself
@@ -819,6 +857,7 @@ extern "C" int sg_sssp_concurrent(sg_engine *e, const uint32_t *sources, int32_t
     // one query per XCD: MI355X has 8 (a launch on fewer XCDs reports
     // SG_ERR_CAPACITY)
     const int per = 8;
+    if ((rc = sssp_group_begin(e))) return rc;
     SG_HIP(hipEventRecord(e->ev0, e->stream));
     for (int32_t i = 0; i < k; i += per) {
         const int nq = std::min<int32_t>(per, k - i);

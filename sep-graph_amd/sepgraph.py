@@ -164,7 +164,6 @@ def lib() -> C.CDLL:
             "sg_engine_gather_value": ([_P, _P, C.c_uint64], C.c_int),
             "sg_engine_get_stats": ([_P, C.POINTER(Stats)], C.c_int),
             "sg_engine_get_trace": ([_P, C.POINTER(RoundTrace), C.c_int, C.POINTER(C.c_int)], C.c_int),
-            "sg_engine_debug_grid": ([_P, C.c_int, _P, _P, C.c_int], C.c_int),
         }
         for name, (args, res) in sig.items():
             f = getattr(L, name)
@@ -497,12 +496,6 @@ class Engine:
              "in_edges": r.in_edges, "time_us": r.time_us}
             for r in rows[: nrow.value]
         ]
-
-    def debug_grid(self, index: int = 0, nwords: int = 1152):
-        info = np.zeros(19, np.uint32)
-        words = np.zeros(nwords, np.uint32)
-        _check(lib().sg_engine_debug_grid(self.h, index, _ptr(info), _ptr(words), nwords))
-        return info, words
 
     def close(self):
         if self.h:

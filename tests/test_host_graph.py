@@ -163,3 +163,41 @@ def test_graph_create_validates_before_device(sg):
     bad = np.array([0, 2, 1], np.uint32)
     rc = L.sg_graph_create(2, 1, bad.ctypes.data_as(C.c_void_p), col.ctypes.data_as(C.c_void_p), None, None, None, None, 0, C.byref(h))
     assert rc == -1
+
+
+def _create_with_csc(sg, n, row, col, cs, src, undirected=False):
+    import ctypes as C
+
+    h = C.c_void_p()
+    rc = sg.lib().sg_graph_create_ex(n, len(col), sg._ptr(row), sg._ptr(col), None, sg._ptr(cs), sg._ptr(src),
+                                     None, 1 if undirected else 0, C.byref(h))
+    if rc == 0:
+        sg.lib().sg_graph_destroy(h)
+    return rc
+
+
+def test_malformed_csc_is_rejected_before_use(sg):
+    """A caller's CSC is validated like the CSR (no device needed: the checks
+    run before the library looks for a GPU)."""
+    n = 4
+    # edges 0->1, 0->2, 1->2, 2->3
+    row = np.array([0, 2, 3, 4, 4], np.uint32)
+    col = np.array([1, 2, 2, 3], np.uint32)
+    cs = np.array([0, 0, 1, 3, 4], np.uint32)
+    src = np.array([0, 0, 1, 2], np.uint32)
+    ERR_INVALID, ERR_RANGE = -1, -2
+    bad_first = cs.copy()
+    bad_first[0] = 1
+    assert _create_with_csc(sg, n, row, col, bad_first, src) == ERR_INVALID
+    bad_last = cs.copy()
+    bad_last[n] = 3
+    assert _create_with_csc(sg, n, row, col, bad_last, src) == ERR_INVALID
+    not_monotone = np.array([0, 2, 1, 3, 4], np.uint32)
+    assert _create_with_csc(sg, n, row, col, not_monotone, src) == ERR_INVALID
+    src_range = np.array([0, 0, 1, 9], np.uint32)
+    assert _create_with_csc(sg, n, row, col, cs, src_range) == ERR_RANGE
+    # right shape, but vertex 1 gets two in-edges and vertex 2 one
+    wrong_degrees = np.array([0, 0, 2, 3, 4], np.uint32)
+    assert _create_with_csc(sg, n, row, col, wrong_degrees, src) == ERR_INVALID
+    # the checks apply to directed graphs only (an undirected CSC aliases the CSR)
+    assert sg.lib().sg_last_error() is not None
