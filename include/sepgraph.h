@@ -197,6 +197,27 @@ int sg_graph_create_ex(uint32_t n, uint64_t m, const uint32_t *row_start, const 
                        const uint32_t *weights, const uint32_t *col_start, const uint32_t *edge_src,
                        const uint32_t *csc_weights, uint32_t flags, sg_graph **out);
 int sg_graph_destroy(sg_graph *g);
+
+/* Device arrays of a graph, for kernels compiled outside the library (the
+ * generic engine of sepgraph_app.hpp; the reference passes its
+ * groute::graphs::dev::CSRGraph / CSCGraph to app kernels by value,
+ * csr_graph.h:745-871). Pointers stay valid until sg_graph_destroy; the
+ * arrays must not be written. Vertex ids are device ids, which equal the
+ * caller's ids unless `relabeled` is set (create with SG_GRAPH_NO_RELABEL to
+ * keep them). CSC arrays alias the CSR ones for an undirected graph; weight
+ * arrays are NULL for an unweighted graph. */
+typedef struct sg_graph_view {
+    uint32_t n;
+    uint32_t relabeled;
+    uint64_t m;
+    const uint32_t *row_start;     /* [n + 1] */
+    const uint32_t *edge_dst;      /* [m] */
+    const uint32_t *weights;       /* [m] or NULL */
+    const uint32_t *col_start;     /* [n + 1] */
+    const uint32_t *edge_src;      /* [m] */
+    const uint32_t *csc_weights;   /* [m] or NULL */
+} sg_graph_view;
+int sg_graph_device_view(const sg_graph *g, sg_graph_view *out);
 /* n, m, and bytes of device memory held by the graph. */
 int sg_graph_info(const sg_graph *g, uint32_t *n, uint64_t *m, uint64_t *device_bytes);
 
@@ -272,6 +293,14 @@ int sg_engine_create(sg_graph *g, sg_engine **out);
 int sg_engine_destroy(sg_engine *e);
 int sg_engine_set_stream(sg_engine *e, void *hip_stream);
 int sg_engine_set_options(sg_engine *e, const sg_engine_options *opt);
+/* The HIP stream the engine's runs are enqueued on. */
+int sg_engine_stream(sg_engine *e, void **hip_stream);
+/* Order a kernel that the caller is about to enqueue on the engine's stream
+ * and that needs the whole GPU (a persistent or cooperative grid) after every
+ * persistent kernel the library enqueued before, on any engine, and before
+ * the ones it enqueues later. No counterpart in the reference (which has one
+ * stream per engine and no persistent kernels outside its own). */
+int sg_engine_order_launch(sg_engine *e);
 
 /* BFS from `source` (hybrid_bfs, samples/hybrid_bfs/hybrid_bfs.cu:173-243):
  * int32 levels, direction chosen per round by the reference's alpha/beta rule

@@ -2,13 +2,13 @@
 // of libsepgraph.so, so a hybrid_* sample driver keeps its structure:
 //
 //   typedef sepgraph::engine::Engine<level_t, level_t, sepgraph::graphs::NoWeight,
-//                                    sepgraph::apps::BFS, index_t> HybridEngine;
+//                                    sepgraph::apps::BFS, index_t, bool> HybridEngine;
 //   HybridEngine engine(sepgraph::policy::AlgoType::TRAVERSAL_SCHEME);
 //   engine.LoadGraph(csr);                   // reference: LoadGraph() from --graphfile
 //   sepgraph::common::EngineOptions opt;
 //   opt.ForceVariant(sepgraph::common::AlgoVariant::ASYNC_PUSH_DD);
 //   engine.SetOptions(opt);
-//   engine.InitGraph(source);
+//   engine.InitGraph(source, non_atomic);
 //   engine.Start();
 //   const std::vector<level_t> &levels = engine.GatherValue();
 //
@@ -19,10 +19,12 @@
 //   sepgraph::engine::Engine<TValue, TBuffer, TWeight, TApp, InitArgs...>
 //       SetOptions / LoadGraph / InitGraph / Start / PrintInfo / GatherValue /
 //       CSRGraph                                include/framework/framework.cuh:56-397
-// The apps are tags: the per-edge hooks of BFS, SSSP, PageRank and BC
-// (samples/hybrid_*/hybrid_*.cu) are compiled into the library's kernels, so
-// an app is selected, not re-implemented. Errors throw sepgraph::Error
-// carrying the SG_ERR_* code instead of exiting the process.
+// The built-in apps (sepgraph::apps) select the library's kernels, whose
+// per-edge hooks are those of samples/hybrid_{bfs,sssp,pr,bc}/hybrid_*.cu.
+// An app written against sepgraph::api::AppBase (the reference's hooks,
+// api.cuh:22-120) runs on a generic engine: include sepgraph_app.hpp and
+// compile the driver with hipcc. Errors throw sepgraph::Error carrying the
+// SG_ERR_* code instead of exiting the process.
 #pragma once
 
 #include <cstdint>
@@ -195,23 +197,70 @@ inline void GenWeights(CSRGraph &g, int32_t weight_range = 100)
 }  // namespace graphs
 
 namespace apps {
-struct BFS {};       // samples/hybrid_bfs: levels from a source, InitGraph(source)
-struct SSSP {};      // samples/hybrid_sssp: distances from a source, InitGraph(source)
-struct PageRank {};  // samples/hybrid_pr: InitGraph(tolerance) (damping 0.85)
-struct BC {};        // samples/hybrid_bc: single-source BC, InitGraph(source)
+// The library's built-in apps, selected rather than re-implemented: the
+// per-edge hooks of samples/hybrid_{bfs,sssp,pr,bc}/hybrid_*.cu are compiled
+// into its kernels. Templates with the reference's app parameters, so that
+// Engine<level_t, level_t, NoWeight, apps::BFS, index_t> reads as the
+// reference's Engine<..., hybrid_bfs::BFS, index_t, bool>.
+template <typename TValue, typename TBuffer, typename TWeight, typename... InitArgs>
+struct BFS {};       // levels from a source: InitGraph(source[, non_atomic])
+template <typename TValue, typename TBuffer, typename TWeight, typename... InitArgs>
+struct SSSP {};      // distances from a source: InitGraph(source[, ...])
+template <typename TValue, typename TBuffer, typename TWeight, typename... InitArgs>
+struct PageRank {};  // ranks: InitGraph(tolerance) (damping 0.85)
+template <typename TValue, typename TBuffer, typename TWeight, typename... InitArgs>
+struct BC {};        // single-source BC dependency: InitGraph(source)
 }  // namespace apps
+
+namespace api {
+// The reference's app base (include/framework/variants/api.cuh:22-120), with
+// device hooks: defined in sepgraph_app.hpp, which needs hipcc.
+template <typename TValue, typename TBuffer, typename TWeight>
+struct AppBase;
+}  // namespace api
 
 namespace detail {
 template <typename T>
 struct always_false : std::false_type {
 };
+
+template <template <typename, typename, typename, typename...> class A, template <typename, typename, typename, typename...> class B>
+struct same_app : std::false_type {
+};
+template <template <typename, typename, typename, typename...> class A>
+struct same_app<A, A> : std::true_type {
+};
+
+// Runs an app derived from api::AppBase on the generic persistent engine;
+// defined in sepgraph_app.hpp (compile the driver with hipcc and include it).
+template <typename TApp, typename TValue, typename TBuffer, typename TWeight>
+struct AppRunner;
 }  // namespace detail
 
 namespace engine {
 
-template <typename TValue, typename TBuffer, typename TWeight, typename TApp, typename... InitArgs>
+// The reference's Engine (include/framework/framework.cuh:56-397):
+// TAppImpl<TValue, TBuffer, TWeight, UnusedData...> is the app; UnusedData are
+// its constructor arguments, given to InitGraph. The built-in apps::* run the
+// library's kernels through the C ABI; an app derived from api::AppBase runs
+// its own hooks on the generic engine of sepgraph_app.hpp.
+template <typename TValue, typename TBuffer, typename TWeight,
+          template <typename, typename, typename, typename...> class TAppImpl, typename... UnusedData>
 class Engine {
+    typedef TAppImpl<TValue, TBuffer, TWeight, UnusedData...> AppImplDeviceObject;
+    static constexpr bool kBfs = detail::same_app<TAppImpl, apps::BFS>::value;
+    static constexpr bool kSssp = detail::same_app<TAppImpl, apps::SSSP>::value;
+    static constexpr bool kPr = detail::same_app<TAppImpl, apps::PageRank>::value;
+    static constexpr bool kBc = detail::same_app<TAppImpl, apps::BC>::value;
+    static constexpr bool kBuiltin = kBfs || kSssp || kPr || kBc;
+
 public:
+    // Graph sizes (the reference's GraphDatum nnodes / nedges).
+    struct GraphDatum {
+        uint32_t nnodes = 0;
+        uint64_t nedges = 0;
+    };
+
     explicit Engine(policy::AlgoType algo_type) : algo_type_(algo_type) {}
     Engine(const Engine &) = delete;
     Engine &operator=(const Engine &) = delete;
@@ -239,35 +288,48 @@ public:
             g_ = nullptr;
         }
         csr_ = csr;
+        datum_.nnodes = csr_.nnodes;
+        datum_.nedges = csr_.nedges;
         const uint32_t *w = (!std::is_same<TWeight, graphs::NoWeight>::value && !csr_.edge_weights.empty())
                                 ? csr_.edge_weights.data()
                                 : nullptr;
-        check(sg_graph_create(csr_.nnodes, csr_.nedges, csr_.row_start.data(), csr_.edge_dst.data(), w, nullptr,
-                              nullptr, nullptr, csr_.undirected ? 1 : 0, &g_));
+        // an app's own hooks see vertex ids, so its graph keeps the caller's order
+        const uint32_t flags = (csr_.undirected ? (uint32_t)SG_GRAPH_UNDIRECTED : 0u) |
+                               (kBuiltin ? 0u : (uint32_t)SG_GRAPH_NO_RELABEL);
+        check(sg_graph_create_ex(csr_.nnodes, csr_.nedges, csr_.row_start.data(), csr_.edge_dst.data(), w, nullptr,
+                                 nullptr, nullptr, flags, &g_));
         check(sg_engine_create(g_, &e_));
         check(sg_engine_set_options(e_, &options_.c_options()));
     }
 
-    // The app's initial state (source vertex for BFS, SSSP and BC; the
-    // convergence tolerance for PageRank).
-    void InitGraph(InitArgs... args) { init_ = std::make_tuple(args...); }
+    // The app's initial state: the constructor arguments of the app (the
+    // source vertex for BFS, SSSP and BC; the tolerance for PageRank).
+    void InitGraph(UnusedData... args) { init_ = std::make_tuple(args...); }
 
     void Start()
     {
         if (!e_) throw Error(SG_ERR_STATE, "sepgraph: Start before LoadGraph");
-        if constexpr (std::is_same<TApp, apps::BFS>::value) {
+        if constexpr (kBfs) {
             check(sg_bfs(e_, source()));
-        } else if constexpr (std::is_same<TApp, apps::SSSP>::value) {
+        } else if constexpr (kSssp) {
             check(sg_sssp(e_, source()));
-        } else if constexpr (std::is_same<TApp, apps::PageRank>::value) {
+        } else if constexpr (kPr) {
             check(sg_pagerank(e_, 0.85, (double)std::get<0>(init_), 0,
                               algo_type_ == policy::AlgoType::ITERATIVE_SCHEME ? SG_PR_ITERATIVE : SG_PR_PULL_TD));
-        } else if constexpr (std::is_same<TApp, apps::BC>::value) {
+        } else if constexpr (kBc) {
             const uint32_t s = source();
             check(sg_bc_dependency(e_, &s, 1, 1));
+        } else if constexpr (std::is_base_of<api::AppBase<TValue, TBuffer, TWeight>, AppImplDeviceObject>::value) {
+            const AppImplDeviceObject app = std::make_from_tuple<AppImplDeviceObject>(init_);
+            stats_ = detail::AppRunner<AppImplDeviceObject, TValue, TBuffer, TWeight>::Start(
+                e_, g_, app, algo_type_ == policy::AlgoType::ITERATIVE_SCHEME, options_.c_options(), value_,
+                buffer_);
+            gathered_ = true;
+            return;
         } else {
-            static_assert(detail::always_false<TApp>::value,
-                          "sepgraph::engine::Engine: TApp must be apps::BFS, apps::SSSP, apps::PageRank or apps::BC");
+            static_assert(detail::always_false<AppImplDeviceObject>::value,
+                          "sepgraph::engine::Engine: the app must be one of apps::BFS, apps::SSSP, apps::PageRank, "
+                          "apps::BC, or derive from sepgraph::api::AppBase (include sepgraph_app.hpp)");
         }
         check(sg_engine_sync(e_));
         check(sg_engine_get_stats(e_, &stats_));
@@ -281,13 +343,25 @@ public:
 
     const std::vector<TValue> &GatherValue()
     {
-        static_assert(sizeof(TValue) == 4, "values are 32-bit");
-        value_.resize(csr_.nnodes);
-        check(sg_engine_gather_value(e_, value_.data(), (uint64_t)value_.size() * sizeof(TValue)));
+        if constexpr (kBuiltin) {
+            static_assert(sizeof(TValue) == 4, "values are 32-bit");
+            value_.resize(csr_.nnodes);
+            check(sg_engine_gather_value(e_, value_.data(), (uint64_t)value_.size() * sizeof(TValue)));
+        } else {
+            if (!gathered_) throw Error(SG_ERR_STATE, "sepgraph: GatherValue before Start");
+        }
         return value_;
     }
 
+    // The app's buffers after the run (apps derived from api::AppBase).
+    const std::vector<TBuffer> &GatherBuffer()
+    {
+        if (kBuiltin || !gathered_) throw Error(SG_ERR_STATE, "sepgraph: GatherBuffer needs an AppBase app that ran");
+        return buffer_;
+    }
+
     const graphs::CSRGraph &CSRGraph() const { return csr_; }
+    const GraphDatum &GetGraphDatum() const { return datum_; }
     const sg_stats &Stats() const { return stats_; }
     sg_engine *Handle() const { return e_; }
 
@@ -305,10 +379,13 @@ private:
     policy::AlgoType algo_type_;
     common::EngineOptions options_;
     graphs::CSRGraph csr_;
+    GraphDatum datum_;
     sg_graph *g_ = nullptr;
     sg_engine *e_ = nullptr;
-    std::tuple<InitArgs...> init_{};
+    std::tuple<UnusedData...> init_{};
     std::vector<TValue> value_;
+    std::vector<TBuffer> buffer_;
+    bool gathered_ = false;
     sg_stats stats_{};
 };
 

@@ -353,6 +353,22 @@ extern "C" int sg_graph_create_ex(uint32_t n, uint64_t m, const uint32_t *row_st
     return SG_OK;
 }
 
+extern "C" int sg_graph_device_view(const sg_graph *g, sg_graph_view *out)
+{
+    if (!g || !out) SG_FAIL(SG_ERR_INVALID, "sg_graph_device_view: null argument");
+    const DevGraph &d = g->dev;
+    out->n = d.n;
+    out->relabeled = g->relabeled ? 1u : 0u;
+    out->m = d.m;
+    out->row_start = d.row;
+    out->edge_dst = d.col;
+    out->weights = d.w;
+    out->col_start = d.cstart;
+    out->edge_src = d.csrc;
+    out->csc_weights = d.cw;
+    return SG_OK;
+}
+
 extern "C" int sg_graph_info(const sg_graph *g, uint32_t *n, uint64_t *m, uint64_t *device_bytes)
 {
     if (!g) SG_FAIL(SG_ERR_INVALID, "sg_graph_info: null graph");
@@ -428,6 +444,22 @@ extern "C" int sg_engine_set_stream(sg_engine *e, void *hip_stream)
     e->stream = (hipStream_t)hip_stream;
     e->own_stream = false;
     return SG_OK;
+}
+
+extern "C" int sg_engine_stream(sg_engine *e, void **hip_stream)
+{
+    if (!e || !hip_stream) SG_FAIL(SG_ERR_INVALID, "sg_engine_stream: null argument");
+    *hip_stream = (void *)e->stream;
+    return SG_OK;
+}
+
+extern "C" int sg_engine_order_launch(sg_engine *e)
+{
+    if (!e) SG_FAIL(SG_ERR_INVALID, "sg_engine_order_launch: null engine");
+    // the caller's kernel follows on this stream: any persistent kernel the
+    // library enqueues later waits for it like for one of its own
+    e->launch_seq++;
+    return engine_launch_persistent(e, []() { return SG_OK; });
 }
 
 extern "C" int sg_engine_set_options(sg_engine *e, const sg_engine_options *o)
