@@ -243,11 +243,19 @@ enum { SG_PRIO_NONE = 0, SG_PRIO_LOW_HIGH = 1, SG_PRIO_SAMPLING = 2 };
 
 typedef struct sg_engine_options {
     int32_t force_variant;      /* EngineOptions::ForceVariant, or SG_VARIANT_AUTO */
-    int32_t fused;              /* EngineOptions::SetFused (persistent multi-round kernel) */
+    int32_t fused;              /* EngineOptions::SetFused. Every run here is one persistent
+                                   multi-round kernel (the reference's fused form), so the
+                                   field only records the request */
     int32_t priority;           /* SG_PRIO_* */
     float priority_threshold;   /* SetTwoLevelBasedPriority delta (SSSP); 0 = heuristic */
     float cut_threshold;        /* SetSampleBasedPriority */
-    int32_t lb_push, lb_pull;   /* SetLoadBalancing, SG_LB_* */
+    int32_t lb_push, lb_pull;   /* SetLoadBalancing, SG_LB_*, for push / pull rounds.
+                                   SG_LB_NONE: one lane per vertex walks its edges in
+                                   order, no hub splitting (sg_bfs, sg_sssp only; other
+                                   entry points return SG_ERR_UNSUPPORTED). DEFAULT,
+                                   COARSE, FINE and HYBRID select the balanced schedulers:
+                                   hub edge lists split into chunks over the grid, the
+                                   rest expanded by waves with prefix-sum scheduling */
     double alpha;               /* FLAGS_alpha, default 15 (include/utils/app_skeleton.h:50) */
     double beta;                /* FLAGS_beta, default 18 (app_skeleton.h:51) */
     double edge_factor;         /* FLAGS_edge_factor, default 1 (app_skeleton.h:49) */
@@ -255,7 +263,8 @@ typedef struct sg_engine_options {
                                    cuts high-diameter traversals short, framework.cuh:359-364) */
     int32_t block_size;         /* threads per workgroup for traversal kernels; 0 = tuned */
     int32_t large_degree;       /* out-degree above which a push vertex is split across
-                                   the grid; 0 = tuned */
+                                   the grid: 0 or 256 (the split built with the graph);
+                                   other values are SG_ERR_UNSUPPORTED */
     int32_t grid_blocks;        /* cap on the persistent grid's workgroups (0: fill the
                                    GPU); small grids make rounds cheaper on high-diameter
                                    graphs whose frontiers stay small */

@@ -79,6 +79,9 @@ struct BfsArgs {
     uint32_t max_rounds;
     uint32_t probe;   // launch is a residency probe of the grid
     uint32_t full_init;  // bitmaps and counters not known to be clean
+    // LoadBalancing::NONE (one lane per vertex, its edges in order, no hub
+    // chunks) for push / pull rounds; otherwise the balanced schedulers
+    uint32_t lb_push_none, lb_pull_none;
 };
 
 // Largest source out-degree whose expansion (round 0) block 0 does alone.
@@ -145,6 +148,59 @@ __device__ __forceinline__ void push_expand(Round &R, bool valid, uint32_t u)
         const int k = owner_lane(incl, j);
         const uint32_t e = shfl(off, k) + j;
         push_visit(R, act, act ? g.col[e] : 0u);
+    }
+}
+
+// LoadBalancing::NONE push: each lane walks its own vertex's out-edges in
+// CSR order, hubs included (the reference's thread-per-vertex relax,
+// async_push_dd.cuh:24-78); the wave steps through the longest list.
+__device__ __forceinline__ void push_expand_serial(Round &R, bool valid, uint32_t u)
+{
+    const DevGraph &g = R.a->g;
+    uint32_t b = 0, d = 0;
+    if (valid) {
+        b = g.row[u];
+        d = g.row[u + 1] - b;
+    }
+    uint32_t dmax = d;
+#pragma unroll
+    for (int o = kWave / 2; o > 0; o >>= 1) dmax = max(dmax, (uint32_t)__shfl_xor(dmax, o, kWave));
+    for (uint32_t j = 0; j < dmax; j++) {
+        const bool act = j < d;
+        push_visit(R, act, act ? g.col[b + j] : 0u);
+    }
+}
+
+// LoadBalancing::NONE pull for the word `w` (visited word vw): each
+// unvisited lane walks its own in-edges in CSC order and stops at its first
+// parent in the frontier (async_pull_dd.cuh:26-70).
+__device__ __forceinline__ void pull_word_serial(Round &R, uint32_t w, uint64_t vw)
+{
+    const BfsArgs &a = *R.a;
+    const DevGraph &g = a.g;
+    const int ln = lane();
+    const uint32_t v = w * 64 + ln;
+    const uint64_t visited = ballot(v >= g.n) | vw;
+    const bool want = !((visited >> ln) & 1);
+    uint32_t b = want ? g.cstart[v] : 0u, e = want ? g.cstart[v + 1] : 0u;
+    bool found = false;
+    while (ballot(b < e && !found)) {
+        if (b < e && !found) {
+            found = in_frontier(R, g.csrc[b]);
+            b++;
+        }
+    }
+    const uint64_t fm = ballot(found);
+    if (ln == 0) {
+        a.vis[w] = visited | fm;
+        R.fnext[w] = fm;
+        R.count += __popcll(fm);
+    }
+    if (found) {
+        a.level[caller_id(a, v)] = R.L + 1;
+        const uint32_t dv = g.row[v + 1] - g.row[v];
+        R.scout += dv;
+        R.huge |= dv > kHugeDegree;
     }
 }
 
@@ -387,7 +443,7 @@ __global__ SG_BFS_BOUNDS(BS) void k_bfs(BfsArgs a)
             // (1) high-degree frontier vertices through the static chunk list;
             //     lane ln of wave gw takes chunk base + ln*nwaves + gw, so the
             //     chunks of one hub land on different waves.
-            const uint32_t nhuge = rc.has_huge ? g.nhuge : 0u;
+            const uint32_t nhuge = rc.has_huge && !a.lb_push_none ? g.nhuge : 0u;
             for (uint32_t base = 0; base < nhuge; base += nwaves * kWave) {
                 const uint32_t c = base + (uint32_t)ln * nwaves + gwave;
                 uint2 dsc = make_uint2(0u, 0u);
@@ -422,7 +478,9 @@ __global__ SG_BFS_BOUNDS(BS) void k_bfs(BfsArgs a)
                         const int t = first_lane(todo);
                         todo &= todo - 1;
                         const uint64_t fw = shfl(fl, t);
-                        push_expand(R, (fw >> ln) & 1, (wb + (uint32_t)t * nwaves + gwave) * 64 + ln);
+                        const uint32_t u = (wb + (uint32_t)t * nwaves + gwave) * 64 + ln;
+                        if (a.lb_push_none) push_expand_serial(R, (fw >> ln) & 1, u);
+                        else push_expand(R, (fw >> ln) & 1, u);
                     }
                 }
             } else {
@@ -434,7 +492,8 @@ __global__ SG_BFS_BOUNDS(BS) void k_bfs(BfsArgs a)
                     const uint32_t i = i0 + (uint32_t)ln * nwaves + gwave;
                     const bool valid = i < rc.qlen;
                     if (!ballot(valid)) break;
-                    push_expand(R, valid, valid ? qin[i] : 0u);
+                    if (a.lb_push_none) push_expand_serial(R, valid, valid ? qin[i] : 0u);
+                    else push_expand(R, valid, valid ? qin[i] : 0u);
                 }
             }
         } else {
@@ -448,6 +507,13 @@ __global__ SG_BFS_BOUNDS(BS) void k_bfs(BfsArgs a)
                 const uint64_t vl = wl < W ? a.vis[wl] : ~0ull;
                 if (wl < W && vl == ~0ull) R.fnext[wl] = 0;
                 uint64_t todo = ballot(vl != ~0ull);
+                if (a.lb_pull_none) {
+                    while (todo) {
+                        const int t = first_lane(todo);
+                        todo &= todo - 1;
+                        pull_word_serial(R, wb + (uint32_t)t * nwaves + gwave, shfl(vl, t));
+                    }
+                }
                 while (todo) {
                     constexpr int G = kPullGroup;
                     uint32_t wg[G];
@@ -558,6 +624,8 @@ int bfs_launch(sg_engine *e, uint32_t source, int32_t *out, bool traced)
     a.beta = o.beta;
     a.total_workload0 = (uint32_t)((double)d.m * o.edge_factor);
     a.max_rounds = (uint32_t)o.max_iteration;
+    a.lb_push_none = o.lb_push == SG_LB_NONE;
+    a.lb_pull_none = o.lb_pull == SG_LB_NONE;
     e->last_algo = 0;
     const int bs = o.block_size ? o.block_size : 1024;
     switch (bs) {

@@ -473,6 +473,12 @@ extern "C" int sg_engine_set_options(sg_engine *e, const sg_engine_options *o)
     if (o->max_iteration < 0) SG_FAIL(SG_ERR_INVALID, "sg_engine_set_options: max_iteration < 0");
     if (o->grid_blocks < 0 || o->grid_blocks > (int)kMaxBlocks)
         SG_FAIL(SG_ERR_INVALID, "sg_engine_set_options: grid_blocks must be in [0, 8192]");
+    if (o->lb_push < SG_LB_DEFAULT || o->lb_push > SG_LB_HYBRID || o->lb_pull < SG_LB_DEFAULT ||
+        o->lb_pull > SG_LB_HYBRID)
+        SG_FAIL(SG_ERR_INVALID, "sg_engine_set_options: unknown load-balancing mode");
+    if (o->large_degree != 0 && o->large_degree != (int32_t)kHugeDegree)
+        SG_FAIL(SG_ERR_UNSUPPORTED, "sg_engine_set_options: large_degree must be 0 or 256 (the split the graph "
+                                    "was built with)");
     e->opt = *o;
     return SG_OK;
 }

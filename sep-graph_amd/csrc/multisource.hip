@@ -1042,6 +1042,8 @@ int ms_launch(sg_engine *e, int kind, const uint32_t *dev_src, uint32_t nsrc, vo
     const sg_engine_options &o = e->opt;
     const int force = forced_direction(o);
     if (force < -1) SG_FAIL(SG_ERR_UNSUPPORTED, "batched launch: only data-driven push/pull variants apply");
+    if (o.lb_push == SG_LB_NONE || o.lb_pull == SG_LB_NONE)
+        SG_FAIL(SG_ERR_UNSUPPORTED, "batched launch: LoadBalancing::NONE applies to sg_bfs and sg_sssp only");
     const int bs = o.block_size ? o.block_size : 1024;
     if (bs != 256 && bs != 512 && bs != 1024) SG_FAIL(SG_ERR_UNSUPPORTED, "batched launch: block_size must be 256, 512 or 1024");
     const double alpha = o.alpha == 0.0 ? 1.0 : o.alpha;
@@ -1130,6 +1132,8 @@ int ms_run(sg_engine *e, int kind, const uint32_t *sources, int32_t k, int32_t b
         if (sources[i] >= d.n) SG_FAIL(SG_ERR_RANGE, std::string(name) + ": source >= n");
     if (forced_direction(e->opt) < -1)
         SG_FAIL(SG_ERR_UNSUPPORTED, std::string(name) + ": only data-driven push/pull variants apply");
+    if (e->opt.lb_push == SG_LB_NONE || e->opt.lb_pull == SG_LB_NONE)
+        SG_FAIL(SG_ERR_UNSUPPORTED, std::string(name) + ": LoadBalancing::NONE applies to sg_bfs and sg_sssp only");
     int rc = ms_prepare(e, kind);
     if (rc) return rc;
     const uint64_t n = d.n;

@@ -792,6 +792,8 @@ extern "C" int sg_pagerank(sg_engine *e, double damping, double tol, int32_t max
     if (!(tol > 0.0)) SG_FAIL(SG_ERR_RANGE, "sg_pagerank: tol must be > 0");
     if (max_iter < 0) SG_FAIL(SG_ERR_RANGE, "sg_pagerank: max_iter < 0");
     const sg_engine_options &o = e->opt;
+    if (o.lb_push == SG_LB_NONE || o.lb_pull == SG_LB_NONE)
+        SG_FAIL(SG_ERR_UNSUPPORTED, "sg_pagerank: LoadBalancing::NONE applies to sg_bfs and sg_sssp only");
     if (mode == SG_PR_AUTO) {
         const int fv = o.force_variant;
         mode = (fv == SG_ASYNC_PUSH_DD || fv == SG_ASYNC_PULL_DD || fv == SG_SYNC_PUSH_DD || fv == SG_SYNC_PULL_DD)

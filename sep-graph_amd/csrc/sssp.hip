@@ -73,6 +73,9 @@ struct SsspArgs {
     const uint32_t *inv;
     uint32_t grouped;          // one query per XCD (gr), per-query state arrays
     SsspGroups gr;
+    // LoadBalancing::NONE (one lane per vertex, its edges in order, no hub
+    // chunks) for push / pull rounds; otherwise the balanced schedulers
+    uint32_t lb_push_none, lb_pull_none;
 };
 
 struct Round {
@@ -178,6 +181,75 @@ __device__ __forceinline__ void push_expand(Round &R, bool valid, uint32_t u)
             nd = sat_add(duk, edge_w(g.w, e));
         }
         relax(R, act, v, nd);
+    }
+}
+
+// LoadBalancing::NONE push: each lane relaxes its own vertex's out-edges in
+// CSR order, hubs included (async_push_dd.cuh:24-78).
+__device__ __forceinline__ void push_expand_serial(Round &R, bool valid, uint32_t u)
+{
+    const DevGraph &g = R.a->g;
+    uint32_t b = 0, d = 0, du = kInf;
+    bool later = false;
+    if (valid) {
+        b = g.row[u];
+        d = g.row[u + 1] - b;
+        du = R.a->dist[u];
+        if (du == kInf) d = 0;
+        if (du != kInf && du >= R.thr) {
+            later = true;
+            d = 0;
+        }
+    }
+    defer(R, later, u, du);
+    uint32_t dmax = d;
+#pragma unroll
+    for (int o = kWave / 2; o > 0; o >>= 1) dmax = max(dmax, (uint32_t)__shfl_xor(dmax, o, kWave));
+    for (uint32_t j = 0; j < dmax; j++) {
+        const bool act = j < d;
+        uint32_t v = 0, nd = kInf;
+        if (act) {
+            v = g.col[b + j];
+            nd = sat_add(du, edge_w(g.w, b + j));
+        }
+        relax(R, act, v, nd);
+    }
+}
+
+// LoadBalancing::NONE pull for the word w: each lane's vertex takes the
+// minimum of dist[u] + w over its in-neighbours u in the frontier, walking
+// its whole in-edge list in CSC order (async_pull_dd.cuh:26-70).
+__device__ __forceinline__ void pull_word_serial(Round &R, uint32_t w)
+{
+    const SsspArgs &a = *R.a;
+    const DevGraph &g = a.g;
+    const int ln = lane();
+    const uint32_t v = w * 64 + ln;
+    uint32_t b = 0, e = 0, best = kInf;
+    if (v < g.n) {
+        b = g.cstart[v];
+        e = g.cstart[v + 1];
+        best = a.dist[v];
+    }
+    const uint32_t dv0 = best;
+    for (; b < e; b++) {
+        const uint32_t u = g.csrc[b];
+        if (in_frontier(R, u)) {
+            const uint32_t du = a.dist[u];
+            if (du != kInf) best = min(best, sat_add(du, edge_w(g.cw, b)));
+        }
+    }
+    const bool improved = v < g.n && best < dv0;
+    if (improved) {
+        a.dist[v] = best;
+        const uint32_t dv = g.row[v + 1] - g.row[v];
+        R.scout += dv;
+        R.huge |= dv > kHugeDegree;
+    }
+    const uint64_t fm = ballot(improved);
+    if (ln == 0 && fm) {
+        R.fnext[w] = fm;
+        R.count += __popcll(fm);
     }
 }
 
@@ -415,7 +487,7 @@ __device__ __forceinline__ void sssp_run(const SsspArgs &a, uint32_t brank, uint
         const uint32_t *qin = a.queue + (size_t)(L % 3) * a.qcap;
 
         if (!pull) {
-            const uint32_t nhuge = rc.has_huge ? g.nhuge : 0u;
+            const uint32_t nhuge = rc.has_huge && !a.lb_push_none ? g.nhuge : 0u;
             for (uint32_t base = 0; base < nhuge; base += nwaves * kWave) {
                 const uint32_t c = base + (uint32_t)ln * nwaves + gwave;
                 uint2 dsc = make_uint2(0u, 0u);
@@ -457,7 +529,9 @@ __device__ __forceinline__ void sssp_run(const SsspArgs &a, uint32_t brank, uint
                         const int t = first_lane(todo);
                         todo &= todo - 1;
                         const uint64_t fw = shfl(fl, t);
-                        push_expand(R, (fw >> ln) & 1, (wb + (uint32_t)t * nwaves + gwave) * 64 + ln);
+                        const uint32_t u = (wb + (uint32_t)t * nwaves + gwave) * 64 + ln;
+                        if (a.lb_push_none) push_expand_serial(R, (fw >> ln) & 1, u);
+                        else push_expand(R, (fw >> ln) & 1, u);
                     }
                 }
             } else if (!rc.overflow) {
@@ -468,12 +542,14 @@ __device__ __forceinline__ void sssp_run(const SsspArgs &a, uint32_t brank, uint
                     const uint32_t i = i0 + (uint32_t)ln * nwaves + gwave;
                     const bool valid = i < rc.qlen;
                     if (!ballot(valid)) break;
-                    push_expand(R, valid, valid ? qin[i] : 0u);
+                    if (a.lb_push_none) push_expand_serial(R, valid, valid ? qin[i] : 0u);
+                    else push_expand(R, valid, valid ? qin[i] : 0u);
                 }
             } else {
                 for (uint32_t w = gwave; w < W; w += nwaves) {
                     const uint32_t v = w * 64 + ln;
-                    push_expand(R, v < n && test_bit(R.fcur, v), v);
+                    if (a.lb_push_none) push_expand_serial(R, v < n && test_bit(R.fcur, v), v);
+                    else push_expand(R, v < n && test_bit(R.fcur, v), v);
                 }
             }
         } else {
@@ -483,14 +559,15 @@ __device__ __forceinline__ void sssp_run(const SsspArgs &a, uint32_t brank, uint
             // Pull: (1) hubs' in-edges through the static chunk list, spread
             // over all waves, kHubChunks chunks per wave at a time; (2) every
             // other vertex with in-edges, a word at a time.
-            for (uint32_t c0 = gwave; c0 < g.nhugein; c0 += kHubChunks * nwaves) {
+            const uint32_t nhugein = a.lb_pull_none ? 0u : g.nhugein;
+            for (uint32_t c0 = gwave; c0 < nhugein; c0 += kHubChunks * nwaves) {
                 uint2 dsc[kHubChunks];
                 int nc = 0;
 #pragma unroll
                 for (int i = 0; i < kHubChunks; i++) {
                     const uint32_t c = c0 + (uint32_t)i * nwaves;
-                    dsc[i] = c < g.nhugein ? g.hugein[c] : make_uint2(0u, 0u);
-                    nc += c < g.nhugein ? 1 : 0;
+                    dsc[i] = c < nhugein ? g.hugein[c] : make_uint2(0u, 0u);
+                    nc += c < nhugein ? 1 : 0;
                 }
                 pull_hub_chunks<kHubChunks>(R, dsc, nc);
             }
@@ -499,7 +576,8 @@ __device__ __forceinline__ void sssp_run(const SsspArgs &a, uint32_t brank, uint
             // few waves with most of the in-edges
             for (uint32_t w = gwave; w < W; w += nwaves) {
                 if (g.inert && g.inert[w] == ~0ull) continue;
-                pull_word<kPullSteps>(R, w);
+                if (a.lb_pull_none) pull_word_serial(R, w);
+                else pull_word<kPullSteps>(R, w);
             }
         }
         // counters first: their atomics are then in flight while the queue
@@ -650,6 +728,8 @@ int sssp_args(sg_engine *e, SsspArgs &a)
     a.beta = o.beta;
     a.total_workload0 = (uint32_t)((double)d.m * o.edge_factor);
     a.max_rounds = (uint32_t)o.max_iteration;
+    a.lb_push_none = o.lb_push == SG_LB_NONE;
+    a.lb_pull_none = o.lb_pull == SG_LB_NONE;
     if (o.priority == SG_PRIO_LOW_HIGH) {
         // two-level priority (the reference's SetTwoLevelBasedPriority,
         // algo_variants.cuh:605-685) runs push rounds only; the bucket width

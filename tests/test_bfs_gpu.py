@@ -122,6 +122,11 @@ def test_bfs_rejects_bad_source(sg, gpu):
     {"max_iteration": 0, "edge_factor": 4.0},
     {"alpha": 0.0},
     {"alpha": 2.0, "beta": 2.0},
+    {"lb_push": 0},
+    {"lb_pull": 0},
+    {"lb_push": 0, "lb_pull": 0, "force_variant": "async_pull_dd"},
+    {"lb_push": 0, "force_variant": "async_push_dd", "block_size": 256},
+    {"lb_push": 2, "lb_pull": 3, "fused": 1, "large_degree": 256},
 ])
 def test_bfs_rmat16_config1(sg, orc, gpu, rmat16_sym, opts):
     """Config 1: symmetrised RMAT-16, source 0 plus seeded sources, bit-exact."""
@@ -245,3 +250,20 @@ def test_two_engines_interleaved_without_sync(sg, orc, gpu, rmat16_sym):
     assert np.array_equal(e1.gather(np.int32), orc.bfs(hg.row, hg.col, srcs[-1]))
     e2.bfs(srcs[0])
     assert np.array_equal(e2.gather(np.int32), orc.bfs(hg.row, hg.col, srcs[0]))
+
+
+def test_options_outside_the_implementation_are_rejected(sg, gpu, rmat16_sym):
+    """LoadBalancing::NONE exists for BFS and SSSP only; large_degree is the
+    split the graph was built with; unknown modes are invalid."""
+    e = sg.Engine(sg.Graph(rmat16_sym))
+    for kw, code in (({"lb_push": 7}, -1), ({"lb_pull": -2}, -1), ({"large_degree": 100}, -8)):
+        with pytest.raises(sg.SepGraphError) as ei:
+            e.set_options(**kw)
+        assert ei.value.code == code, kw
+    e.set_options(lb_pull=0)
+    with pytest.raises(sg.SepGraphError) as ei:
+        e.pagerank(0.85, 1e-4, 3, sg.PR_PULL_TD)
+    assert ei.value.code == -8
+    with pytest.raises(sg.SepGraphError) as ei:
+        e.bc_dependency([0, 1], 2)
+    assert ei.value.code == -8

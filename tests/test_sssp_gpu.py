@@ -96,7 +96,9 @@ def test_sssp_rejects_bad_source(sg, gpu):
 
 
 @pytest.mark.parametrize("opts", [{}, {"force_variant": "async_push_dd"}, {"force_variant": "async_pull_dd"},
-                                  {"block_size": 256}, {"block_size": 512}, {"alpha": 2.0}])
+                                  {"block_size": 256}, {"block_size": 512}, {"alpha": 2.0},
+                                  {"lb_push": 0}, {"lb_pull": 0, "force_variant": "async_pull_dd"},
+                                  {"lb_push": 0, "lb_pull": 0}, {"lb_push": 0, "priority": 1}])
 def test_sssp_rmat16_weighted(sg, orc, gpu, opts):
     hg = sg.rmat_graph(16, 16, symmetrize=True, seed=1, weights=(1, 64))
     srcs = sg.pick_sources(hg.row, 8, seed=3)
