@@ -135,6 +135,8 @@ public:
     }
     void SetBlockSize(int block_size) { o_.block_size = block_size; }
     void SetMaxIteration(int max_iteration) { o_.max_iteration = max_iteration; }
+    // FLAGS_edge_factor: the traversal's unvisited-edge budget is edge_factor * nedges
+    void SetEdgeFactor(double edge_factor) { o_.edge_factor = edge_factor; }
     const sg_engine_options &c_options() const { return o_; }
 
 private:
