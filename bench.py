@@ -74,17 +74,27 @@ def cpu_baseline(hg, sources, gpu_levels, budget_s):
     }
 
 
-def serving_bench(sg, g, hg, e, args):
-    """Queries/s and latency of the batching server against plain one-at-a-time
-    execution, on the resident graph: a seeded mix of 60% BFS, 30% SSSP, 10%
-    BC-dependency queries with sources uniform over non-isolated vertices."""
-    rng = np.random.default_rng(4)
-    pool = sg.pick_sources(hg.row, 4096, seed=5)
-    # plain: the single-query entry points, one query at a time, synchronised
-    kinds = rng.choice([0, 1, 2], size=args.serve_plain, p=[0.6, 0.3, 0.1])
-    srcs = rng.choice(pool, size=args.serve_plain)
+def draw_mix(rng, pools, share, count):
+    """count queries of the 60/30/10 BFS/SSSP/BC-dependency mix: graph index
+    drawn with probabilities `share`, source uniform over that graph's pool."""
+    gi = rng.choice(len(pools), size=count, p=share)
+    kinds = rng.choice([0, 1, 2], size=count, p=[0.6, 0.3, 0.1])
+    srcs = np.array([pools[k][rng.integers(len(pools[k]))] for k in gi], np.uint32)
+    return gi.astype(np.int32), kinds.astype(np.int32), srcs
+
+
+def serving_bench(sg, graphs, hgs, engines, share, label, rates, count, plain_n, seed, mode=0):
+    """Queries/s and latency of the server against plain one-at-a-time
+    execution on the same resident graphs: a seeded Poisson stream of the
+    60/30/10 BFS/SSSP/BC-dependency mix, sources uniform over non-isolated
+    vertices. Plain = the single-query entry points (sg_bfs, sg_sssp,
+    sg_bc_dependency with one source) one query at a time, synchronised."""
+    rng = np.random.default_rng(seed)
+    pools = [sg.pick_sources(h.row, min(1 << 16, h.n // 4), seed=seed + 1 + i) for i, h in enumerate(hgs)]
+    gi, kinds, srcs = draw_mix(rng, pools, share, plain_n)
     t0 = time.perf_counter()
-    for k, s in zip(kinds, srcs):
+    for g_, k, s in zip(gi, kinds, srcs):
+        e = engines[g_]
         if k == 0:
             e.bfs(int(s))
         elif k == 1:
@@ -92,51 +102,107 @@ def serving_bench(sg, g, hg, e, args):
         else:
             e.bc_dependency([int(s)], 1)
         e.sync()
-    plain_qps = args.serve_plain / (time.perf_counter() - t0)
-    log(f"plain one-at-a-time: {plain_qps:.1f} queries/s")
-    mb = [int(x) for x in args.serve_batch.split(",")]
-    srv = sg.Server(g, memory_budget=8 << 30, max_rows=1024, max_batch=mb)
-    # warm the batched kernels (grid probes, arenas) outside the timed runs,
-    # an SSSP batch large enough for the bit-packed launch included
-    srv.replay([0, 1, 2] + [1] * 8, pool[:11], [0.0] * 11)
+    plain_qps = plain_n / (time.perf_counter() - t0)
+    log(f"[{label}] plain one-at-a-time: {plain_qps:.1f} queries/s")
+    srv = sg.Server(graphs, memory_budget=16 << 30, max_rows=1024, mode=mode)
+    # warm the serving kernel (residency probe done at creation; first launch)
+    wg, wk, ws = draw_mix(rng, pools, share, 16)
+    srv.replay(wk, ws, [0.0] * 16, graphs=wg)
     runs = []
-    for rate in [float(r) for r in args.serve_rates.split(",")]:
-        kinds = rng.choice([0, 1, 2], size=args.serve_count, p=[0.6, 0.3, 0.1])
-        srcs = rng.choice(pool, size=args.serve_count)
-        arr = np.cumsum(rng.exponential(1.0 / rate, args.serve_count))
+    for rate in rates:
+        gi, kinds, srcs = draw_mix(rng, pools, share, count)
+        arr = np.cumsum(rng.exponential(1.0 / rate, count))
         before = srv.stats()
         t0 = time.perf_counter()
-        lat = srv.replay(kinds, srcs, arr)
+        lat = srv.replay(kinds, srcs, arr, graphs=gi)
         wall = time.perf_counter() - t0
         after = srv.stats()
         nb = after["batches"] - before["batches"]
         run = {
             "offered_qps": rate,
-            "achieved_qps": round(args.serve_count / wall, 1),
+            "achieved_qps": round(count / wall, 1),
             "p50_ms": round(float(np.percentile(lat, 50)), 3),
             "p99_ms": round(float(np.percentile(lat, 99)), 3),
-            "batches": int(nb),
-            "mean_batch": round(args.serve_count / max(nb, 1), 2),
+            "launches": int(nb),
+            "queries": count,
         }
-        log(f"serving at {rate:.0f} q/s offered: {run}")
+        if len(graphs) > 1:
+            for j in range(len(graphs)):
+                sel = gi == j
+                if sel.any():
+                    run[f"p99_ms_graph{j}"] = round(float(np.percentile(lat[sel], 99)), 3)
+        log(f"[{label}] {rate:.0f} q/s offered: {run}")
         runs.append(run)
+        if run["achieved_qps"] < 0.8 * rate and run["p99_ms"] > 200.0:
+            break  # saturated: higher rates only queue longer
     st = srv.stats()
     srv.close()
     best = max(r["achieved_qps"] for r in runs)
     ok50 = [r["achieved_qps"] for r in runs if r["p99_ms"] <= 50.0]
     best50 = max(ok50) if ok50 else None
     return {
-        "mix": f"60% BFS / 30% SSSP / 10% BC-dependency, RMAT-{args.scale} directed",
+        "workload": label,
+        "mix": "60% BFS / 30% SSSP / 10% BC-dependency, Poisson arrivals",
+        "mode": ["slots (serving kernel, one query slot per XCD)", "bit-packed batches"][mode],
+        "graph_share": [round(float(x), 4) for x in share],
         "plain_qps": round(plain_qps, 1),
-        "batched_qps": best,
+        "plain_queries": plain_n,
+        "served_qps": best,
         "speedup_vs_plain": round(best / plain_qps, 2),
-        "batched_qps_p99_under_50ms": best50,
+        "served_qps_p99_under_50ms": best50,
         "speedup_vs_plain_p99_under_50ms": None if best50 is None else round(best50 / plain_qps, 2),
         "runs": runs,
-        "queries_per_rate": args.serve_count,
-        "max_batch": mb,
+        "queries_per_rate": count,
         "device_bytes": int(st["device_bytes"]),
     }
+
+
+def config5(sg, args, rates):
+    """Serving on a directed RMAT-24 (edge factor 16, weights in [1,64)) and
+    the 2048x2048 grid (10% of edges deleted, weights in [1,1000)), resident
+    together in one server; each query's graph is drawn in proportion to the
+    graphs' non-isolated vertex counts, so sources are uniform over the
+    non-isolated vertices of both."""
+    t0 = time.time()
+    h24 = sg.rmat_graph(24, 16, symmetrize=False, seed=1, weights=(1, 64), weight_seed=2)
+    hgrid = sg.grid_graph(2048, 2048, p_delete=0.1, seed=1, weights=(1, 1000), weight_seed=2)
+    g24, ggrid = sg.Graph(h24), sg.Graph(hgrid)
+    log(f"[config5] RMAT-24 n={h24.n} m={h24.m}, grid n={hgrid.n} m={hgrid.m}, resident in {time.time() - t0:.1f}s "
+        f"({(g24.info()[2] + ggrid.info()[2]) / 1e9:.2f} GB)")
+    live = [int((np.diff(h.row.astype(np.int64)) > 0).sum()) for h in (h24, hgrid)]
+    share = np.array(live, np.float64) / sum(live)
+    engines = [sg.Engine(g24), sg.Engine(ggrid)]
+    out = serving_bench(sg, [g24, ggrid], [h24, hgrid], engines, share, "rmat24-directed+grid2048", rates,
+                        args.serve_count, args.serve_plain, seed=40)
+    for e in engines:
+        e.close()
+    g24.close()
+    ggrid.close()
+    return out
+
+
+def kernel_sweep(sg, e, hg, sources, reps):
+    """Each traversal kernel `reps` times on the benchmark graph, one kind
+    after another (k_bfs, k_sssp, k_pr_pull, k_pr_dd, k_bc1, k_bc, k_msbfs,
+    k_mssssp), for per-kernel rocprofv3 statistics and counters."""
+    src = [int(s) for s in sources]
+    steps = [
+        ("k_bfs", lambda i: e.bfs(src[i % len(src)])),
+        ("k_sssp", lambda i: e.sssp(src[i % len(src)])),
+        ("k_pr_pull", lambda i: e.pagerank(0.85, 1e-6, 0, sg.PR_PULL_TD)),
+        ("k_pr_dd", lambda i: e.pagerank(0.85, 1e-6, 0, sg.PR_PUSH_DD)),
+        ("k_bc1", lambda i: e.bc_dependency([src[i % len(src)]], 1)),
+        ("k_bc", lambda i: e.bc(src[:64], 64)),
+        ("k_msbfs", lambda i: e.bfs_multi(src[:64], 64)),
+        ("k_mssssp", lambda i: e.sssp_multi(src[:64], 64)),
+    ]
+    for name, run in steps:
+        t0 = time.perf_counter()
+        for i in range(reps):
+            run(i)
+            e.sync()
+        log(f"{name}: {reps} runs in {(time.perf_counter() - t0) * 1e3:.1f} ms")
+    print(json.dumps({"kernels": [n for n, _ in steps], "reps": reps, "n": hg.n, "m": hg.m}), flush=True)
 
 
 def main():
@@ -150,13 +216,17 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--sssp-sources", type=int, default=16)
     ap.add_argument("--no-serving", action="store_true")
-    ap.add_argument("--serve-rates", type=str, default="400,800,1000,1200,1600,2000,2600",
-                    help="Poisson arrival rates (queries/s) for the serving run")
-    ap.add_argument("--serve-count", type=int, default=1200, help="queries per arrival rate")
-    ap.add_argument("--serve-plain", type=int, default=100, help="queries timed one at a time (plain baseline)")
-    ap.add_argument("--serve-batch", type=str, default="64,16,32", help="max queries per batch: bfs,sssp,bc")
+    ap.add_argument("--serve-rates", type=str, default="1000,2000,3000,4000,5000,7000,10000,20000,50000",
+                    help="Poisson arrival rates (queries/s) for the serving runs")
+    ap.add_argument("--serve-count", type=int, default=2000, help="queries per arrival rate")
+    ap.add_argument("--serve-plain", type=int, default=200, help="queries timed one at a time (plain baseline)")
+    ap.add_argument("--config5", type=int, default=1,
+                    help="also serve RMAT-24 and the 2048x2048 grid resident together (0: skip)")
     ap.add_argument("--copy-mb", type=int, default=0,
                     help="only run a plain device copy of this many MiB (counter calibration)")
+    ap.add_argument("--kernels", type=int, default=0,
+                    help="only run every traversal kernel this many times on the benchmark graph "
+                         "(for rocprofv3 kernel statistics and counters), then exit")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
@@ -190,6 +260,9 @@ def main():
     sources = sg.pick_sources(hg.row, args.sources, seed=3)
     stream = torch.cuda.current_stream()
     e = sg.Engine(g, stream=stream.cuda_stream)
+    if args.kernels:
+        kernel_sweep(sg, e, hg, sources, args.kernels)
+        return
 
     # Warm-up pass: per-source reach statistics (deterministic), kept levels for
     # the CPU-baseline cross-check.
@@ -246,8 +319,13 @@ def main():
     log(f"SSSP: {ns} queries, mean {sssp_ms / max(ns, 1):.2f} ms, {sssp_gteps:.2f} GTEPS")
 
     serving = None
+    rates = [float(r) for r in args.serve_rates.split(",")]
     if not args.no_serving:
-        serving = serving_bench(sg, g, hg, e, args)
+        serving = serving_bench(sg, [g], [hg], [e], [1.0], f"rmat{args.scale}-directed", rates,
+                                args.serve_count, args.serve_plain, seed=4)
+    serving5 = None
+    if args.config5:
+        serving5 = config5(sg, args, rates)
 
     cpu = None
     if not args.no_cpu:
@@ -291,6 +369,7 @@ def main():
         "cpu_baseline": cpu,
         "sssp": {"gteps": round(sssp_gteps, 3), "queries": ns, "mean_ms": round(sssp_ms / max(ns, 1), 3)},
         "serving": serving,
+        "serving_config5": serving5,
     }
     print(json.dumps(out), flush=True)
 

@@ -413,19 +413,30 @@ int sg_sssp_concurrent(sg_engine *e, const uint32_t *sources, int32_t k);
 
 /* ---------------------------------------------------------------- serving
  * No reference counterpart: the reference answers one query per process
- * (samples/hybrid_bfs/main.cpp and its siblings). A server keeps a graph resident and answers
- * BFS / SSSP / BC-dependency queries with continuous batching: a worker
- * forms each batch (up to 64 sources of one kind, the kind whose oldest
- * query has waited longest) when the GPU frees up, from whatever is queued.
- * BFS and BC-dependency batches run as one bit-packed multi-source launch,
- * as do SSSP batches of 8 or more queries; a smaller SSSP batch runs its
- * queries back to back, each over the whole GPU.
+ * (samples/hybrid_bfs/main.cpp and its siblings). A server keeps a graph
+ * resident and answers BFS / SSSP / BC-dependency queries with continuous
+ * batching, in one of two forms:
+ *   SG_SERVE_SLOTS (default): one persistent serving kernel in which every
+ *   XCD of the GPU is a query slot. Queries are published, oldest first, into
+ *   a ring in pinned host memory as soon as a result row is free; a slot that
+ *   finishes its query claims the next entry, so arrivals join while other
+ *   queries run and up to one query per XCD runs at a time, each on the
+ *   single-source kernels (BFS, SSSP, BC dependency) over its XCD's blocks.
+ *   The kernel ends after a stretch without work and is launched again when
+ *   queries arrive.
+ *   SG_SERVE_BATCHES: a worker forms each batch (up to 64 sources of one
+ *   kind, the kind whose oldest query has waited longest) when the GPU frees
+ *   up, from whatever is queued. BFS and BC-dependency batches run as one
+ *   bit-packed multi-source launch, as do SSSP batches of 8 or more queries;
+ *   a smaller SSSP batch runs its queries back to back, each over the whole
+ *   GPU.
  * Admission is by device memory: each query holds one result row of n
  * 32-bit values (int32 levels, uint32 distances, FP32 0.5*delta) from the
  * start of its batch until released; rows = memory_budget / (4 n). Results
  * equal those of sg_bfs / sg_sssp / sg_bc_dependency for the same source.
  * Thread-safe: any thread may submit, wait, read and release. */
 enum { SG_QUERY_BFS = 0, SG_QUERY_SSSP = 1, SG_QUERY_BC_DEP = 2, SG_QUERY_KINDS = 3 };
+enum { SG_SERVE_SLOTS = 0, SG_SERVE_BATCHES = 1 };
 
 typedef struct sg_server sg_server;
 
@@ -434,9 +445,11 @@ typedef struct sg_server_config {
     int32_t max_rows;        /* cap on result rows (0: the budget alone) */
     int32_t max_batch[3];    /* per kind: queries per batch, 1..64 (0: 64) */
     /* smallest BFS / SSSP batch run as one bit-packed launch; smaller ones run
-     * whole-GPU single queries back to back (0: 16 for BFS, 8 for SSSP) */
+     * whole-GPU single queries back to back (0: 16 for BFS, 8 for SSSP);
+     * SG_SERVE_BATCHES only */
     int32_t bfs_packed_min, sssp_packed_min;
-    int32_t reserved[6];
+    int32_t mode;            /* SG_SERVE_SLOTS (default) or SG_SERVE_BATCHES */
+    int32_t reserved[5];
 } sg_server_config;
 
 typedef struct sg_server_stats {
@@ -451,12 +464,21 @@ typedef struct sg_server_stats {
 void sg_server_config_default(sg_server_config *c);
 /* opt may be NULL (defaults); the server owns an engine on its own stream. */
 int sg_server_create(sg_graph *g, const sg_server_config *cfg, const sg_engine_options *opt, sg_server **out);
+/* A server over 1..4 resident graphs on one device (SG_SERVE_SLOTS only):
+ * queries name their graph by its index in `graphs`; result rows are sized
+ * for the largest graph (memory_budget counts rows of 4 n_max bytes). The
+ * graphs must outlive the server. */
+int sg_server_create_multi(sg_graph *const *graphs, int32_t ngraphs, const sg_server_config *cfg,
+                           const sg_engine_options *opt, sg_server **out);
 int sg_server_destroy(sg_server *s);
 int sg_server_submit(sg_server *s, int32_t kind, uint32_t source, uint64_t *ticket);
+/* sg_server_submit on graph `graph` of a multi-graph server (0 is the first). */
+int sg_server_submit_to(sg_server *s, int32_t graph, int32_t kind, uint32_t source, uint64_t *ticket);
 /* timeout_ms < 0 waits without limit; SG_ERR_TIMEOUT if not finished in time */
 int sg_server_wait(sg_server *s, uint64_t ticket, int32_t timeout_ms);
 /* device pointer to the finished query's row (valid until released), its
- * size, time queued before its batch started and arrival-to-done time */
+ * size (4 n of the query's graph), time queued before its batch (or slot)
+ * started and arrival-to-done time */
 int sg_server_result(sg_server *s, uint64_t ticket, void **dev_ptr, uint64_t *bytes, double *queue_ms,
                      double *total_ms);
 int sg_server_copy_result(sg_server *s, uint64_t ticket, void *host_out, uint64_t bytes);
@@ -468,6 +490,9 @@ int sg_server_get_stats(sg_server *s, sg_server_stats *out);
  * when all have finished. */
 int sg_server_replay(sg_server *s, const int32_t *kinds, const uint32_t *sources, const double *arrival_s,
                      int64_t count, double *latency_ms);
+/* sg_server_replay with query i on graph graphs[i] (NULL: all on graph 0). */
+int sg_server_replay_multi(sg_server *s, const int32_t *graphs, const int32_t *kinds, const uint32_t *sources,
+                           const double *arrival_s, int64_t count, double *latency_ms);
 
 /* ---------------------------------------------------------------- autotuner
  * No reference counterpart (the reference fixes alpha = 15, beta = 18 and

@@ -1,0 +1,433 @@
+// Device side of single-source Brandes BC (one source per run), on the
+// structure of the BFS kernel: one query runs as a persistent grid (a whole
+// launch, or the blocks of one XCD).
+//
+// Reference path: samples/hybrid_bc/hybrid_bc.cu:303-411 runs one source per
+// Engine::Start. Stage 1 (the BC app, hybrid_bc.cu:23-131) is a hybrid BFS
+// that also sums path counts sigma and pushes every level onto a
+// WorklistStack (include/framework/worklist_stack.cuh:16-111); stage 2
+// (StageTwoDDFused, hybrid_bc.cu:134-225) pops the stack level by level and
+// adds sigma(u)/sigma(w) (1 + delta(w)) over successors w with atomicAdd; the
+// result is halved (hybrid_bc.cu:374-376). Sigma and delta are FP32 there.
+//
+// Here:
+//   1. levels: bfs_run (push/pull by the alpha/beta rule, exact claims),
+//      levels kept in device vertex order;
+//   2. level lists: a counting sort of the reached vertices by level (block
+//      histograms in LDS, one atomic per block and level, positions handed out
+//      per wave and level), so each later phase visits a level's vertices
+//      without scanning all n;
+//   3. sigma, level by level: every vertex gathers sigma(u) over in-neighbours
+//      one level up (pull over the CSC, wave prefix-sum scheduling); vertices
+//      with in-degree above kPullHubDegree are summed per 256-edge chunk of
+//      the graph's static in-hub list and merged with an FP64 atomicAdd --
+//      path counts are integers below 2^53, so the order of the additions
+//      cannot change them;
+//   4. dependencies, level by level from the deepest: acc(u) = sum over
+//      successors w one level down of psi(w) = 1/sigma(w) + acc(w), which is
+//      (1 + delta(w)) / sigma(w) with delta(w) = sigma(w) acc(w); sigma and
+//      acc sit side by side (one 16-byte load per successor); vertices with
+//      out-degree above kHugeDegree are summed per 64-edge chunk of the
+//      static out-hub list with an FP64 atomicAdd;
+//   5. 0.5 delta(v) = 0.5 sigma(v) acc(v) written out (0 for the source and
+//      for unreached vertices, as BetweennessCentralityHost,
+//      hybrid_bc_host.cpp:16-92).
+// Sigma and acc are FP64; the output is FP32.
+#pragma once
+
+#include "bfs_dev.hpp"
+
+namespace {
+namespace bc1k {
+
+using namespace bfsk;
+
+// Deepest level a single-source BC run can hold, and the levels a block
+// counts in LDS (deeper ones use global atomics).
+constexpr uint32_t kBcMaxDepth = 1u << 16;
+constexpr uint32_t kBcLdsLevels = 2048;
+
+struct Bc1Args {
+    BfsArgs f;          // forward BFS: f.level (device order, f.perm null), f.stats
+    double2 *sa;        // [n]: (sigma, acc) per device vertex
+    uint32_t *order;    // [n]: reached vertices grouped by level
+    uint32_t *lcnt;     // [kBcMaxDepth + 1]: vertices per level
+    uint32_t *loff;     // [kBcMaxDepth + 2]: first position of each level in order
+    uint32_t *lfill;    // [kBcMaxDepth + 1]: positions handed out per level
+    uint32_t *ctl;      // [0]: deepest level reached, [1]: a level beyond kBcMaxDepth
+    // outputs (any may be null):
+    float *dep;         // 0.5 delta(v) at dep[perm ? perm[v] : v]
+    const uint32_t *perm;
+    double *bcsum;      // FP64 sum over sources, device order: = (first) or += 0.5 delta(v)
+    float *bcout;       // with `last`: FP32 copy of bcsum, device order
+    uint32_t first, last;
+};
+
+// For each distinct level among the wave's `valid` lanes, one LDS atomicAdd
+// of its lane count into hist[level] (levels below kBcLdsLevels; deeper
+// levels go to the global counter glob[level]). With `take`, each lane gets
+// the value hist held before its wave's add plus its rank among the wave's
+// lanes of that level (a position). Wave-convergent.
+__device__ __forceinline__ uint32_t wave_level_add(uint32_t *hist, uint32_t *glob, bool valid, uint32_t l, bool take)
+{
+    uint64_t todo = ballot(valid);
+    uint32_t got = 0;
+    while (todo) {
+        const uint32_t l0 = shfl(l, first_lane(todo));
+        const uint64_t m = ballot(valid && l == l0);
+        todo &= ~m;
+        uint32_t base = 0;
+        if (lane() == first_lane(m)) {
+            const uint32_t c = (uint32_t)__popcll(m);
+            base = l0 < kBcLdsLevels ? atomicAdd(&hist[l0], c) : atomicAdd(&glob[l0], c);
+        }
+        base = shfl(base, first_lane(m));
+        if (valid && l == l0) got = base + (uint32_t)rank_in(m);
+    }
+    return take ? got : 0u;
+}
+
+// Sum over the edges [b, e) of each lane's vertex (edge ids into `adj`) of
+// val(neighbour) for neighbours at level `want`, the wave walking the
+// concatenated lists kLevelSteps x 64 edges per step with every load of a
+// stage issued for all sub-steps before any is used (one memory latency
+// covers them all); FP64.
+constexpr int kLevelSteps = 4;
+
+template <typename Val>
+__device__ __forceinline__ double wave_level_sum(const uint32_t *adj, const int32_t *lvl, int32_t want, uint32_t b,
+                                                 uint32_t e, Val val)
+{
+    constexpr int U = kLevelSteps;
+    const int ln = lane();
+    const uint32_t r = e - b;
+    const uint32_t incl = wave_incl_scan(r);
+    const uint32_t total = shfl(incl, kWave - 1);
+    const uint32_t excl = incl - r;
+    double acc = 0.0;
+    for (uint32_t jb = 0; jb < total; jb += U * kWave) {
+        int k[U];
+        uint32_t u[U];
+        bool act[U];
+#pragma unroll
+        for (int s = 0; s < U; s++) {
+            const uint32_t j = jb + (uint32_t)(s * kWave + ln);
+            act[s] = j < total;
+            k[s] = owner_lane(incl, act[s] ? j : 0u);
+            const uint32_t ed = shfl(b, k[s]) + (j - shfl(excl, k[s]));
+            u[s] = act[s] ? adj[ed] : 0u;
+        }
+        bool on[U];
+#pragma unroll
+        for (int s = 0; s < U; s++) on[s] = act[s] && lvl[u[s]] == want;
+        double x[U];
+#pragma unroll
+        for (int s = 0; s < U; s++) x[s] = on[s] ? val(u[s]) : 0.0;
+#pragma unroll
+        for (int s = 0; s < U; s++) {
+            // segmented sum over the lanes of one owner (owners are contiguous)
+            double y = x[s];
+#pragma unroll
+            for (int d = 1; d < kWave; d <<= 1) {
+                const double t = __shfl_up(y, d, kWave);
+                const int ko = __shfl_up(k[s], d, kWave);
+                if (ln >= d && ko == k[s]) y += t;
+            }
+            // this lane's vertex covers positions [excl, incl) of the concatenation
+            const uint32_t base = jb + (uint32_t)(s * kWave);
+            const uint32_t lo = min(max(excl, base), base + 64u) - base;
+            const uint32_t hi = min(max(incl, base), base + 64u) - base;
+            const double seg = __shfl(y, (int)(hi > 0 ? hi - 1 : 0), kWave);
+            if (hi > lo) acc += seg;
+        }
+    }
+    return acc;
+}
+
+template <int BS>
+__device__ __forceinline__ bool bc1_run(const Bc1Args &a, uint32_t brank, uint32_t nblocks, uint32_t xcc,
+                                        BarState &gen)
+{
+    constexpr int WPB = BS / kWave;
+    const DevGraph &g = a.f.g;
+    const uint32_t n = g.n;
+    const uint32_t gtid = brank * BS + threadIdx.x;
+    const uint32_t nthreads = nblocks * BS;
+    const uint32_t gwave = brank * WPB + threadIdx.x / kWave;
+    const uint32_t nwaves = nblocks * WPB;
+    const int ln = lane();
+    const int32_t *lvl = a.f.level;
+
+    __shared__ uint32_t s_hist[kBcLdsLevels];
+    __shared__ uint32_t s_max, s_scan[BS / kWave], s_D;
+
+    // level counters start at zero (the BFS's first barrier publishes this)
+    for (uint32_t i = gtid; i <= kBcMaxDepth; i += nthreads) {
+        a.lcnt[i] = 0;
+        a.lfill[i] = 0;
+    }
+    if (gtid == 0) {
+        a.ctl[0] = 0;
+        a.ctl[1] = 0;
+    }
+    bool ok = bfs_run<BS>(a.f, brank, nblocks, xcc, gen);
+    int32_t status = ok ? SG_OK : SG_ERR_TIMEOUT;
+
+    // ---- per-level counts: block histograms, one atomic per block and level
+    if (ok) {
+        for (uint32_t i = threadIdx.x; i < kBcLdsLevels; i += BS) s_hist[i] = 0;
+        if (threadIdx.x == 0) s_max = 0;
+        __syncthreads();
+        uint32_t lmax = 0;
+        for (uint32_t v0 = brank * BS; v0 < n; v0 += nthreads) {
+            const uint32_t v = v0 + threadIdx.x;
+            const int32_t l = v < n ? lvl[v] : -1;
+            const bool valid = l >= 0 && (uint32_t)l <= kBcMaxDepth;
+            if (valid) lmax = max(lmax, (uint32_t)l);
+            if (l >= 0 && !valid) a.ctl[1] = 1u;
+            wave_level_add(s_hist, a.lcnt, valid, valid ? (uint32_t)l : 0u, false);
+        }
+#pragma unroll
+        for (int d = kWave / 2; d > 0; d >>= 1) lmax = max(lmax, (uint32_t)__shfl_xor(lmax, d, kWave));
+        if (ln == 0) atomicMax(&s_max, lmax);
+        __syncthreads();
+        const uint32_t top = min(s_max, kBcLdsLevels - 1);
+        for (uint32_t l = threadIdx.x; l <= top; l += BS)
+            if (s_hist[l]) atomicAdd(&a.lcnt[l], s_hist[l]);
+        if (threadIdx.x == 0) atomicMax(&a.ctl[0], s_max);
+        ok = grid_sync(a.f.bar, xcc, gen, a.f.abort);
+    }
+    // ---- level offsets: one block scans the counts
+    if (ok) {
+        if (threadIdx.x == 0) {
+            s_D = __hip_atomic_load(&a.ctl[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            s_max = __hip_atomic_load(&a.ctl[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+        __syncthreads();
+        if (s_max) {
+            // a level too deep for the level lists: every block sees the flag
+            ok = false;
+            status = SG_ERR_CAPACITY;
+        }
+    }
+    if (ok) {
+        const uint32_t D = s_D;
+        if (brank == 0) {
+            // thread t sums a contiguous run of levels, then a block scan
+            const uint32_t per = (D + 1 + BS - 1) / BS;
+            const uint32_t l0 = threadIdx.x * per, l1 = min(l0 + per, D + 1);
+            uint32_t sum = 0;
+            for (uint32_t l = l0; l < l1; l++) sum += a.lcnt[l];
+            const uint32_t incl = wave_incl_scan(sum);
+            if (ln == kWave - 1) s_scan[threadIdx.x / kWave] = incl;
+            __syncthreads();
+            uint32_t wbase = 0;
+            for (uint32_t w = 0; w < threadIdx.x / kWave; w++) wbase += s_scan[w];
+            uint32_t pos = wbase + incl - sum;
+            for (uint32_t l = l0; l < l1; l++) {
+                a.loff[l] = pos;
+                pos += a.lcnt[l];
+            }
+            if (threadIdx.x == BS - 1) a.loff[D + 1] = pos;
+        }
+        ok = grid_sync(a.f.bar, xcc, gen, a.f.abort);
+    }
+    // ---- scatter the reached vertices into their levels' ranges; (sigma,
+    // acc) start at (1, 0) for the source and (0, 0) elsewhere
+    if (ok) {
+        for (uint32_t i = threadIdx.x; i < kBcLdsLevels; i += BS) s_hist[i] = 0;
+        __syncthreads();
+        // the same vertices as the count pass, so the same block histogram
+        for (uint32_t v0 = brank * BS; v0 < n; v0 += nthreads) {
+            const uint32_t v = v0 + threadIdx.x;
+            const int32_t l = v < n ? lvl[v] : -1;
+            const bool valid = l >= 0 && (uint32_t)l < kBcLdsLevels;
+            wave_level_add(s_hist, nullptr, valid, valid ? (uint32_t)l : 0u, false);
+        }
+        __syncthreads();
+        const uint32_t top = min(s_D, kBcLdsLevels - 1);
+        for (uint32_t l = threadIdx.x; l <= top; l += BS)
+            if (s_hist[l]) s_hist[l] = a.loff[l] + atomicAdd(&a.lfill[l], s_hist[l]);
+        __syncthreads();
+        for (uint32_t v0 = brank * BS; v0 < n; v0 += nthreads) {
+            const uint32_t v = v0 + threadIdx.x;
+            const int32_t l = v < n ? lvl[v] : -1;
+            const bool valid = l >= 0;
+            // deep levels take positions from the global fill counter, offset below
+            uint32_t pos = wave_level_add(s_hist, a.lfill, valid, valid ? (uint32_t)l : 0u, true);
+            if (valid) {
+                if ((uint32_t)l >= kBcLdsLevels) pos += a.loff[l];
+                a.order[pos] = v;
+                a.sa[v] = make_double2(l == 0 ? 1.0 : 0.0, 0.0);
+            }
+        }
+        ok = grid_sync(a.f.bar, xcc, gen, a.f.abort);
+    }
+    const uint32_t D = ok ? s_D : 0u;
+    // ---- sigma, level by level
+    for (uint32_t L = 1; ok && L <= D; L++) {
+        const uint32_t b0 = a.loff[L], e0 = a.loff[L + 1];
+        for (uint32_t i0 = b0 + gwave * kWave; i0 < e0; i0 += nwaves * kWave) {
+            const uint32_t i = i0 + (uint32_t)ln;
+            uint32_t v = 0, b = 0, e = 0;
+            if (i < e0) {
+                v = a.order[i];
+                b = g.cstart[v];
+                e = g.cstart[v + 1];
+                if (e - b > kPullHubDegree) e = b;  // in-hubs: the chunk pass below
+            }
+            const double s = wave_level_sum(g.csrc, lvl, (int32_t)L - 1, b, e,
+                                            [&](uint32_t u) { return a.sa[u].x; });
+            if (i < e0 && e > b) a.sa[v].x = s;
+        }
+        // in-hub chunks of this level, kPullChunk in-edges each
+        for (uint32_t c0 = 0; c0 < g.nhugein; c0 += nwaves * kWave) {
+            const uint32_t c = c0 + (uint32_t)ln * nwaves + gwave;
+            uint2 dsc = make_uint2(0u, 0u);
+            bool mine = false;
+            if (c < g.nhugein) {
+                dsc = g.hugein[c];
+                mine = lvl[dsc.x] == (int32_t)L;
+            }
+            uint64_t mm = ballot(mine);
+            while (mm) {
+                const int t = first_lane(mm);
+                mm &= mm - 1;
+                const uint32_t v = shfl(dsc.x, t), beg = shfl(dsc.y, t);
+                const uint32_t end = min(g.cstart[v + 1], beg + kPullChunk);
+                double x = 0.0;
+                for (uint32_t e = beg + (uint32_t)ln; e < end; e += kWave) {
+                    const uint32_t u = g.csrc[e];
+                    if (lvl[u] == (int32_t)L - 1) x += a.sa[u].x;
+                }
+                x = wave_sum_f64(x);
+                if (ln == 0 && x != 0.0) atomicAdd(&a.sa[v].x, x);
+            }
+        }
+        ok = grid_sync(a.f.bar, xcc, gen, a.f.abort);
+    }
+    // ---- dependencies, from the deepest level up (the deepest has no
+    // successors: acc stays 0); the source's level needs none
+    for (uint32_t L = D; ok && L-- > 1;) {
+        const uint32_t b0 = a.loff[L], e0 = a.loff[L + 1];
+        auto psi = [&](uint32_t w) {
+            const double2 x = a.sa[w];
+            return 1.0 / x.x + x.y;
+        };
+        for (uint32_t i0 = b0 + gwave * kWave; i0 < e0; i0 += nwaves * kWave) {
+            const uint32_t i = i0 + (uint32_t)ln;
+            uint32_t u = 0, b = 0, e = 0;
+            if (i < e0) {
+                u = a.order[i];
+                b = g.row[u];
+                e = g.row[u + 1];
+                if (e - b > kHugeDegree) e = b;  // out-hubs: the chunk pass below
+            }
+            const double s = wave_level_sum(g.col, lvl, (int32_t)L + 1, b, e, psi);
+            if (i < e0 && e > b) a.sa[u].y = s;
+        }
+        for (uint32_t c0 = 0; c0 < g.nhuge; c0 += nwaves * kWave) {
+            const uint32_t c = c0 + (uint32_t)ln * nwaves + gwave;
+            uint2 dsc = make_uint2(0u, 0u);
+            bool mine = false;
+            if (c < g.nhuge) {
+                dsc = g.huge[c];
+                mine = lvl[dsc.x] == (int32_t)L;
+            }
+            uint64_t mm = ballot(mine);
+            while (mm) {
+                const int t = first_lane(mm);
+                mm &= mm - 1;
+                const uint32_t u = shfl(dsc.x, t), beg = shfl(dsc.y, t);
+                const uint32_t end = min(g.row[u + 1], beg + kHugeChunk);
+                double x = 0.0;
+                for (uint32_t e = beg + (uint32_t)ln; e < end; e += kWave) {
+                    const uint32_t w = g.col[e];
+                    if (lvl[w] == (int32_t)L + 1) x += psi(w);
+                }
+                x = wave_sum_f64(x);
+                if (ln == 0 && x != 0.0) atomicAdd(&a.sa[u].y, x);
+            }
+        }
+        ok = grid_sync(a.f.bar, xcc, gen, a.f.abort);
+    }
+    // ---- 0.5 delta(v) = 0.5 sigma(v) acc(v); 0 for the source and for
+    // unreached vertices
+    if (ok) {
+        for (uint32_t v = gtid; v < n; v += nthreads) {
+            const int32_t l = lvl[v];
+            double h = 0.0;
+            if (l > 0) {
+                const double2 x = a.sa[v];
+                h = 0.5 * x.x * x.y;
+            }
+            if (a.dep) a.dep[a.perm ? a.perm[v] : v] = (float)h;
+            if (a.bcsum) {
+                const double t = a.first ? h : a.bcsum[v] + h;
+                a.bcsum[v] = t;
+                if (a.last && a.bcout) a.bcout[v] = (float)t;
+            }
+        }
+    }
+    if (!ok && status == SG_OK) status = SG_ERR_TIMEOUT;
+    if (gtid == 0 && status != SG_OK) a.f.stats->status = status;
+    return ok;
+}
+
+inline Bc1Layout bc1_layout(uint64_t n)
+{
+    Bc1Layout L{};
+    uint64_t o = 0;
+    auto put = [&](uint64_t &off, uint64_t bytes) {
+        off = o;
+        o += (bytes + 255) / 256 * 256;
+    };
+    put(L.sa, n * 16);
+    put(L.lvl, n * 4);
+    put(L.order, n * 4);
+    put(L.lcnt, (uint64_t)(kBcMaxDepth + 2) * 4);
+    put(L.loff, (uint64_t)(kBcMaxDepth + 2) * 4);
+    put(L.lfill, (uint64_t)(kBcMaxDepth + 2) * 4);
+    put(L.ctl, 256);
+    put(L.fstats, sizeof(DevStats));
+    put(L.bcsum, n * 8);
+    L.total = o;
+    return L;
+}
+
+inline int bc1_fill(const sg_engine *e, char *base, const Bc1Layout &L, uint32_t dev_source, Bc1Args &b)
+{
+    const DevGraph &d = e->g->dev;
+    const sg_engine_options &o = e->opt;
+    const int force = forced_direction(o);
+    if (force < -1) SG_FAIL(SG_ERR_UNSUPPORTED, "BC: only data-driven push/pull variants apply");
+    if (dev_source >= d.n) SG_FAIL(SG_ERR_RANGE, "BC: source >= n");
+    b = Bc1Args{};
+    BfsArgs &f = b.f;
+    f.g = d;
+    f.level = (int32_t *)(base + L.lvl);
+    f.perm = nullptr;
+    f.full_init = 1;
+    f.qcap = e->qcap;
+    f.bar.spin_limit = 1u << 22;
+    f.abort = e->abort_flag;
+    f.stats = (DevStats *)(base + L.fstats);
+    f.trace = nullptr;
+    f.source = dev_source;
+    f.source_id = dev_source;  // levels stay in device order
+    f.force = force;
+    f.alpha = o.alpha == 0.0 ? 1.0 : o.alpha;
+    f.beta = o.beta;
+    f.total_workload0 = (uint32_t)((double)d.m * o.edge_factor);
+    f.max_rounds = 0;  // dependencies need every level
+    b.sa = (double2 *)(base + L.sa);
+    b.order = (uint32_t *)(base + L.order);
+    b.lcnt = (uint32_t *)(base + L.lcnt);
+    b.loff = (uint32_t *)(base + L.loff);
+    b.lfill = (uint32_t *)(base + L.lfill);
+    b.ctl = (uint32_t *)(base + L.ctl);
+    return SG_OK;
+}
+
+}  // namespace bc1k
+}  // namespace

@@ -491,19 +491,24 @@ static hipStream_t g_pk_stream = nullptr;  // stream of the last persistent laun
 
 int engine_launch_persistent(sg_engine *e, const std::function<int()> &launch)
 {
+    return engine_launch_persistent_on(e->stream, launch);
+}
+
+int engine_launch_persistent_on(hipStream_t s, const std::function<int()> &launch)
+{
     std::lock_guard<std::mutex> lk(g_pk_mu);
-    if (g_pk_stream && g_pk_stream != e->stream) {
+    if (g_pk_stream && g_pk_stream != s) {
         // recorded now on the other stream, the event completes no earlier
         // than that stream's last persistent kernel; launches that stay on
         // one stream record nothing (a record costs ~5 us between
         // back-to-back kernels)
         if (!g_pk_last) SG_HIP(hipEventCreateWithFlags(&g_pk_last, hipEventDisableTiming));
         SG_HIP(hipEventRecord(g_pk_last, g_pk_stream));
-        SG_HIP(hipStreamWaitEvent(e->stream, g_pk_last, 0));
+        SG_HIP(hipStreamWaitEvent(s, g_pk_last, 0));
     }
     int rc = launch();
     if (rc) return rc;
-    g_pk_stream = e->stream;
+    g_pk_stream = s;
     return SG_OK;
 }
 

@@ -1134,7 +1134,8 @@ int ms_run(sg_engine *e, int kind, const uint32_t *sources, int32_t k, int32_t b
         SG_FAIL(SG_ERR_UNSUPPORTED, std::string(name) + ": only data-driven push/pull variants apply");
     if (e->opt.lb_push == SG_LB_NONE || e->opt.lb_pull == SG_LB_NONE)
         SG_FAIL(SG_ERR_UNSUPPORTED, std::string(name) + ": LoadBalancing::NONE applies to sg_bfs and sg_sssp only");
-    int rc = ms_prepare(e, kind);
+    const bool single_bc = batch == 1 && (kind == kMsBcDep || kind == kMsBcSum);
+    int rc = single_bc ? engine_ensure_bfs_state(e) : ms_prepare(e, kind);
     if (rc) return rc;
     const uint64_t n = d.n;
     const uint64_t rows = kind == kMsBcSum ? 1 : (uint64_t)k;
@@ -1149,6 +1150,18 @@ int ms_run(sg_engine *e, int kind, const uint32_t *sources, int32_t k, int32_t b
     SG_HIP(hipMemsetAsync(e->stats, 0, sizeof(DevStats), e->stream));
     e->last_algo = 3 + kind;
     SG_HIP(hipEventRecord(e->ev0, e->stream));
+    if (single_bc) {
+        // one source per launch: the single-source kernel (no bit-packed rows)
+        for (int32_t i = 0; i < k; i++) {
+            float *dep = kind == kMsBcDep ? (float *)res + (size_t)i * n : nullptr;
+            float *sum = kind == kMsBcSum ? (float *)res : nullptr;
+            if ((rc = bc1_launch(e, dsrc[i], dep, nullptr, sum, i == 0, i == k - 1))) return rc;
+        }
+        SG_HIP(hipEventRecord(e->ev1, e->stream));
+        if ((rc = engine_finish(e, rows))) return rc;
+        e->timed = true;
+        return SG_OK;
+    }
     const int nb = (k + batch - 1) / batch;
     for (int i = 0; i < nb; i++) {
         const uint32_t ns = (uint32_t)std::min(batch, k - i * batch);

@@ -21,6 +21,13 @@
 // Batches are serialised on the engine's stream: a persistent kernel assumes
 // all of its blocks are co-resident, which two concurrent launches would not
 // be. Host work (queueing, result hand-off) overlaps the running batch.
+//
+// That is the SG_SERVE_BATCHES form. The default, SG_SERVE_SLOTS, keeps one
+// persistent serving kernel (serve_kernel.hip) in which every XCD is a query
+// slot: the worker publishes each admitted query into the kernel's ring at
+// once, oldest first, and a slot that finishes claims the next, so queries
+// that arrive while others run join without waiting for a batch to end.
+#include "serve.hpp"
 #include "sg_internal.hpp"
 
 #include <algorithm>
@@ -64,7 +71,8 @@ constexpr uint32_t kMsBfsMin = 16;
 }  // namespace
 
 struct sg_server {
-    sg_graph *g = nullptr;
+    sg_graph *g = nullptr;                 // the first graph (the engine's)
+    std::vector<sg_graph *> graphs;        // every graph queries may name
     sg_engine *e = nullptr;
     sg_server_config cfg{};
     uint64_t row_bytes = 0;
@@ -73,8 +81,12 @@ struct sg_server {
     uint32_t *d_src = nullptr, *d_rows = nullptr;  // 64 entries each
     uint32_t *h_src = nullptr, *h_rows = nullptr;  // pinned staging
     uint32_t h_qsrc[64] = {};                      // the batch's sources, caller's ids
+    ServeDev *sd = nullptr;                        // SG_SERVE_SLOTS: the serving kernel
+    uint64_t slot_launches = 0;
+    double t_launch = 0.0;
 
     struct Req {
+        int graph;
         int kind;
         uint32_t source;     // device vertex id
         uint32_t source_id;  // the caller's id
@@ -119,8 +131,96 @@ struct sg_server {
         }
     }
 
+    // SG_SERVE_SLOTS worker: publish admitted queries into the serving
+    // kernel's ring (oldest first, as result rows allow), keep the kernel
+    // running while entries are unclaimed, and hand back finished queries.
+    void run_slots()
+    {
+        std::vector<std::pair<uint32_t, uint64_t>> inflight;  // (ring sequence, ticket)
+        std::unique_lock<std::mutex> lk(mu);
+        for (;;) {
+            if (inflight.empty()) cv_work.wait(lk, [&] { return stop || has_work(); });
+            if (stop) return;
+            const double tp = now_s();
+            while (inflight.size() < kServeRing && !free_rows.empty()) {
+                int kind = -1;
+                double oldest = 0.0;
+                for (int k = 0; k < SG_QUERY_KINDS; k++) {
+                    if (queue[k].empty()) continue;
+                    const double t = reqs[queue[k].front()].t_arrival;
+                    if (kind < 0 || t < oldest) {
+                        kind = k;
+                        oldest = t;
+                    }
+                }
+                if (kind < 0) break;
+                const uint64_t t = queue[kind].front();
+                queue[kind].pop_front();
+                Req &r = reqs[t];
+                r.row = (int32_t)free_rows.back();
+                free_rows.pop_back();
+                r.state = kRunning;
+                r.t_start = tp;
+                inflight.emplace_back(serve_publish(sd, r.graph, kind, r.source, r.source_id, (uint32_t)r.row), t);
+            }
+            lk.unlock();
+            const bool was_running = running_;
+            bool running = false;
+            const int rc = serve_pump(sd, &running);
+            const double tn = now_s();
+            lk.lock();
+            if (was_running && !running) busy_s += tn - t_launch;
+            if (running && !running_) {
+                slot_launches++;
+                t_launch = tn;
+            }
+            running_ = running;
+            bool changed = false;
+            for (size_t i = 0; i < inflight.size();) {
+                int32_t st = SG_OK;
+                if (serve_poll(sd, inflight[i].first, &st)) {
+                    const uint64_t t = inflight[i].second;
+                    batched_queries++;
+                    finish_locked(t, reqs[t], st == SG_OK, tn);
+                    inflight[i] = inflight.back();
+                    inflight.pop_back();
+                    changed = true;
+                } else {
+                    i++;
+                }
+            }
+            if (rc != SG_OK && !running) {
+                // the launch that ended failed: queries it had claimed and not
+                // finished are lost; the others stay in the ring
+                const uint32_t head = serve_claimed(sd);
+                for (size_t i = 0; i < inflight.size();) {
+                    if ((int32_t)(head - inflight[i].first) > 0) {
+                        const uint64_t t = inflight[i].second;
+                        finish_locked(t, reqs[t], false, tn);
+                        inflight[i] = inflight.back();
+                        inflight.pop_back();
+                        changed = true;
+                    } else {
+                        i++;
+                    }
+                }
+            }
+            if (changed) {
+                cv_done.notify_all();
+                continue;
+            }
+            // nothing finished: wait briefly, or until a new query arrives
+            if (!inflight.empty()) cv_work.wait_for(lk, std::chrono::microseconds(20));
+        }
+    }
+    bool running_ = false;
+
     void run()
     {
+        if (sd) {
+            run_slots();
+            return;
+        }
         std::vector<uint64_t> tickets;
         for (;;) {
             std::unique_lock<std::mutex> lk(mu);
@@ -220,11 +320,17 @@ extern "C" void sg_server_config_default(sg_server_config *c)
     for (int k = 0; k < SG_QUERY_KINDS; k++) c->max_batch[k] = 64;
 }
 
-extern "C" int sg_server_create(sg_graph *g, const sg_server_config *cfg, const sg_engine_options *opt,
-                                sg_server **out)
+namespace {
+
+int server_create(sg_graph *const *graphs, int32_t ngraphs, const sg_server_config *cfg, const sg_engine_options *opt,
+                  sg_server **out)
 {
-    if (!g || !out) SG_FAIL(SG_ERR_INVALID, "sg_server_create: null argument");
+    if (!graphs || !out) SG_FAIL(SG_ERR_INVALID, "sg_server_create: null argument");
     *out = nullptr;
+    if (ngraphs < 1 || ngraphs > kServeGraphs) SG_FAIL(SG_ERR_RANGE, "sg_server_create: 1..4 graphs");
+    for (int32_t i = 0; i < ngraphs; i++)
+        if (!graphs[i]) SG_FAIL(SG_ERR_INVALID, "sg_server_create: null graph");
+    sg_graph *g = graphs[0];
     sg_server_config c;
     if (cfg) c = *cfg;
     else sg_server_config_default(&c);
@@ -232,15 +338,21 @@ extern "C" int sg_server_create(sg_graph *g, const sg_server_config *cfg, const 
         if (c.max_batch[k] < 0 || c.max_batch[k] > 64) SG_FAIL(SG_ERR_RANGE, "sg_server_create: max_batch in [0, 64]");
     if (c.bfs_packed_min < 0 || c.bfs_packed_min > 65 || c.sssp_packed_min < 0 || c.sssp_packed_min > 65)
         SG_FAIL(SG_ERR_RANGE, "sg_server_create: packed_min in [0, 65]");
+    if (c.mode != SG_SERVE_SLOTS && c.mode != SG_SERVE_BATCHES) SG_FAIL(SG_ERR_INVALID, "sg_server_create: unknown mode");
+    if (c.mode == SG_SERVE_BATCHES && ngraphs > 1)
+        SG_FAIL(SG_ERR_UNSUPPORTED, "sg_server_create: several graphs need SG_SERVE_SLOTS");
     if (!c.bfs_packed_min) c.bfs_packed_min = (int32_t)kMsBfsMin;
     if (!c.sssp_packed_min) c.sssp_packed_min = (int32_t)kMsSsspMin;
-    const uint64_t n = g->dev.n;
+    // a result row holds one query's values on the largest graph
+    uint64_t n = 0;
+    for (int32_t i = 0; i < ngraphs; i++) n = std::max<uint64_t>(n, graphs[i]->dev.n);
     const uint64_t row_bytes = n * 4;
     uint64_t rows = c.memory_budget / row_bytes;
     if (c.max_rows) rows = std::min<uint64_t>(rows, (uint64_t)c.max_rows);
     if (rows < 1) SG_FAIL(SG_ERR_CAPACITY, "sg_server_create: memory budget below one result row");
     std::unique_ptr<sg_server> s(new sg_server());
     s->g = g;
+    s->graphs.assign(graphs, graphs + ngraphs);
     s->cfg = c;
     s->row_bytes = row_bytes;
     s->nrows = (uint32_t)std::min<uint64_t>(rows, 0xFFFFFFFFull);
@@ -255,14 +367,19 @@ extern "C" int sg_server_create(sg_graph *g, const sg_server_config *cfg, const 
         sg_server_destroy(p);
         return code;
     };
-    // working arenas of every kind (the largest wins) and the result pool;
-    // small SSSP batches run single-source queries on the engine's own
-    // frontier state, everything else the bit-packed kernels
-    if ((rc = ms_prepare(s->e, kMsBfs))) return fail(rc);
-    if ((rc = ms_prepare(s->e, kMsSssp))) return fail(rc);
-    if ((rc = ms_prepare(s->e, kMsBcDep))) return fail(rc);
-    if (g->relabeled && (rc = engine_ensure_work(s->e, 64ull * g->dev.n * 4))) return fail(rc);
+    if (c.mode == SG_SERVE_BATCHES) {
+        // working arenas of every kind (the largest wins); small SSSP batches
+        // run single-source queries on the engine's own frontier state,
+        // everything else the bit-packed kernels
+        if ((rc = ms_prepare(s->e, kMsBfs))) return fail(rc);
+        if ((rc = ms_prepare(s->e, kMsSssp))) return fail(rc);
+        if ((rc = ms_prepare(s->e, kMsBcDep))) return fail(rc);
+        if (g->relabeled && (rc = engine_ensure_work(s->e, 64ull * g->dev.n * 4))) return fail(rc);
+    }
     if ((rc = engine_alloc(&s->pool, (uint64_t)s->nrows * row_bytes))) return fail(rc);
+    if (c.mode == SG_SERVE_SLOTS &&
+        (rc = serve_create(graphs, ngraphs, s->e->opt, s->e->stream, s->pool, row_bytes, &s->sd)))
+        return fail(rc);
     if ((rc = engine_alloc((void **)&s->d_src, 256))) return fail(rc);
     if ((rc = engine_alloc((void **)&s->d_rows, 256))) return fail(rc);
     if (hipHostMalloc((void **)&s->h_src, 256, hipHostMallocDefault) != hipSuccess ||
@@ -279,6 +396,21 @@ extern "C" int sg_server_create(sg_graph *g, const sg_server_config *cfg, const 
     return SG_OK;
 }
 
+}  // namespace
+
+extern "C" int sg_server_create(sg_graph *g, const sg_server_config *cfg, const sg_engine_options *opt,
+                                sg_server **out)
+{
+    if (!g || !out) SG_FAIL(SG_ERR_INVALID, "sg_server_create: null argument");
+    return server_create(&g, 1, cfg, opt, out);
+}
+
+extern "C" int sg_server_create_multi(sg_graph *const *graphs, int32_t ngraphs, const sg_server_config *cfg,
+                                      const sg_engine_options *opt, sg_server **out)
+{
+    return server_create(graphs, ngraphs, cfg, opt, out);
+}
+
 extern "C" int sg_server_destroy(sg_server *s)
 {
     if (!s) return SG_OK;
@@ -288,6 +420,7 @@ extern "C" int sg_server_destroy(sg_server *s)
     }
     s->cv_work.notify_all();
     if (s->worker.joinable()) s->worker.join();
+    if (s->sd) serve_destroy(s->sd);  // lets a running launch drain the ring, waits for it
     if (s->e) {
         (void)hipStreamSynchronize(s->e->stream);
         sg_engine_destroy(s->e);
@@ -303,13 +436,14 @@ extern "C" int sg_server_destroy(sg_server *s)
 
 namespace {
 
-int submit_locked(sg_server *s, int32_t kind, uint32_t source, double t_arrival, bool auto_release, double *lat,
-                  uint64_t *ticket)
+int submit_locked(sg_server *s, int32_t graph, int32_t kind, uint32_t source, double t_arrival, bool auto_release,
+                  double *lat, uint64_t *ticket)
 {
     const uint64_t t = s->next_ticket++;
     sg_server::Req r{};
+    r.graph = graph;
     r.kind = kind;
-    r.source = dev_vertex(s->g, source);  // device vertex order
+    r.source = dev_vertex(s->graphs[graph], source);  // device vertex order
     r.source_id = source;
     r.row = -1;
     r.state = kQueued;
@@ -326,17 +460,23 @@ int submit_locked(sg_server *s, int32_t kind, uint32_t source, double t_arrival,
 
 }  // namespace
 
-extern "C" int sg_server_submit(sg_server *s, int32_t kind, uint32_t source, uint64_t *ticket)
+extern "C" int sg_server_submit_to(sg_server *s, int32_t graph, int32_t kind, uint32_t source, uint64_t *ticket)
 {
     if (!s || !ticket) SG_FAIL(SG_ERR_INVALID, "sg_server_submit: null argument");
+    if (graph < 0 || graph >= (int32_t)s->graphs.size()) SG_FAIL(SG_ERR_RANGE, "sg_server_submit: unknown graph");
     if (kind < 0 || kind >= SG_QUERY_KINDS) SG_FAIL(SG_ERR_INVALID, "sg_server_submit: unknown query kind");
-    if (source >= s->g->dev.n) SG_FAIL(SG_ERR_RANGE, "sg_server_submit: source >= n");
+    if (source >= s->graphs[graph]->dev.n) SG_FAIL(SG_ERR_RANGE, "sg_server_submit: source >= n");
     {
         std::lock_guard<std::mutex> lk(s->mu);
-        submit_locked(s, kind, source, now_s(), false, nullptr, ticket);
+        submit_locked(s, graph, kind, source, now_s(), false, nullptr, ticket);
     }
     s->cv_work.notify_one();
     return SG_OK;
+}
+
+extern "C" int sg_server_submit(sg_server *s, int32_t kind, uint32_t source, uint64_t *ticket)
+{
+    return sg_server_submit_to(s, 0, kind, source, ticket);
 }
 
 extern "C" int sg_server_wait(sg_server *s, uint64_t ticket, int32_t timeout_ms)
@@ -367,7 +507,7 @@ extern "C" int sg_server_result(sg_server *s, uint64_t ticket, void **dev_ptr, u
     const sg_server::Req &r = it->second;
     if (r.state != kDone) SG_FAIL(SG_ERR_STATE, "sg_server_result: query not finished");
     if (dev_ptr) *dev_ptr = (char *)s->pool + (size_t)r.row * s->row_bytes;
-    if (bytes) *bytes = s->row_bytes;
+    if (bytes) *bytes = (uint64_t)s->graphs[r.graph]->dev.n * 4;
     if (queue_ms) *queue_ms = (r.t_start - r.t_arrival) * 1e3;
     if (total_ms) *total_ms = (r.t_done - r.t_arrival) * 1e3;
     return SG_OK;
@@ -407,25 +547,32 @@ extern "C" int sg_server_get_stats(sg_server *s, sg_server_stats *out)
     out->submitted = s->submitted;
     out->completed = s->completed;
     out->failed = s->failed;
-    out->batches = s->batches;
-    out->mean_batch = s->batches ? (double)s->batched_queries / (double)s->batches : 0.0;
+    // slots: launches of the serving kernel, each serving many queries
+    const uint64_t nb = s->sd ? s->slot_launches : s->batches;
+    out->batches = nb;
+    out->mean_batch = nb ? (double)s->batched_queries / (double)nb : 0.0;
     out->busy_ms = s->busy_s * 1e3;
     out->rows = s->nrows;
     out->free_rows = (uint32_t)s->free_rows.size();
     uint64_t q = 0;
     for (int k = 0; k < SG_QUERY_KINDS; k++) q += s->queue[k].size();
     out->queued = q;
-    out->device_bytes = s->g->device_bytes + s->e->aux_bytes + (uint64_t)s->nrows * s->row_bytes + s->e->value_cap;
+    uint64_t gbytes = 0;
+    for (const sg_graph *g : s->graphs) gbytes += g->device_bytes;
+    out->device_bytes = gbytes + s->e->aux_bytes + (uint64_t)s->nrows * s->row_bytes + s->e->value_cap +
+                        (s->sd ? serve_device_bytes(s->sd) : 0);
     return SG_OK;
 }
 
-extern "C" int sg_server_replay(sg_server *s, const int32_t *kinds, const uint32_t *sources, const double *arrival_s,
-                                int64_t count, double *latency_ms)
+extern "C" int sg_server_replay_multi(sg_server *s, const int32_t *graphs, const int32_t *kinds, const uint32_t *sources,
+                                      const double *arrival_s, int64_t count, double *latency_ms)
 {
     if (!s || !kinds || !sources || !arrival_s || !latency_ms) SG_FAIL(SG_ERR_INVALID, "sg_server_replay: null argument");
     for (int64_t i = 0; i < count; i++) {
+        const int32_t gi = graphs ? graphs[i] : 0;
+        if (gi < 0 || gi >= (int32_t)s->graphs.size()) SG_FAIL(SG_ERR_RANGE, "sg_server_replay: unknown graph");
         if (kinds[i] < 0 || kinds[i] >= SG_QUERY_KINDS) SG_FAIL(SG_ERR_INVALID, "sg_server_replay: unknown query kind");
-        if (sources[i] >= s->g->dev.n) SG_FAIL(SG_ERR_RANGE, "sg_server_replay: source >= n");
+        if (sources[i] >= s->graphs[gi]->dev.n) SG_FAIL(SG_ERR_RANGE, "sg_server_replay: source >= n");
         if (i && arrival_s[i] < arrival_s[i - 1]) SG_FAIL(SG_ERR_INVALID, "sg_server_replay: arrivals must not decrease");
         latency_ms[i] = -1.0;
     }
@@ -447,7 +594,7 @@ extern "C" int sg_server_replay(sg_server *s, const int32_t *kinds, const uint32
         }
         {
             std::lock_guard<std::mutex> lk(s->mu);
-            submit_locked(s, kinds[i], sources[i], due, true, &latency_ms[i], nullptr);
+            submit_locked(s, graphs ? graphs[i] : 0, kinds[i], sources[i], due, true, &latency_ms[i], nullptr);
         }
         s->cv_work.notify_one();
     }
@@ -455,4 +602,10 @@ extern "C" int sg_server_replay(sg_server *s, const int32_t *kinds, const uint32
     s->cv_done.wait(lk, [&] { return s->completed + s->failed - base_completed - base_failed >= (uint64_t)count; });
     if (s->failed != base_failed) SG_FAIL(SG_ERR_HIP, "sg_server_replay: a batch failed");
     return SG_OK;
+}
+
+extern "C" int sg_server_replay(sg_server *s, const int32_t *kinds, const uint32_t *sources, const double *arrival_s,
+                                int64_t count, double *latency_ms)
+{
+    return sg_server_replay_multi(s, nullptr, kinds, sources, arrival_s, count, latency_ms);
 }

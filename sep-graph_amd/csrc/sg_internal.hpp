@@ -5,6 +5,7 @@
 
 #include <cstdint>
 #include <functional>
+#include <mutex>
 #include <string>
 
 #include "../../include/sepgraph.h"
@@ -226,6 +227,20 @@ int ms_prepare(sg_engine *e, int kind);
 int ms_launch(sg_engine *e, int kind, const uint32_t *dev_src, uint32_t nsrc, void *out, const uint32_t *dev_orow,
               bool first, bool last);
 
+// Working arena of one single-source BC run (byte offsets).
+struct Bc1Layout {
+    uint64_t sa, lvl, order, lcnt, loff, lfill, ctl, fstats, bcsum, total;
+};
+
+// Single-source BC from device vertex `dev_source` over the whole grid
+// (asynchronous): `dep` (optional) receives 0.5 delta, in device order, or at
+// dep[perm[v]] when perm is given; with `bcout`, an FP64 sum kept in the
+// engine's arena is started (first) or extended and the last source writes it
+// to bcout as FP32 in device order. The engine's stats sum rounds and count
+// sources over the launches from `first` on.
+int bc1_launch(sg_engine *e, uint32_t dev_source, float *dep, const uint32_t *perm, float *bcout, bool first,
+               bool last);
+
 // Up to one SSSP query per XCD in one launch (the XCD's blocks form the
 // query's grid); query i writes its distances to the device row out[i].
 int sssp_group_launch(sg_engine *e, const uint32_t *sources, int nq, uint32_t *const *out);
@@ -273,4 +288,6 @@ int engine_launch_persistent(sg_engine *e, const std::function<int()> &launch);
 // Drop the engine's stream from that ordering (the stream was synchronised:
 // its persistent kernels are done), before the stream is destroyed or replaced.
 void engine_forget_stream(sg_engine *e);
+// engine_launch_persistent for a stream that belongs to no engine.
+int engine_launch_persistent_on(hipStream_t s, const std::function<int()> &launch);
 }  // namespace sg

@@ -604,7 +604,9 @@ inline std::vector<double> HostBc(const sepgraph::graphs::CSRGraph &g, uint32_t 
             if (dist[v] == dist[u] + 1) sigma[v] += sigma[u];
         }
     }
-    for (size_t i = order.size(); i-- > 0;) {
+    // the source's own dependency is not part of its BC (the reference's
+    // backward loop stops above depth 0, hybrid_bc_host.cpp:64-77)
+    for (size_t i = order.size(); i-- > 1;) {
         const uint32_t u = order[i];
         for (uint32_t e = g.row_start[u]; e < g.row_start[u + 1]; e++) {
             const uint32_t v = g.edge_dst[e];

@@ -490,7 +490,8 @@ class Engine:
         rows = (RoundTrace * max_rows)()
         nrow = C.c_int()
         _check(lib().sg_engine_get_trace(self.h, rows, max_rows, C.byref(nrow)))
-        inv = {v: k for k, v in VARIANTS.items()}
+        # variant -1 marks the final row (not SG_VARIANT_AUTO)
+        inv = {v: k for k, v in VARIANTS.items() if v >= 0}
         return [
             {"round": r.round, "variant": inv.get(r.variant, "end"), "in_nodes": r.in_nodes,
              "in_edges": r.in_edges, "time_us": r.time_us}
@@ -514,6 +515,9 @@ QUERY_BFS, QUERY_SSSP, QUERY_BC_DEP = 0, 1, 2
 _QUERY_DTYPE = {QUERY_BFS: np.int32, QUERY_SSSP: np.uint32, QUERY_BC_DEP: np.float32}
 
 
+SERVE_SLOTS, SERVE_BATCHES = 0, 1
+
+
 class ServerConfig(C.Structure):
     _fields_ = [
         ("memory_budget", C.c_uint64),
@@ -521,7 +525,8 @@ class ServerConfig(C.Structure):
         ("max_batch", C.c_int32 * 3),
         ("bfs_packed_min", C.c_int32),
         ("sssp_packed_min", C.c_int32),
-        ("reserved", C.c_int32 * 6),
+        ("mode", C.c_int32),
+        ("reserved", C.c_int32 * 5),
     ]
 
 
@@ -563,6 +568,10 @@ def _server_lib():
             "sg_server_release": ([_P, C.c_uint64], C.c_int),
             "sg_server_get_stats": ([_P, C.POINTER(ServerStats)], C.c_int),
             "sg_server_replay": ([_P, _P, _P, _P, C.c_int64, _P], C.c_int),
+            "sg_server_create_multi": ([_P, C.c_int32, C.POINTER(ServerConfig), C.POINTER(EngineOptions),
+                                        C.POINTER(_P)], C.c_int),
+            "sg_server_submit_to": ([_P, C.c_int32, C.c_int32, C.c_uint32, C.POINTER(C.c_uint64)], C.c_int),
+            "sg_server_replay_multi": ([_P, _P, _P, _P, _P, C.c_int64, _P], C.c_int),
         }
         for name, (args, res) in sig.items():
             f = getattr(L, name)
@@ -575,11 +584,18 @@ def _server_lib():
 class Server:
     """sg_server: batched BFS / SSSP / BC-dependency queries on a resident graph."""
 
-    def __init__(self, graph: Graph, memory_budget: int = 4 << 30, max_rows: int = 4096, max_batch=(64, 64, 64),
-                 bfs_packed_min: int = 0, sssp_packed_min: int = 0, **opts):
+    def __init__(self, graph, memory_budget: int = 4 << 30, max_rows: int = 4096, max_batch=(64, 64, 64),
+                 bfs_packed_min: int = 0, sssp_packed_min: int = 0, mode: int = 0, **opts):
+        """graph: a Graph, or a list of up to four Graphs (queries then name
+        theirs by index). mode: SERVE_SLOTS (0, the serving kernel with one
+        query slot per XCD) or SERVE_BATCHES (1, bit-packed batches formed
+        when the GPU frees up; max_batch and the packed_min thresholds apply
+        to it; one graph only)."""
         L = _server_lib()
-        self.graph = graph
-        self.n = graph.host.n
+        graphs = list(graph) if isinstance(graph, (list, tuple)) else [graph]
+        self.graphs = graphs
+        self.graph = graphs[0]
+        self.n = graphs[0].host.n
         cfg = ServerConfig()
         L.sg_server_config_default(C.byref(cfg))
         cfg.memory_budget = memory_budget
@@ -588,21 +604,26 @@ class Server:
             cfg.max_batch[i] = b
         cfg.bfs_packed_min = bfs_packed_min
         cfg.sssp_packed_min = sssp_packed_min
+        cfg.mode = mode
         o = default_options(**opts)
         h = _P()
-        _check(L.sg_server_create(graph.h, C.byref(cfg), C.byref(o), C.byref(h)))
+        arr = (_P * len(graphs))(*[gr.h for gr in graphs])
+        _check(L.sg_server_create_multi(arr, len(graphs), C.byref(cfg), C.byref(o), C.byref(h)))
         self.h = h
+        self._kind_graph = {}
 
-    def submit(self, kind: int, source: int) -> int:
+    def submit(self, kind: int, source: int, graph: int = 0) -> int:
         t = C.c_uint64()
-        _check(_server_lib().sg_server_submit(self.h, int(kind), int(source), C.byref(t)))
+        _check(_server_lib().sg_server_submit_to(self.h, int(graph), int(kind), int(source), C.byref(t)))
+        self._kind_graph[t.value] = int(graph)
         return t.value
 
     def wait(self, ticket: int, timeout_ms: int = -1):
         _check(_server_lib().sg_server_wait(self.h, ticket, timeout_ms))
 
     def result(self, ticket: int, kind: int) -> np.ndarray:
-        out = np.empty(self.n, _QUERY_DTYPE[kind])
+        n = self.graphs[self._kind_graph.get(ticket, 0)].host.n
+        out = np.empty(n, _QUERY_DTYPE[kind])
         _check(_server_lib().sg_server_copy_result(self.h, ticket, _ptr(out), out.nbytes))
         return out
 
@@ -613,18 +634,21 @@ class Server:
 
     def release(self, ticket: int):
         _check(_server_lib().sg_server_release(self.h, ticket))
+        self._kind_graph.pop(ticket, None)
 
     def stats(self) -> dict:
         s = ServerStats()
         _check(_server_lib().sg_server_get_stats(self.h, C.byref(s)))
         return s.as_dict()
 
-    def replay(self, kinds, sources, arrival_s) -> np.ndarray:
+    def replay(self, kinds, sources, arrival_s, graphs=None) -> np.ndarray:
+        """Open-loop arrivals; graphs[i] (optional) is query i's graph index."""
         k = np.ascontiguousarray(kinds, np.int32)
         s = _u32(sources)
         a = np.ascontiguousarray(arrival_s, np.float64)
+        gi = None if graphs is None else np.ascontiguousarray(graphs, np.int32)
         lat = np.empty(len(k), np.float64)
-        _check(_server_lib().sg_server_replay(self.h, _ptr(k), _ptr(s), _ptr(a), len(k), _ptr(lat)))
+        _check(_server_lib().sg_server_replay_multi(self.h, _ptr(gi), _ptr(k), _ptr(s), _ptr(a), len(k), _ptr(lat)))
         return lat
 
     def close(self):

@@ -256,7 +256,7 @@ def test_pr_iterative_forced(sg, orc, gpu, variant):
     x, st, e = run_pr(sg, hg, sg.PR_ITERATIVE, tol=1e-8, force_variant=variant)
     ref, _, _ = orc.pagerank(hg.row, hg.col, D, 1e-13, 100000)
     assert rel_l1(x, ref) <= 1e-6, st
-    rows = [r for r in e.trace(4096) if r["variant"] != "auto"]  # the closing row has no variant
+    rows = [r for r in e.trace(4096) if r["variant"] != "end"]  # the closing row has no variant
     assert len(rows) == st["rounds"] and all(r["variant"] == variant for r in rows)
 
 

@@ -1,9 +1,11 @@
 """Serving: continuous batching of BFS / SSSP / BC-dependency queries through
-the C ABI. Every result must equal the same query run alone through the
-single-query entry points (bit-exact for BFS levels and SSSP distances,
-1e-5 relative for BC dependencies, whose FP64 sums may associate differently
-in a different batch), whatever the batch it landed in; admission must hold
-queries back while no result row is free."""
+the C ABI, in both forms -- the serving kernel with one query slot per XCD
+(SERVE_SLOTS) and bit-packed batches (SERVE_BATCHES). Every result must equal
+the same query run alone through the single-query entry points (bit-exact for
+BFS levels and SSSP distances, 1e-5 relative for BC dependencies, whose FP64
+sums may associate differently in a different batch or slot), whatever
+launch it landed in; admission must hold queries back while no result row is
+free."""
 import threading
 
 import numpy as np
@@ -34,9 +36,14 @@ def same(sg, kind, got, ref):
     return np.array_equal(got, ref)
 
 
-def test_server_mix_equals_alone(sg, gpu, wgraph):
+MODES = [0, 1]  # SERVE_SLOTS, SERVE_BATCHES
+
+
+@pytest.mark.parametrize("mode", MODES)
+def test_server_mix_equals_alone(sg, gpu, wgraph, mode):
+    print(f"[server] mix, mode {mode}", flush=True)
     g = sg.Graph(wgraph)
-    srv = sg.Server(g, max_rows=512)
+    srv = sg.Server(g, max_rows=512, mode=mode)
     rng = np.random.default_rng(4)
     srcs = sg.pick_sources(wgraph.row, 150, seed=5)
     kinds = rng.choice([0, 1, 2], size=150, p=[0.6, 0.3, 0.1])
@@ -68,10 +75,12 @@ def test_server_mix_equals_alone(sg, gpu, wgraph):
     srv.close()
 
 
-def test_server_admission_by_rows(sg, gpu, wgraph):
+@pytest.mark.parametrize("mode", MODES)
+def test_server_admission_by_rows(sg, gpu, wgraph, mode):
+    print(f"[server] admission, mode {mode}", flush=True)
     g = sg.Graph(wgraph)
     n = wgraph.n
-    srv = sg.Server(g, memory_budget=3 * n * 4, max_rows=0)
+    srv = sg.Server(g, memory_budget=3 * n * 4, max_rows=0, mode=mode)
     assert srv.stats()["rows"] == 3
     srcs = sg.pick_sources(wgraph.row, 8, seed=6)
     t = [srv.submit(sg.QUERY_BFS, int(s)) for s in srcs]
@@ -102,7 +111,7 @@ def test_server_batch_forms(sg, gpu, wgraph, packed_min):
     65: every batch runs as single whole-GPU queries. Results equal the lone
     entry points either way."""
     g = sg.Graph(wgraph)
-    srv = sg.Server(g, max_rows=256, bfs_packed_min=packed_min, sssp_packed_min=packed_min)
+    srv = sg.Server(g, max_rows=256, bfs_packed_min=packed_min, sssp_packed_min=packed_min, mode=sg.SERVE_BATCHES)
     srcs = sg.pick_sources(wgraph.row, 48, seed=11)
     kinds = [sg.QUERY_BFS, sg.QUERY_SSSP] * 24
     tickets = [srv.submit(k, int(s)) for k, s in zip(kinds, srcs)]
@@ -120,11 +129,15 @@ def test_server_batch_forms(sg, gpu, wgraph, packed_min):
         assert same(sg, k, got, alone(sg, e, k, s)), (packed_min, k, int(s))
     with pytest.raises(sg.SepGraphError):
         sg.Server(g, bfs_packed_min=66)
+    with pytest.raises(sg.SepGraphError):
+        sg.Server(g, mode=2)
 
 
-def test_server_replay(sg, gpu, wgraph):
+@pytest.mark.parametrize("mode", MODES)
+def test_server_replay(sg, gpu, wgraph, mode):
+    print(f"[server] replay, mode {mode}", flush=True)
     g = sg.Graph(wgraph)
-    srv = sg.Server(g, max_rows=1024)
+    srv = sg.Server(g, max_rows=1024, mode=mode)
     rng = np.random.default_rng(8)
     count = 2000
     kinds = rng.choice([0, 1, 2], size=count, p=[0.6, 0.3, 0.1])
@@ -136,3 +149,41 @@ def test_server_replay(sg, gpu, wgraph):
     assert st["completed"] == count and st["free_rows"] == st["rows"]
     assert st["mean_batch"] > 1.0
     srv.close()
+
+
+def test_server_two_graphs(sg, gpu, wgraph):
+    """One serving kernel over an RMAT graph and a grid resident together:
+    queries on either graph, mixed in one arrival stream, equal lone runs on
+    their own graph."""
+    print("[server] two graphs", flush=True)
+    grid = sg.grid_graph(48, 40, p_delete=0.1, seed=3, weights=(1, 1000))
+    ga, gb = sg.Graph(wgraph), sg.Graph(grid)
+    srv = sg.Server([ga, gb], max_rows=128)
+    rng = np.random.default_rng(12)
+    sa = sg.pick_sources(wgraph.row, 40, seed=13)
+    sb = sg.pick_sources(grid.row, 40, seed=14)
+    which = rng.integers(0, 2, size=80)
+    kinds = rng.choice([0, 1, 2], size=80, p=[0.6, 0.3, 0.1])
+    plan = []
+    ia = ib = 0
+    for w, k in zip(which, kinds):
+        if w == 0:
+            plan.append((0, int(k), int(sa[ia])))
+            ia += 1
+        else:
+            plan.append((1, int(k), int(sb[ib])))
+            ib += 1
+    tickets = [srv.submit(k, s, graph=gi) for gi, k, s in plan]
+    results = []
+    for t, (gi, k, s) in zip(tickets, plan):
+        srv.wait(t, 60000)
+        results.append(srv.result(t, k))
+        srv.release(t)
+    assert srv.stats()["failed"] == 0
+    srv.close()
+    engines = [sg.Engine(ga), sg.Engine(gb)]
+    for got, (gi, k, s) in zip(results, plan):
+        assert len(got) == (wgraph.n, grid.n)[gi]
+        assert same(sg, k, got, alone(sg, engines[gi], k, s)), (gi, k, s)
+    with pytest.raises(sg.SepGraphError):
+        sg.Server([ga, gb], mode=sg.SERVE_BATCHES)
