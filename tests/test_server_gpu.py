@@ -160,8 +160,8 @@ def test_server_two_graphs(sg, gpu, wgraph):
     ga, gb = sg.Graph(wgraph), sg.Graph(grid)
     srv = sg.Server([ga, gb], max_rows=128)
     rng = np.random.default_rng(12)
-    sa = sg.pick_sources(wgraph.row, 40, seed=13)
-    sb = sg.pick_sources(grid.row, 40, seed=14)
+    sa = sg.pick_sources(wgraph.row, 80, seed=13)
+    sb = sg.pick_sources(grid.row, 80, seed=14)
     which = rng.integers(0, 2, size=80)
     kinds = rng.choice([0, 1, 2], size=80, p=[0.6, 0.3, 0.1])
     plan = []
