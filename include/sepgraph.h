@@ -316,6 +316,24 @@ int sg_engine_order_launch(sg_engine *e);
  * (include/framework/hybrid_policy.h:237-291). Asynchronous. */
 int sg_bfs(sg_engine *e, uint32_t source);
 
+/* sg_bfs for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc). No
+ * counterpart in the reference. It takes no engine and allocates nothing: the
+ * caller passes the graph, the source and `levels`, a device buffer of n
+ * int32 that receives the levels in the caller's vertex order (-1 when
+ * unreached), and the HIP stream (as void*; NULL for the default stream) the
+ * kernel runs on. opt may be NULL (the defaults: alpha 15, beta 18, auto
+ * direction); block_size must be 0 or 1024 and grid_blocks 0. The kernel's
+ * working state lives in a fixed device array of about 17 MiB that the
+ * library holds on each GPU it runs on until the process ends, so graphs are
+ * limited to n <= 2^23 vertices (SG_ERR_CAPACITY beyond). The first call on a
+ * device sizes the persistent grid (a residency probe). Synchronous: returns
+ * once the levels are written, with the device-side status. Calls from one
+ * process are serialised per device, since they share that array; each call
+ * resets the state it reads. Results equal sg_bfs's. */
+int sg_bfs_noalloc(const sg_graph *g, uint32_t source, int32_t *levels, void *hip_stream,
+                   const sg_engine_options *opt);
+
 /* Single-source shortest paths from `source` (hybrid_sssp,
  * samples/hybrid_sssp/hybrid_sssp.cu:151-242): uint32 distances, UINT32_MAX
  * for unreached, relaxation d(u)+w(u,v) with atomicMin as the reference's

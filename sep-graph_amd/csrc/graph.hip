@@ -518,6 +518,19 @@ void engine_forget_stream(sg_engine *e)
     if (g_pk_stream == e->stream) g_pk_stream = nullptr;
 }
 
+std::mutex &persistent_mutex() { return g_pk_mu; }
+
+int persistent_wait_other(hipStream_t s)
+{
+    // the previous persistent kernel went to another stream: wait for that
+    // stream on the host (no event is created, nothing is allocated); the
+    // caller completes its own kernel before releasing the lock, so later
+    // launches need not wait for it
+    if (g_pk_stream && g_pk_stream != s) SG_HIP(hipStreamSynchronize(g_pk_stream));
+    g_pk_stream = nullptr;
+    return SG_OK;
+}
+
 // Probe one grid size: true when every block was resident at once. The
 // barrier words are zeroed first, so the grid's launches start from clean
 // counters (every launch takes its own per-XCD census).

@@ -288,6 +288,13 @@ int engine_launch_persistent(sg_engine *e, const std::function<int()> &launch);
 // Drop the engine's stream from that ordering (the stream was synchronised:
 // its persistent kernels are done), before the stream is destroyed or replaced.
 void engine_forget_stream(sg_engine *e);
+// The lock behind that ordering, for a caller that enqueues a persistent
+// kernel on a stream of its own and waits for it to finish before releasing
+// the lock (the allocation-free BFS).
+std::mutex &persistent_mutex();
+// With persistent_mutex() held: wait on the host for the device's previous
+// persistent kernel when it went to a stream other than `s`, and forget it.
+int persistent_wait_other(hipStream_t s);
 // engine_launch_persistent for a stream that belongs to no engine.
 int engine_launch_persistent_on(hipStream_t s, const std::function<int()> &launch);
 }  // namespace sg
