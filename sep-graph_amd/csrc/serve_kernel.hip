@@ -13,8 +13,11 @@
 // entry. So queries that arrive while others run are admitted as soon as a
 // slot frees up, and up to one query per XCD runs at a time, each keeping
 // its working set in its XCD's L2. Slots leave when the host asks them to
-// and the ring is drained, or after a stretch with no work (the host
-// relaunches the kernel when more arrives), or when any bounded wait expires.
+// and the ring is drained, when every slot of the launch has found no work
+// for a stretch (the host relaunches the kernel when more arrives), or when
+// any bounded wait expires. A slot never leaves alone while others work: a
+// launch whose idle slots had left would serve later arrivals with fewer
+// slots until its last query ended.
 #include "bc1_dev.hpp"
 #include "serve.hpp"
 #include "sssp_dev.hpp"
@@ -60,6 +63,7 @@ struct ServeArgs {
     const uint32_t *stop;   // host memory: leave once the ring is drained
     unsigned long long *done;  // host memory: per entry, (seq + 1) | status << 32
     uint32_t *head;         // entries claimed
+    uint32_t *idle;         // [0]: slots voting to end this launch, [1]: the launch is closing (zeroed per launch)
     Mail *mail;             // kMaxXcd
     SlotCtl *ctl;           // kMaxXcd
     char *slots;            // one arena of sg::ServeLayout per slot
@@ -80,19 +84,27 @@ __device__ __forceinline__ uint32_t sys_load(const uint32_t *p)
     return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
 }
 
-// The slot leader waits (bounded by the idle limit) for a ring entry and
-// claims it; kExit when the host asked to stop and nothing is left, when
-// nothing arrived within the idle limit, or after an abort.
-__device__ Mail claim_next(const ServeArgs &a, uint32_t xcc)
+// The slot leader waits for a ring entry and claims it; kExit when the
+// host asked to stop and nothing is left, after an abort, or when the launch
+// is closing. A slot that has found nothing for the idle limit votes to
+// close; the vote that makes every one of the launch's `nlive` slots idle
+// closes the launch. A slot withdraws its vote before claiming, so a claim
+// can only race with the closing vote, and a query claimed then still runs.
+__device__ Mail claim_next(const ServeArgs &a, uint32_t xcc, uint32_t nlive)
 {
     Mail m{};
     m.seq = kExit;
     const unsigned long long t0 = wall_clock64();
+    bool voted = false;
     for (;;) {
-        if (poll_load(a.abort)) return m;
+        if (poll_load(a.abort) || poll_load(a.idle + 1)) return m;
         const uint32_t t = sys_load(a.tail);
         uint32_t h = poll_load(a.head);
         if ((int32_t)(t - h) > 0) {
+            if (voted) {
+                atomicSub(a.idle, 1u);
+                voted = false;
+            }
             if (atomicCAS(a.head, h, h + 1u) != h) continue;
             const sg::ServeEntry *en = a.ring + (h % sg::kServeRing);
             m.seq = h;
@@ -107,7 +119,17 @@ __device__ Mail claim_next(const ServeArgs &a, uint32_t xcc)
             return m;
         }
         if (sys_load(a.stop)) return m;
-        if (wall_clock64() - t0 > a.idle_ticks) return m;
+        if (!voted && wall_clock64() - t0 > a.idle_ticks) {
+            voted = true;
+            if (atomicAdd(a.idle, 1u) + 1u >= nlive) {
+                flag_store(a.idle + 1, 1u);
+                return m;
+            }
+        }
+        if (voted && poll_load(a.idle) >= nlive) {
+            flag_store(a.idle + 1, 1u);
+            return m;
+        }
         __builtin_amdgcn_s_sleep(2);
     }
 }
@@ -215,13 +237,18 @@ __global__ __launch_bounds__(BS) void k_serve(ServeArgs a)
     }
     const uint32_t xcc = xcc_id();
     BarState st = bar_begin(a.bar, xcc);
-    __shared__ uint32_t s_rank, s_cnt, s_ok;
+    __shared__ uint32_t s_rank, s_cnt, s_ok, s_live;
     __shared__ Mail s_mail;
     if (threadIdx.x == 0) {
-        const bool ok = bar_census(a.bar, xcc, st, a.abort);
+        uint32_t all[kMaxXcd];
+        const bool ok = bar_census(a.bar, xcc, st, a.abort, all);
+        // slots of this launch: XCDs holding blocks that have an arena
+        uint32_t live = 0;
+        for (int x = 0; x < kMaxXcd; x++) live += (ok && all[x] != 0u && a.slot_of[x] >= 0) ? 1u : 0u;
         s_rank = st.rank;
         s_cnt = st.cnt;
         s_ok = ok;
+        s_live = live;
     }
     __syncthreads();
     if (!s_ok) return;
@@ -238,7 +265,7 @@ __global__ __launch_bounds__(BS) void k_serve(ServeArgs a)
     DevStats *stats = (DevStats *)(base + a.L.stats);
     const bool leader = brank == 0 && threadIdx.x == 0;
     for (;;) {
-        if (leader) a.mail[xcc] = claim_next(a, xcc);
+        if (leader) a.mail[xcc] = claim_next(a, xcc, s_live);
         if (!grid_sync(xb, xcc, gen, a.abort)) break;
         if (threadIdx.x == 0) s_mail = a.mail[xcc];
         __syncthreads();
@@ -454,6 +481,7 @@ int serve_create(sg_graph *const *graphs, int ngraphs, const sg_engine_options &
     a.stop = d->stop;
     a.done = d->done;
     a.head = (uint32_t *)(d->dev + o_head);
+    a.idle = (uint32_t *)(d->dev + o_head + 128);  // its own cache line, before the mailboxes
     a.mail = (Mail *)(d->dev + o_mail);
     a.ctl = (SlotCtl *)(d->dev + o_ctl);
     a.abort = (uint32_t *)(d->dev + o_abort);
@@ -567,6 +595,9 @@ int serve_pump(ServeDev *d, bool *running)
     ServeArgs a = d->a;
     a.bar.rset = (uint32_t)(d->launches & 1u);
     a.probe = 0;
+    // the launch starts with no idle votes and not closing (in stream order)
+    int zr = serve_zero(d, d->a.idle, 8);
+    if (zr) return zr;
     int rc = engine_launch_persistent_on(d->stream, [&]() -> int {
         hipLaunchKernelGGL(k_serve<kServeBlock>, dim3(d->blocks), dim3(kServeBlock), 0, d->stream, a);
         SG_HIP(hipGetLastError());
