@@ -18,12 +18,17 @@
 //      per wave and level), so each later phase visits a level's vertices
 //      without scanning all n;
 //   3. sigma, level by level: every vertex gathers sigma(u) over in-neighbours
-//      one level up (pull over the CSC, wave prefix-sum scheduling); vertices
+//      one level up (pull over the CSC, wave prefix-sum scheduling), testing
+//      "u is one level up" in a bitmap of that level (n/8 bytes, L2-resident)
+//      rather than reading u's level from the n-word level array; three such
+//      bitmaps rotate, each set while its level is processed and cleared
+//      through its level list two levels later; vertices
 //      with in-degree above kPullHubDegree are summed per 256-edge chunk of
 //      the graph's static in-hub list and merged with an FP64 atomicAdd --
 //      path counts are integers below 2^53, so the order of the additions
 //      cannot change them;
-//   4. dependencies, level by level from the deepest: acc(u) = sum over
+//   4. dependencies, level by level from the deepest (the same rotating
+//      level bitmaps): acc(u) = sum over
 //      successors w one level down of psi(w) = 1/sigma(w) + acc(w), which is
 //      (1 + delta(w)) / sigma(w) with delta(w) = sigma(w) acc(w); sigma and
 //      acc sit side by side (one 16-byte load per successor); vertices with
@@ -55,6 +60,7 @@ struct Bc1Args {
     uint32_t *loff;     // [kBcMaxDepth + 2]: first position of each level in order
     uint32_t *lfill;    // [kBcMaxDepth + 1]: positions handed out per level
     uint32_t *ctl;      // [0]: deepest level reached, [1]: a level beyond kBcMaxDepth
+    uint64_t *lbm;      // 3 x words: bitmaps of three consecutive levels (level j in slot j % 3)
     // outputs (any may be null):
     float *dep;         // 0.5 delta(v) at dep[perm ? perm[v] : v]
     const uint32_t *perm;
@@ -88,15 +94,15 @@ __device__ __forceinline__ uint32_t wave_level_add(uint32_t *hist, uint32_t *glo
 }
 
 // Sum over the edges [b, e) of each lane's vertex (edge ids into `adj`) of
-// val(neighbour) for neighbours at level `want`, the wave walking the
+// val(neighbour) for neighbours whose bit is set in `inlevel`, the wave walking the
 // concatenated lists kLevelSteps x 64 edges per step with every load of a
 // stage issued for all sub-steps before any is used (one memory latency
 // covers them all); FP64.
 constexpr int kLevelSteps = 4;
 
 template <typename Val>
-__device__ __forceinline__ double wave_level_sum(const uint32_t *adj, const int32_t *lvl, int32_t want, uint32_t b,
-                                                 uint32_t e, Val val)
+__device__ __forceinline__ double wave_level_sum(const uint32_t *adj, const uint64_t *inlevel, uint32_t b, uint32_t e,
+                                                 Val val)
 {
     constexpr int U = kLevelSteps;
     const int ln = lane();
@@ -119,7 +125,7 @@ __device__ __forceinline__ double wave_level_sum(const uint32_t *adj, const int3
         }
         bool on[U];
 #pragma unroll
-        for (int s = 0; s < U; s++) on[s] = act[s] && lvl[u[s]] == want;
+        for (int s = 0; s < U; s++) on[s] = act[s] && test_bit(inlevel, u[s]);
         double x[U];
 #pragma unroll
         for (int s = 0; s < U; s++) x[s] = on[s] ? val(u[s]) : 0.0;
@@ -261,12 +267,21 @@ __device__ __forceinline__ bool bc1_run(const Bc1Args &a, uint32_t brank, uint32
                 a.sa[v] = make_double2(l == 0 ? 1.0 : 0.0, 0.0);
             }
         }
+        // level bitmaps: slot 0 holds level 0 (the source), slots 1 and 2 start clear
+        const uint32_t W = g.words, src = a.f.source;
+        for (uint32_t w = gtid; w < 3 * W; w += nthreads) a.lbm[w] = w == (src >> 6) ? 1ull << (src & 63) : 0ull;
         ok = grid_sync(a.f.bar, xcc, gen, a.f.abort);
     }
     const uint32_t D = ok ? s_D : 0u;
     // ---- sigma, level by level
+    const uint32_t W = g.words;
     for (uint32_t L = 1; ok && L <= D; L++) {
         const uint32_t b0 = a.loff[L], e0 = a.loff[L + 1];
+        const uint64_t *up = a.lbm + (size_t)((L - 1) % 3) * W;  // level L-1
+        uint64_t *cur = a.lbm + (size_t)(L % 3) * W;             // level L, set now
+        uint64_t *old = a.lbm + (size_t)((L + 1) % 3) * W;       // level L-2, cleared now
+        if (L >= 2)
+            for (uint32_t i = a.loff[L - 2] + gtid; i < a.loff[L - 1]; i += nthreads) old[a.order[i] >> 6] = 0ull;
         for (uint32_t i0 = b0 + gwave * kWave; i0 < e0; i0 += nwaves * kWave) {
             const uint32_t i = i0 + (uint32_t)ln;
             uint32_t v = 0, b = 0, e = 0;
@@ -276,8 +291,8 @@ __device__ __forceinline__ bool bc1_run(const Bc1Args &a, uint32_t brank, uint32
                 e = g.cstart[v + 1];
                 if (e - b > kPullHubDegree) e = b;  // in-hubs: the chunk pass below
             }
-            const double s = wave_level_sum(g.csrc, lvl, (int32_t)L - 1, b, e,
-                                            [&](uint32_t u) { return a.sa[u].x; });
+            wave_set_bits(cur, i < e0, v);
+            const double s = wave_level_sum(g.csrc, up, b, e, [&](uint32_t u) { return a.sa[u].x; });
             if (i < e0 && e > b) a.sa[v].x = s;
         }
         // in-hub chunks of this level, kPullChunk in-edges each
@@ -298,7 +313,7 @@ __device__ __forceinline__ bool bc1_run(const Bc1Args &a, uint32_t brank, uint32
                 double x = 0.0;
                 for (uint32_t e = beg + (uint32_t)ln; e < end; e += kWave) {
                     const uint32_t u = g.csrc[e];
-                    if (lvl[u] == (int32_t)L - 1) x += a.sa[u].x;
+                    if (test_bit(up, u)) x += a.sa[u].x;
                 }
                 x = wave_sum_f64(x);
                 if (ln == 0 && x != 0.0) atomicAdd(&a.sa[v].x, x);
@@ -310,6 +325,11 @@ __device__ __forceinline__ bool bc1_run(const Bc1Args &a, uint32_t brank, uint32
     // successors: acc stays 0); the source's level needs none
     for (uint32_t L = D; ok && L-- > 1;) {
         const uint32_t b0 = a.loff[L], e0 = a.loff[L + 1];
+        const uint64_t *down = a.lbm + (size_t)((L + 1) % 3) * W;  // level L+1
+        uint64_t *cur = a.lbm + (size_t)(L % 3) * W;               // level L, set now
+        uint64_t *old = a.lbm + (size_t)((L + 2) % 3) * W;         // level L+2, cleared now
+        if (L + 2 <= D)
+            for (uint32_t i = a.loff[L + 2] + gtid; i < a.loff[L + 3]; i += nthreads) old[a.order[i] >> 6] = 0ull;
         auto psi = [&](uint32_t w) {
             const double2 x = a.sa[w];
             return 1.0 / x.x + x.y;
@@ -323,7 +343,8 @@ __device__ __forceinline__ bool bc1_run(const Bc1Args &a, uint32_t brank, uint32
                 e = g.row[u + 1];
                 if (e - b > kHugeDegree) e = b;  // out-hubs: the chunk pass below
             }
-            const double s = wave_level_sum(g.col, lvl, (int32_t)L + 1, b, e, psi);
+            wave_set_bits(cur, i < e0, u);
+            const double s = wave_level_sum(g.col, down, b, e, psi);
             if (i < e0 && e > b) a.sa[u].y = s;
         }
         for (uint32_t c0 = 0; c0 < g.nhuge; c0 += nwaves * kWave) {
@@ -343,7 +364,7 @@ __device__ __forceinline__ bool bc1_run(const Bc1Args &a, uint32_t brank, uint32
                 double x = 0.0;
                 for (uint32_t e = beg + (uint32_t)ln; e < end; e += kWave) {
                     const uint32_t w = g.col[e];
-                    if (lvl[w] == (int32_t)L + 1) x += psi(w);
+                    if (test_bit(down, w)) x += psi(w);
                 }
                 x = wave_sum_f64(x);
                 if (ln == 0 && x != 0.0) atomicAdd(&a.sa[u].y, x);
@@ -389,6 +410,7 @@ inline Bc1Layout bc1_layout(uint64_t n)
     put(L.loff, (uint64_t)(kBcMaxDepth + 2) * 4);
     put(L.lfill, (uint64_t)(kBcMaxDepth + 2) * 4);
     put(L.ctl, 256);
+    put(L.lbm, 3 * ((n + 63) / 64) * 8);
     put(L.fstats, sizeof(DevStats));
     put(L.bcsum, n * 8);
     L.total = o;
@@ -426,6 +448,7 @@ inline int bc1_fill(const sg_engine *e, char *base, const Bc1Layout &L, uint32_t
     b.loff = (uint32_t *)(base + L.loff);
     b.lfill = (uint32_t *)(base + L.lfill);
     b.ctl = (uint32_t *)(base + L.ctl);
+    b.lbm = (uint64_t *)(base + L.lbm);
     return SG_OK;
 }
 

@@ -283,6 +283,18 @@ extern "C" int sg_graph_create_ex(uint32_t n, uint64_t m, const uint32_t *row_st
         std::vector<uint32_t> head(n);
         for (uint32_t v = 0; v < n; v++) head[v] = hcs[v + 1] > hcs[v] ? hsrc[hcs[v]] : 0xFFFFFFFFu;
         rc = up(head.data(), n, &d.chead);
+        const uint32_t *hcw = undirected ? weights : cw;
+        if (!rc && hcw) {
+            // reuse `head` for the smallest in-edge weight of each vertex
+            sg::parallel_for_dynamic(n, 65536, [&](uint64_t v0, uint64_t v1) {
+                for (uint64_t v = v0; v < v1; v++) {
+                    uint32_t mw = 0xFFFFFFFFu;
+                    for (uint32_t e = hcs[v]; e < hcs[v + 1]; e++) mw = std::min(mw, hcw[e]);
+                    head[v] = mw;
+                }
+            });
+            rc = up(head.data(), n, &d.cminw);
+        }
     }
     if (!rc && relabel) {
         const uint32_t *p = nullptr;

@@ -46,6 +46,11 @@ struct DevGraph {
     // has none): a pull round's first parent test streams this n-word array
     // instead of touching one csrc cache line per vertex.
     const uint32_t *chead;
+    // Smallest in-edge weight of every vertex (UINT32_MAX when it has no
+    // in-edges; null for an unweighted graph, whose weights are all 1): a
+    // vertex whose distance is at most the frontier's smallest distance plus
+    // this cannot be lowered by an SSSP pull round.
+    const uint32_t *cminw;
     // Static split of high-degree vertices: every vertex with out-degree >
     // kHugeDegree contributes ceil(deg / kHugeChunk) chunk descriptors
     // (vertex, first edge), so a push round can spread its edges over the grid.
@@ -229,7 +234,7 @@ int ms_launch(sg_engine *e, int kind, const uint32_t *dev_src, uint32_t nsrc, vo
 
 // Working arena of one single-source BC run (byte offsets).
 struct Bc1Layout {
-    uint64_t sa, lvl, order, lcnt, loff, lfill, ctl, fstats, bcsum, total;
+    uint64_t sa, lvl, order, lcnt, loff, lfill, ctl, lbm, fstats, bcsum, total;
 };
 
 // Single-source BC from device vertex `dev_source` over the whole grid

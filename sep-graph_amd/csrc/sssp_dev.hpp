@@ -99,6 +99,7 @@ struct Round {
     bool huge;
     uint32_t thr;   // entries with dist >= thr are deferred (priority mode)
     uint32_t minv;  // smallest distance this lane queued for the next round
+    uint32_t fmin;  // no distance in this round's frontier is below this
 };
 
 // The smallest distance queued for a round is kept complemented in the
@@ -118,6 +119,19 @@ __device__ __forceinline__ bool in_frontier(const Round &R, uint32_t u)
 }
 
 __device__ __forceinline__ uint32_t edge_w(const uint32_t *w, uint32_t e) { return w ? w[e] : 1u; }
+
+// True when no in-neighbour in this round's frontier can lower v: every one
+// has a distance of at least R.fmin, and v's in-edges weigh at least its
+// minimum in-weight, so a candidate is at least fmin + minw(v). A frontier
+// vertex lowered within this same round below fmin is in the next frontier,
+// from which v is reached then, so skipping v now only defers that.
+__device__ __forceinline__ bool settled(const Round &R, uint32_t v, uint32_t dv)
+{
+    if (dv == kInf) return false;
+    const DevGraph &g = R.a->g;
+    const uint32_t mw = g.cminw ? g.cminw[v] : (g.cw ? 0u : 1u);
+    return (uint64_t)dv <= (uint64_t)R.fmin + mw;
+}
 
 // Relax u->v with tentative distance nd for `act` lanes; wave-convergent.
 __device__ __forceinline__ void relax(Round &R, bool act, uint32_t v, uint32_t nd)
@@ -251,6 +265,7 @@ __device__ __forceinline__ void pull_word_serial(Round &R, uint32_t w)
     const bool improved = v < g.n && best < dv0;
     if (improved) {
         a.dist[v] = best;
+        R.minv = min(R.minv, best);
         const uint32_t dv = g.row[v + 1] - g.row[v];
         R.scout += dv;
         R.huge |= dv > kHugeDegree;
@@ -280,6 +295,7 @@ __device__ __forceinline__ void pull_word(Round &R, uint32_t w)
         e = g.cstart[v + 1];
         best = a.dist[v];
         if (e - b > kPullHubDegree) b = e;  // long in-edge lists go through the chunk list
+        if (settled(R, v, best)) b = e;
     }
     const uint32_t dv0 = best;
     for (;;) {
@@ -327,6 +343,7 @@ __device__ __forceinline__ void pull_word(Round &R, uint32_t w)
     const bool improved = v < g.n && best < dv0;
     if (improved) {
         a.dist[v] = best;
+        R.minv = min(R.minv, best);
         const uint32_t dv = g.row[v + 1] - g.row[v];
         R.scout += dv;
         R.huge |= dv > kHugeDegree;
@@ -351,7 +368,11 @@ __device__ __forceinline__ void pull_hub_chunks(Round &R, const uint2 (&dsc)[NC]
     constexpr int U = kPullChunk / kWave;
     uint32_t end[NC], u[NC][U], du[NC][U], wt[NC][U];
 #pragma unroll
-    for (int c = 0; c < NC; c++) end[c] = c < nc ? min(g.cstart[dsc[c].x + 1], dsc[c].y + kPullChunk) : 0u;
+    for (int c = 0; c < NC; c++) {
+        end[c] = c < nc ? min(g.cstart[dsc[c].x + 1], dsc[c].y + kPullChunk) : 0u;
+        // a hub no frontier in-neighbour can lower skips its chunks
+        if (c < nc && settled(R, dsc[c].x, a.dist[dsc[c].x])) end[c] = 0u;
+    }
 #pragma unroll
     for (int c = 0; c < NC; c++)
 #pragma unroll
@@ -381,6 +402,7 @@ __device__ __forceinline__ void pull_hub_chunks(Round &R, const uint2 (&dsc)[NC]
         if (ln == 0 && m < a.dist[v]) {
             const uint32_t old = atomicMin(&a.dist[v], m);
             if (m < old) {
+                R.minv = min(R.minv, m);
                 // several chunks may lower the hub: the one setting its bit counts it
                 const uint64_t bit = 1ull << (v & 63);
                 if (!(atomicOr((unsigned long long *)&R.fnext[v >> 6], (unsigned long long)bit) & bit)) {
@@ -493,6 +515,9 @@ __device__ __forceinline__ bool sssp_run(const SsspArgs &a, uint32_t brank, uint
         R.huge = false;
         R.thr = thr;
         R.minv = kInf;
+        // the smallest distance queued for this round (every path that adds
+        // a vertex to a frontier records it), 0 when none was recorded
+        R.fmin = s_cmin ? ~s_cmin : 0u;
         const uint32_t *qin = a.queue + (size_t)(L % 3) * a.qcap;
 
         if (!pull) {
@@ -592,8 +617,9 @@ __device__ __forceinline__ bool sssp_run(const SsspArgs &a, uint32_t brank, uint
         // counters first: their atomics are then in flight while the queue
         // flush waits for its position
         publish_counts<WPB>(R.S.nxt, R.scout, R.count, R.huge, s_red);
-        if (a.delta) {
+        {
             // block minimum of the queued distances, one atomic per block
+            // (the priority threshold and the settled test of pull rounds)
             uint32_t m = R.minv;
 #pragma unroll
             for (int d = kWave / 2; d > 0; d >>= 1) m = min(m, (uint32_t)__shfl_xor(m, d, kWave));
