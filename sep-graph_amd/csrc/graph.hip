@@ -73,7 +73,7 @@ __global__ void k_reach_stats(const uint32_t *value, const uint32_t *row, const 
 
 int dev_alloc(sg_graph *g, void **p, uint64_t bytes)
 {
-    if (g->nallocs >= 16) SG_FAIL(SG_ERR_STATE, "sg_graph_create: too many allocations");
+    if (g->nallocs >= 24) SG_FAIL(SG_ERR_STATE, "sg_graph_create: too many allocations");
     hipError_t e = hipMalloc(p, bytes ? bytes : 4);
     if (e != hipSuccess) {
         (void)hipGetLastError();
@@ -351,6 +351,51 @@ extern "C" int sg_graph_create_ex(uint32_t n, uint64_t m, const uint32_t *row_st
         if (!rc) rc = upload2(out_chunks, &d.huge, &d.nhuge);
         if (!rc) rc = upload2(in_chunks, &d.hugein, &d.nhugein);
         if (!rc) rc = upload2(in_hubs, &d.hubin, &d.nhubin);
+    }
+    if (!rc) {
+        // sliced in-edges (64 vertices per slice, in-hubs left to the chunks)
+        const uint32_t *hcs = undirected ? row_start : cs;
+        const uint32_t *hsrc = undirected ? edge_dst : src;
+        const uint32_t *hcw = undirected ? weights : cw;
+        const uint32_t W = d.words;
+        std::vector<uint32_t> off((size_t)W + 1, 0u);
+        sg::parallel_for_dynamic(W, 4096, [&](uint64_t w0, uint64_t w1) {
+            for (uint64_t w = w0; w < w1; w++) {
+                uint32_t len = 0;
+                for (uint64_t v = w * 64; v < std::min<uint64_t>(n, w * 64 + 64); v++) {
+                    const uint32_t deg = hcs[v + 1] - hcs[v];
+                    if (deg <= kPullHubDegree) len = std::max(len, deg);
+                }
+                off[w + 1] = len * 64;
+            }
+        });
+        uint64_t total = 0;
+        for (uint32_t w = 0; w < W; w++) {
+            total += off[w + 1];
+            off[w + 1] = (uint32_t)std::min<uint64_t>(total, 0xFFFFFFFFull);
+        }
+        // offsets are 32-bit: a larger padded layout is not built (the pull
+        // rounds then schedule in-edges per wave instead)
+        if (total >= 0xFFFFFFFFull) total = 0;
+        std::vector<uint32_t> ssrc(total ? total : 1, 0xFFFFFFFFu), sw;
+        if (hcw) sw.assign(total ? total : 1, 0u);
+        if (total) sg::parallel_for_dynamic(W, 4096, [&](uint64_t w0, uint64_t w1) {
+            for (uint64_t w = w0; w < w1; w++)
+                for (uint64_t v = w * 64; v < std::min<uint64_t>(n, w * 64 + 64); v++) {
+                    const uint32_t deg = hcs[v + 1] - hcs[v];
+                    if (deg > kPullHubDegree) continue;
+                    for (uint32_t j = 0; j < deg; j++) {
+                        const uint64_t at = off[w] + (uint64_t)j * 64 + (v - w * 64);
+                        ssrc[at] = hsrc[hcs[v] + j];
+                        if (hcw) sw[at] = hcw[hcs[v] + j];
+                    }
+                }
+        });
+        if (total) {
+            rc = up(off.data(), (uint64_t)W + 1, &d.sell_off);
+            if (!rc) rc = up(ssrc.data(), total, &d.sell_src);
+            if (!rc && hcw) rc = up(sw.data(), total, &d.sell_w);
+        }
     }
     if (rc) {
         sg_graph_destroy(g);

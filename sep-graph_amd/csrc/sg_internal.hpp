@@ -51,6 +51,17 @@ struct DevGraph {
     // vertex whose distance is at most the frontier's smallest distance plus
     // this cannot be lowered by an SSSP pull round.
     const uint32_t *cminw;
+    // In-edges in 64-vertex slices (sliced ELLPACK with the wave width as
+    // slice height): slice w holds the vertices of bitmap word w, its j-th
+    // row the j-th in-edge of each of them, 64 consecutive entries
+    // (0xFFFFFFFF past a vertex's in-degree), so a wave reads one in-edge per
+    // lane per coalesced load, with no scheduling. Vertices with in-degree
+    // above kPullHubDegree are left out (the in-hub chunk list holds them).
+    // sell_off[w] is slice w's first entry (words + 1 entries); sell_w holds
+    // the weights (null for an unweighted graph).
+    const uint32_t *sell_off;
+    const uint32_t *sell_src;
+    const uint32_t *sell_w;
     // Static split of high-degree vertices: every vertex with out-degree >
     // kHugeDegree contributes ceil(deg / kHugeChunk) chunk descriptors
     // (vertex, first edge), so a push round can spread its edges over the grid.
@@ -91,7 +102,7 @@ struct sg_graph {
     bool undirected = false;
     bool weighted = false;
     uint64_t device_bytes = 0;
-    void *allocs[16] = {};
+    void *allocs[24] = {};
     int nallocs = 0;
     int device = 0;
     int num_cus = 0;

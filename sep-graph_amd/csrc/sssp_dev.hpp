@@ -356,6 +356,63 @@ __device__ __forceinline__ void pull_word(Round &R, uint32_t w)
     }
 }
 
+// Pull step for the 64 vertices of word w through the sliced in-edges
+// (DevGraph::sell_*): row j of the slice gives every lane the j-th in-edge of
+// its own vertex in one coalesced load, so a lane keeps its vertex's running
+// minimum in a register, with no edge scheduling or cross-lane reduction.
+// kPullSteps rows are loaded before any is used. A slice whose vertices are
+// all settled, hubs or without in-edges is skipped whole.
+__device__ __forceinline__ void pull_slice(Round &R, uint32_t w)
+{
+    const SsspArgs &a = *R.a;
+    const DevGraph &g = a.g;
+    const int ln = lane();
+    const uint32_t v = w * 64 + ln;
+    uint32_t best = kInf;
+    bool want = false;
+    if (v < g.n) {
+        const uint32_t deg = g.cstart[v + 1] - g.cstart[v];
+        best = a.dist[v];
+        want = deg != 0 && deg <= kPullHubDegree && !settled(R, v, best);
+    }
+    const uint32_t dv0 = best;
+    if (ballot(want)) {
+        const uint32_t beg = g.sell_off[w];
+        const uint32_t rows = (g.sell_off[w + 1] - beg) / kWave;
+        const uint32_t *S = g.sell_src + beg + ln;
+        const uint32_t *Wt = g.sell_w ? g.sell_w + beg + ln : nullptr;
+        constexpr int U = kPullSteps;
+        for (uint32_t j0 = 0; j0 < rows; j0 += U) {
+            uint32_t u[U], du[U], wt[U];
+#pragma unroll
+            for (int s = 0; s < U; s++) u[s] = (want && j0 + s < rows) ? S[(size_t)(j0 + s) * kWave] : kInf;
+#pragma unroll
+            for (int s = 0; s < U; s++) {
+                const bool f = u[s] != kInf && in_frontier(R, u[s]);
+                du[s] = f ? a.dist[u[s]] : kInf;
+                wt[s] = f ? (Wt ? Wt[(size_t)(j0 + s) * kWave] : 1u) : 0u;
+            }
+#pragma unroll
+            for (int s = 0; s < U; s++)
+                if (du[s] != kInf) best = min(best, sat_add(du[s], wt[s]));
+        }
+    }
+    const bool improved = v < g.n && best < dv0;
+    if (improved) {
+        a.dist[v] = best;
+        R.minv = min(R.minv, best);
+        const uint32_t dv = g.row[v + 1] - g.row[v];
+        R.scout += dv;
+        R.huge |= dv > kHugeDegree;
+    }
+    const uint64_t fm = ballot(improved);
+    if (ln == 0 && fm) {
+        // hubs of this word may set bits concurrently from the chunk list
+        atomicOr((unsigned long long *)&R.fnext[w], (unsigned long long)fm);
+        R.count += __popcll(fm);
+    }
+}
+
 // Pull for up to NC chunks of hubs' in-edges at once (each load of a stage
 // issued for every chunk before any is used). A chunk's minimum candidate is
 // merged into its hub with atomicMin, as several waves share a hub.
@@ -611,6 +668,7 @@ __device__ __forceinline__ bool sssp_run(const SsspArgs &a, uint32_t brank, uint
             for (uint32_t w = gwave; w < W; w += nwaves) {
                 if (g.inert && g.inert[w] == ~0ull) continue;
                 if (a.lb_pull_none) pull_word_serial(R, w);
+                else if (g.sell_src) pull_slice(R, w);
                 else pull_word<kPullSteps>(R, w);
             }
         }
