@@ -11,23 +11,30 @@
 // The ranks computed here are the normalised PageRank with uniform teleport
 // and uniform redistribution of dangling mass,
 //     x = (1-d)/n + d (P^T x + (sum of x over dangling vertices)/n),
-// stored as FP32. Per-vertex sums, the dangling mass and the L1 residual are
-// accumulated in FP64. In the pull mode every reduction runs in a fixed order
-// for a given grid, so repeated runs give identical bits.
+// stored as FP32. Sums are compensated FP32 (hi + lo, Neumaier): in the pull
+// mode the per-vertex in-edge sums and the dangling mass, in the residual
+// mode the unnormalised ranks, the residuals (each atomic addition's
+// rounding error, recomputed by the adding lane, goes to a second FP32
+// array) and the normalising sum; only the L1 residual (pull) and the
+// residual mass absorbed at the end (DD) are reduced in FP64. In the pull
+// mode every reduction runs in a fixed order for a given grid, so repeated
+// runs give identical bits.
 //
 // Pull TD: one lane per vertex sums x[u]/outdeg(u) (kept as an FP32 array
-// `contrib`, ping-ponged between iterations) over its in-edges; the wave walks
-// the concatenated in-edge lists 64 edges per step and reduces each lane's
-// segment with a segmented shuffle scan. Vertices with in-degree above
-// kPullHubDegree are summed per 256-edge chunk into FP64 slots and finished
-// after a grid barrier, one wave per hub, in chunk order.
+// `contrib`, ping-ponged between iterations) over its in-edges, reading them
+// row by row from the graph's sliced in-edge layout (a lane's own entries,
+// in order; without that layout, the wave walks the concatenated in-edge
+// lists 64 edges per step with a segmented shuffle scan). Vertices with
+// in-degree above kPullHubDegree are summed per 256-edge chunk into
+// compensated slots and finished after a grid barrier, one wave per hub, in
+// chunk order.
 //
 // Residual DD: solves the unnormalised system y = (1-d)/n + d P^T y by
 // residual propagation (dangling mass leaks) and normalises y at the end; for
 // uniform teleport this is the same ranking vector, x = y / |y|_1. A round
 // activates every vertex whose residual reached theta = tol (1-d) / n (the
 // rounds' worklist holds vertices whose residual crossed theta), moves the
-// residual into y and spreads d r / outdeg along out-edges: with FP64 atomics
+// residual into y and spreads d r / outdeg along out-edges: with FP32 atomics
 // when the active out-degree is small (push), or by a gather over in-edges
 // from active sources when it exceeds m / alpha (pull).
 #include "frontier.hpp"
@@ -38,99 +45,145 @@ namespace {
 
 // ------------------------------------------------------------------ helpers
 
-// Deterministic block sum of two doubles; the result is valid in thread 0.
-// s_red holds one double2 per wave.
-template <int WPB>
-__device__ __forceinline__ double2 block_sum2(double a, double b, double2 *s_red)
+// ------------------------------------------- compensated FP32 accumulation
+// A sum kept as hi + lo, where lo collects the rounding error of every
+// addition into hi (Neumaier's variant of Kahan summation): FP32 storage and
+// arithmetic with an error close to one rounding of the final sum.
+struct CSum {
+    float hi, lo;
+};
+
+__device__ __forceinline__ void csum_add(CSum &a, float x)
 {
-    a = wave_sum_f64(a);
-    b = wave_sum_f64(b);
+    const float s = a.hi + x;
+    const float t = fabsf(a.hi) >= fabsf(x) ? (a.hi - s) + x : (x - s) + a.hi;
+    a.hi = s;
+    a.lo += t;
+}
+
+// a then b (the order fixes the rounding)
+__device__ __forceinline__ CSum csum_merge(CSum a, const CSum &b)
+{
+    csum_add(a, b.hi);
+    a.lo += b.lo;
+    return a;
+}
+
+__device__ __forceinline__ float csum_value(const CSum &a) { return a.hi + a.lo; }
+
+// Wave sum; partners merge lower lane first, so every lane ends with the
+// same bits.
+__device__ __forceinline__ CSum wave_csum(CSum a)
+{
+    const int ln = lane();
+#pragma unroll
+    for (int d = 1; d < kWave; d <<= 1) {
+        const CSum o{__shfl_xor(a.hi, d, kWave), __shfl_xor(a.lo, d, kWave)};
+        a = (ln & d) ? csum_merge(o, a) : csum_merge(a, o);
+    }
+    return a;
+}
+
+// Segmented inclusive compensated sum (owners non-decreasing along the wave).
+__device__ __forceinline__ CSum seg_csum(CSum s, int k)
+{
+    const int ln = lane();
+#pragma unroll
+    for (int d = 1; d < kWave; d <<= 1) {
+        const CSum y{__shfl_up(s.hi, d, kWave), __shfl_up(s.lo, d, kWave)};
+        const int ko = __shfl_up(k, d, kWave);
+        if (ln >= d && ko == k) s = csum_merge(y, s);
+    }
+    return s;
+}
+
+// Per-block partials of the pull iteration: the L1 residual in FP64 and the
+// dangling mass as a compensated FP32 sum (16 bytes, the size of a double2).
+struct PrPart {
+    double res;
+    float dhi, dlo;
+};
+static_assert(sizeof(PrPart) == sizeof(double2), "PrPart shares the partial-sum area with double2 partials");
+static_assert(sizeof(CSum) == 8, "hub chunk sums are 8 bytes");
+
+// Block sums of (residual, dangling) in a fixed order; valid in thread 0.
+template <int WPB>
+__device__ __forceinline__ PrPart block_part(double res, CSum dang, PrPart *s_red)
+{
+    res = wave_sum_f64(res);
+    dang = wave_csum(dang);
     const int wid = threadIdx.x / kWave;
-    if (lane() == 0) s_red[wid] = make_double2(a, b);
+    if (lane() == 0) s_red[wid] = PrPart{res, dang.hi, dang.lo};
     __syncthreads();
-    double2 t = make_double2(0.0, 0.0);
+    PrPart t{0.0, 0.f, 0.f};
     if (threadIdx.x == 0) {
+        CSum c{0.f, 0.f};
         for (int i = 0; i < WPB; i++) {
-            t.x += s_red[i].x;
-            t.y += s_red[i].y;
+            t.res += s_red[i].res;
+            c = csum_merge(c, CSum{s_red[i].dhi, s_red[i].dlo});
         }
+        t.dhi = c.hi;
+        t.dlo = c.lo;
     }
     __syncthreads();
     return t;
 }
 
-// Sum of the per-block partials part[0 .. nblocks) in a fixed order; every
-// block computes the same value. Result returned to all threads.
+// Sum of the per-block partials in a fixed order, returned to every thread.
 template <int BS>
-__device__ __forceinline__ double2 grid_sum2(const double2 *part, uint32_t nblocks, double2 *s_red, double2 *s_out)
+__device__ __forceinline__ PrPart grid_part(const PrPart *part, uint32_t nblocks, PrPart *s_red, PrPart *s_out)
 {
-    double a = 0.0, b = 0.0;
+    double res = 0.0;
+    CSum dang{0.f, 0.f};
     for (uint32_t i = threadIdx.x; i < nblocks; i += BS) {
-        const double2 q = part[i];
-        a += q.x;
-        b += q.y;
+        const PrPart q = part[i];
+        res += q.res;
+        dang = csum_merge(dang, CSum{q.dhi, q.dlo});
     }
-    const double2 t = block_sum2<BS / kWave>(a, b, s_red);
+    const PrPart t = block_part<BS / kWave>(res, dang, s_red);
     if (threadIdx.x == 0) *s_out = t;
     __syncthreads();
-    const double2 r = *s_out;
+    const PrPart r = *s_out;
     __syncthreads();
     return r;
 }
 
-// Segmented inclusive sum across lanes whose segment owner k is the same
-// (owners are non-decreasing along the wave).
-__device__ __forceinline__ double seg_sum(double s, int k)
-{
-    const int ln = lane();
-#pragma unroll
-    for (int d = 1; d < kWave; d <<= 1) {
-        const double y = __shfl_up(s, d, kWave);
-        const int ko = __shfl_up(k, d, kWave);
-        if (ln >= d && ko == k) s += y;
-    }
-    return s;
-}
-
-// Sub-steps of 64 edges per iteration of a gather walk: all of an
-// iteration's loads are issued before its reductions, so one memory latency
-// covers kGatherSteps x 64 edges.
-constexpr int kGatherSteps = 4;
-
-// Sum over the in-edges [b, e) of each lane's vertex of val(edge) (FP64),
-// the wave walking the concatenated lists kGatherSteps x 64 edges per
-// iteration. Each 64-edge sub-step reduces its segments in a fixed order, so
-// the result is deterministic.
+// Sum over the in-edges [b, e) of each lane's vertex of val(edge) as a
+// compensated FP32 sum, the wave walking the concatenated lists 4 x 64
+// edges per iteration (all of an iteration's loads issued before its
+// reductions); each 64-edge sub-step reduces its segments in a fixed order,
+// so the result is deterministic.
 template <typename F>
-__device__ __forceinline__ double wave_gather_sum(uint32_t b, uint32_t e, F val)
+__device__ __forceinline__ CSum wave_gather_csum(uint32_t b, uint32_t e, F val)
 {
+    constexpr int Q = 4;
     const int ln = lane();
-    double acc = 0.0;
+    CSum acc{0.f, 0.f};
     for (;;) {
         const uint32_t r = e - b;
         const uint32_t incl = wave_incl_scan(r);
         const uint32_t total = shfl(incl, kWave - 1);
         if (total == 0) break;
         const uint32_t excl = incl - r;
-        int k[kGatherSteps];
-        double cand[kGatherSteps];
+        int k[Q];
+        float cand[Q];
 #pragma unroll
-        for (int q = 0; q < kGatherSteps; q++) {
+        for (int q = 0; q < Q; q++) {
             const uint32_t j = (uint32_t)(q * kWave + ln);
             k[q] = owner_lane(incl, j < total ? j : total - 1);
             const uint32_t bk = shfl(b, k[q]), xk = shfl(excl, k[q]);
-            cand[q] = j < total ? val(bk + (j - xk)) : 0.0;
+            cand[q] = j < total ? val(bk + (j - xk)) : 0.f;
         }
-        const uint32_t span = min(total, (uint32_t)(kGatherSteps * kWave));
+        const uint32_t span = min(total, (uint32_t)(Q * kWave));
 #pragma unroll
-        for (int q = 0; q < kGatherSteps; q++) {
+        for (int q = 0; q < Q; q++) {
             const uint32_t s0 = (uint32_t)(q * kWave);
             if (s0 >= span) break;  // wave-uniform
-            const double s = seg_sum(cand[q], k[q]);
-            // this lane's segment within sub-step q: positions [lo, hi)
+            const CSum s = seg_csum(CSum{cand[q], 0.f}, k[q]);
             const uint32_t lo = min(max(excl, s0), s0 + kWave), hi = min(max(incl, s0), min(span, s0 + kWave));
-            const double seg = __shfl(s, (int)(hi > s0 ? hi - 1 - s0 : 0), kWave);
-            if (hi > lo) acc += seg;
+            const int at = (int)(hi > s0 ? hi - 1 - s0 : 0);
+            const CSum seg{__shfl(s.hi, at, kWave), __shfl(s.lo, at, kWave)};
+            if (hi > lo) acc = csum_merge(acc, seg);
         }
         b += min(incl, span) - min(excl, span);
     }
@@ -143,8 +196,8 @@ struct PrArgs {
     DevGraph g;
     float *x;          // ranks [n]
     float *contrib;    // 2 x n: x[u] / outdeg(u) of the current / next iteration
-    double *hubpart;   // [nhugein] chunk sums
-    double2 *part;     // 2 x kMaxBlocks (residual, dangling) partials
+    CSum *hubpart;     // [nhugein] chunk sums
+    PrPart *part;      // 2 x kMaxBlocks (residual, dangling) partials
     GridBarrier bar;
     uint32_t *abort;
     DevStats *stats;
@@ -172,11 +225,11 @@ __global__ __launch_bounds__(BS) void k_pr_pull(PrArgs a)
     const uint32_t xcc = xcc_id();
     BarState gen = bar_begin(a.bar, xcc);
     const long long t_start = wall_clock64();
-    const double d = a.damping;
-    const double inv_n = 1.0 / (double)n;
+    const float df = (float)a.damping;
+    const float inv_nf = (float)(1.0 / (double)n);
 
-    __shared__ double2 s_red[WPB];
-    __shared__ double2 s_glob;
+    __shared__ PrPart s_red[WPB];
+    __shared__ PrPart s_glob;
     // The lowest ids' contributions (in a degree-ordered graph the sources of
     // most in-edges) are copied to LDS each iteration: they are read-only
     // until the next barrier, and the gather reads them there instead of
@@ -188,15 +241,15 @@ __global__ __launch_bounds__(BS) void k_pr_pull(PrArgs a)
     __shared__ float s_cc[kCcLds];
 
     // x = 1/n; contributions of iteration 0; dangling mass
-    double dang = 0.0;
+    CSum dang{0.f, 0.f};
     for (uint32_t v = gtid; v < n; v += nthreads) {
-        a.x[v] = (float)inv_n;
+        a.x[v] = inv_nf;
         const uint32_t dout = g.row[v + 1] - g.row[v];
-        a.contrib[v] = dout ? (float)(inv_n / dout) : 0.0f;
-        if (!dout) dang += inv_n;
+        a.contrib[v] = dout ? inv_nf / (float)dout : 0.0f;
+        if (!dout) csum_add(dang, inv_nf);
     }
     {
-        const double2 t = block_sum2<WPB>(0.0, dang, s_red);
+        const PrPart t = block_part<WPB>(0.0, dang, s_red);
         if (threadIdx.x == 0) a.part[blockIdx.x] = t;
     }
     bool ok = grid_sync(a.bar, xcc, gen, a.abort);
@@ -204,10 +257,10 @@ __global__ __launch_bounds__(BS) void k_pr_pull(PrArgs a)
     uint32_t it = 0;
     double res = INFINITY;
     while (ok) {
-        const double2 gs = grid_sum2<BS>(a.part + (size_t)(it & 1) * kMaxBlocks, gridDim.x, s_red, &s_glob);
-        if (it > 0) res = gs.x;
+        const PrPart gs = grid_part<BS>(a.part + (size_t)(it & 1) * kMaxBlocks, gridDim.x, s_red, &s_glob);
+        if (it > 0) res = gs.res;
         if ((it > 0 && res < a.tol) || it >= a.max_iter) break;
-        const double base = (1.0 - d) * inv_n + d * gs.y * inv_n;
+        const float base = (1.0f - df) * inv_nf + df * csum_value(CSum{gs.dhi, gs.dlo}) * inv_nf;
         const float *cc = a.contrib + (size_t)(it & 1) * n;
         float *cn = a.contrib + (size_t)((it + 1) & 1) * n;
         if (gtid == 0 && it < kTraceRows) {
@@ -223,29 +276,34 @@ __global__ __launch_bounds__(BS) void k_pr_pull(PrArgs a)
         const uint32_t ncc = min(n, kCcLds);
         for (uint32_t i = threadIdx.x; i < ncc; i += BS) s_cc[i] = cc[i];
         __syncthreads();
-        auto contrib = [&](uint32_t u) -> double { return (double)(u < ncc ? s_cc[u] : cc[u]); };
-        double r_acc = 0.0, d_acc = 0.0;
-        auto finish = [&](uint32_t v, double s) {
-            const double xn = base + d * s;
-            r_acc += fabs(xn - (double)a.x[v]);
-            a.x[v] = (float)xn;
+        auto contrib = [&](uint32_t u) -> float { return u < ncc ? s_cc[u] : cc[u]; };
+        double r_acc = 0.0;
+        CSum d_acc{0.f, 0.f};
+        // x(v) from its in-edge sum: FP32 ranks and contributions, the
+        // residual |x' - x| summed in FP64
+        auto finish = [&](uint32_t v, const CSum &s) {
+            const float xn = base + df * csum_value(s);
+            r_acc += fabs((double)xn - (double)a.x[v]);
+            a.x[v] = xn;
             const uint32_t dout = g.row[v + 1] - g.row[v];
-            if (dout) cn[v] = (float)(xn / dout);
-            else d_acc += xn;
+            if (dout) cn[v] = xn / (float)dout;
+            else csum_add(d_acc, xn);
         };
         // (A1) chunk sums of the in-hubs
         for (uint32_t c = gwave; c < g.nhugein; c += nwaves) {
             const uint2 dsc = g.hugein[c];
             const uint32_t end = min(g.cstart[dsc.x + 1], dsc.y + kPullChunk);
-            double s = 0.0;
+            CSum s{0.f, 0.f};
             for (uint32_t e0 = dsc.y; e0 < end; e0 += kWave) {
                 const uint32_t e = e0 + ln;
-                if (e < end) s += contrib(g.csrc[e]);
+                if (e < end) csum_add(s, contrib(g.csrc[e]));
             }
-            s = wave_sum_f64(s);
+            s = wave_csum(s);
             if (ln == 0) a.hubpart[c] = s;
         }
-        // (A2) every other vertex, a 64-vertex word per wave at a time
+        // (A2) every other vertex, a 64-vertex word per wave at a time: from
+        // the sliced in-edges (each lane sums its own vertex's row entries in
+        // order) or, without them, by a scheduled walk of the word's lists
         for (uint32_t w = gwave; w < W; w += nwaves) {
             const uint32_t v = w * 64 + ln;
             uint32_t b = 0, e = 0;
@@ -256,7 +314,26 @@ __global__ __launch_bounds__(BS) void k_pr_pull(PrArgs a)
                 own = e - b <= kPullHubDegree;
                 if (!own) b = e;
             }
-            const double s = wave_gather_sum(b, e, [&](uint32_t ed) { return contrib(g.csrc[ed]); });
+            CSum s{0.f, 0.f};
+            if (g.sell_src) {
+                const uint32_t beg = g.sell_off[w];
+                const uint32_t rows = (g.sell_off[w + 1] - beg) / kWave;
+                const uint32_t *S = g.sell_src + beg + ln;
+                constexpr int U = 4;
+                for (uint32_t j0 = 0; j0 < rows; j0 += U) {
+                    uint32_t u[U];
+#pragma unroll
+                    for (int q = 0; q < U; q++) u[q] = (own && j0 + q < rows) ? S[(size_t)(j0 + q) * kWave] : 0xFFFFFFFFu;
+                    float c[U];
+#pragma unroll
+                    for (int q = 0; q < U; q++) c[q] = u[q] != 0xFFFFFFFFu ? contrib(u[q]) : 0.f;
+#pragma unroll
+                    for (int q = 0; q < U; q++)
+                        if (u[q] != 0xFFFFFFFFu) csum_add(s, c[q]);
+                }
+            } else {
+                s = wave_gather_csum(b, e, [&](uint32_t ed) { return contrib(g.csrc[ed]); });
+            }
             if (own) finish(v, s);
         }
         ok = grid_sync(a.bar, xcc, gen, a.abort);
@@ -264,13 +341,13 @@ __global__ __launch_bounds__(BS) void k_pr_pull(PrArgs a)
         // (B) in-hubs: chunk sums in chunk order, one wave per hub
         for (uint32_t h = gwave; h < g.nhubin; h += nwaves) {
             const uint2 hb = g.hubin[h];
-            double s = 0.0;
-            for (uint32_t i = ln; i < hb.y; i += kWave) s += a.hubpart[hb.x + i];
-            s = wave_sum_f64(s);
+            CSum s{0.f, 0.f};
+            for (uint32_t i = ln; i < hb.y; i += kWave) s = csum_merge(s, a.hubpart[hb.x + i]);
+            s = wave_csum(s);
             if (ln == 0) finish(g.hugein[hb.x].x, s);
         }
         {
-            const double2 t = block_sum2<WPB>(r_acc, d_acc, s_red);
+            const PrPart t = block_part<WPB>(r_acc, d_acc, s_red);
             if (threadIdx.x == 0) a.part[(size_t)((it + 1) & 1) * kMaxBlocks + blockIdx.x] = t;
         }
         it++;
@@ -300,14 +377,15 @@ __global__ __launch_bounds__(BS) void k_pr_pull(PrArgs a)
 struct PrDdArgs {
     DevGraph g;
     float *x;          // normalised ranks [n] (output)
-    double *y;         // unnormalised ranks [n]
-    double *r;         // residuals [n]
-    double *p;         // d r / outdeg of this round's active vertices [n]
+    CSum *y;           // unnormalised ranks [n], compensated FP32
+    float *r;          // residuals [n]: r + rlo, the rounding error of every
+    float *rlo;        // atomic addition into r collected in rlo
+    float *p;          // d r / outdeg of this round's active vertices [n]
     uint64_t *abit;    // 2 x W: active vertices of rounds k (k % 2)
     uint32_t *queue;   // 2 x qcap: vertices whose residual crossed theta
     uint32_t qcap;
     FrontierCtr *ctr;  // [0,1]: activation totals; [2,3,4]: queue appends
-    double2 *part;     // kMaxBlocks partials for the final sums
+    PrPart *part;      // kMaxBlocks partials for the final sums
     GridBarrier bar;
     uint32_t *abort;
     DevStats *stats;
@@ -348,7 +426,7 @@ __device__ double sampled_threshold(const PrDdArgs &a, uint32_t k, double *s_smp
 {
     const uint32_t n = a.g.n, size = min(kPrioSample, n);
     for (uint32_t i = threadIdx.x; i < kPrioSort; i += BS)
-        s_smp[i] = i < size ? a.r[(uint32_t)(sample_hash(((uint64_t)k << 32) | i) % n)] : __builtin_inf();
+        s_smp[i] = i < size ? (double)a.r[(uint32_t)(sample_hash(((uint64_t)k << 32) | i) % n)] : __builtin_inf();
     __syncthreads();
     for (uint32_t len = 2; len <= kPrioSort; len <<= 1) {
         for (uint32_t j = len >> 1; j > 0; j >>= 1) {
@@ -416,20 +494,30 @@ struct DdRound {
     bool qdeg;
 };
 
-// Add `add` to r[v] (atomically when `atomic`), queueing v if this addition
-// took its residual across theta. Wave-convergent.
-__device__ __forceinline__ void dd_accumulate(DdRound &R, bool act, uint32_t v, double add, bool atomic)
+// Add `add` to v's residual (atomically when `atomic`), queueing v if this
+// addition took its residual across theta. An atomic addition returns the
+// old value, so the lane recomputes the rounded sum the atomic stored and
+// adds its rounding error (TwoSum, Knuth) to rlo[v]. Wave-convergent.
+__device__ __forceinline__ void dd_accumulate(DdRound &R, bool act, uint32_t v, CSum add, bool atomic)
 {
     bool crossed = false;
-    if (act && add > 0.0) {
-        double old;
+    const float ad = csum_value(add);
+    if (act && ad > 0.f) {
+        float old;
         if (atomic) {
-            old = atomicAdd(&R.a->r[v], add);
+            old = atomicAdd(&R.a->r[v], add.hi);
+            const float s = old + add.hi;
+            const float bb = s - old;
+            const float err = (old - (s - bb)) + (add.hi - bb) + add.lo;
+            if (err != 0.f) atomicAdd(&R.a->rlo[v], err);
         } else {
-            old = R.a->r[v];
-            R.a->r[v] = old + add;
+            CSum c{R.a->r[v], R.a->rlo[v]};
+            old = c.hi;
+            c = csum_merge(c, add);
+            R.a->r[v] = c.hi;
+            R.a->rlo[v] = c.lo;
         }
-        crossed = old < R.theta && old + add >= R.theta;
+        crossed = (double)old < R.theta && (double)old + (double)ad >= R.theta;
         R.count += crossed;
         if (crossed && R.qdeg) R.qscout += R.a->g.row[v + 1] - R.a->g.row[v];
     }
@@ -445,7 +533,7 @@ __device__ __forceinline__ void dd_push_word(DdRound &R, uint64_t word, uint32_t
     const int ln = lane();
     const uint32_t u = w * 64 + ln;
     uint32_t b = 0, dg = 0;
-    double pu = 0.0;
+    float pu = 0.f;
     if ((word >> ln) & 1) {
         b = g.row[u];
         dg = g.row[u + 1] - b;
@@ -460,9 +548,9 @@ __device__ __forceinline__ void dd_push_word(DdRound &R, uint64_t word, uint32_t
         const bool act = j < total;
         const int k = owner_lane(incl, j);
         const uint32_t e = shfl(off, k) + j;
-        const double pk = __shfl(pu, k, kWave);
+        const float pk = __shfl(pu, k, kWave);
         const uint32_t v = act ? g.col[e] : 0u;
-        dd_accumulate(R, act, v, pk, true);
+        dd_accumulate(R, act, v, CSum{pk, 0.f}, true);
     }
 }
 
@@ -497,17 +585,19 @@ __global__ __launch_bounds__(BS) void k_pr_dd(PrDdArgs a)
 #endif
     constexpr uint32_t kAbLds = SG_PRDD_LDS * BS;
     __shared__ uint64_t s_ab[kAbLds];
-    __shared__ double2 s_red2[WPB];
-    __shared__ double2 s_glob;
+    __shared__ PrPart s_red2[WPB];
+    __shared__ PrPart s_glob;
     __shared__ unsigned long long s_red[2][WPB];
     __shared__ unsigned long long s_lvl[5];
     __shared__ uint32_t s_q[kLdsQ];
     __shared__ uint32_t s_qn, s_flush[2];
 
-    const double r0 = (1.0 - d) / (double)n;
+    const float r0 = (float)((1.0 - d) / (double)n);
+    const float df = (float)d;
     for (uint32_t v = gtid; v < n; v += nthreads) {
-        a.y[v] = 0.0;
+        a.y[v] = CSum{0.f, 0.f};
         a.r[v] = r0;
+        a.rlo[v] = 0.f;
     }
     for (uint32_t w = gtid; w < 2 * W; w += nthreads) a.abit[w] = 0;
     if (blockIdx.x == 0) {
@@ -550,10 +640,12 @@ __global__ __launch_bounds__(BS) void k_pr_dd(PrDdArgs a)
         }
         uint64_t *ab = a.abit + (size_t)(k & 1) * W;
         unsigned long long scout = 0, cnt = 0;
-        auto activate = [&](uint32_t u, double ru) {
-            a.y[u] += ru;
+        auto activate = [&](uint32_t u, float ru) {
+            CSum yu = a.y[u];
+            csum_add(yu, ru);
+            a.y[u] = yu;
             const uint32_t dout = g.row[u + 1] - g.row[u];
-            if (dout) a.p[u] = d * ru / dout;
+            if (dout) a.p[u] = df * ru / (float)dout;
             scout += dout;
             cnt++;
         };
@@ -567,11 +659,12 @@ __global__ __launch_bounds__(BS) void k_pr_dd(PrDdArgs a)
                 const uint32_t v = w * 64 + ln;
                 bool act = false;
                 if (v < n) {
-                    const double rv = a.r[v];
-                    act = rv >= thr;
-                    waiting |= !act && rv >= theta;
+                    const float rv = a.r[v] + a.rlo[v];
+                    act = (double)rv >= thr;
+                    waiting |= !act && (double)rv >= theta;
                     if (act) {
-                        a.r[v] = 0.0;
+                        a.r[v] = 0.f;
+                        a.rlo[v] = 0.f;
                         activate(v, rv);
                     }
                 }
@@ -583,12 +676,17 @@ __global__ __launch_bounds__(BS) void k_pr_dd(PrDdArgs a)
             for (uint32_t i = gtid; i < qc.qlen; i += nthreads) {
                 const uint32_t u = qin[i];
                 // claim the residual: a vertex queued twice is activated once
-                const double ru = atomicExch(&a.r[u], 0.0);
-                if (ru >= theta) {
+                // (no propagation runs in this phase; a claim split between
+                // two copies returns the part it took)
+                const float hi = atomicExch(&a.r[u], 0.f);
+                const float lo = atomicExch(&a.rlo[u], 0.f);
+                const float ru = hi + lo;
+                if ((double)ru >= theta) {
                     atomicOr((unsigned long long *)&ab[u >> 6], 1ull << (u & 63));
                     activate(u, ru);
-                } else if (ru != 0.0) {
-                    atomicAdd(&a.r[u], ru);
+                } else if (hi != 0.f || lo != 0.f) {
+                    atomicAdd(&a.r[u], hi);
+                    atomicAdd(&a.rlo[u], lo);
                 }
             }
         }
@@ -660,11 +758,11 @@ __global__ __launch_bounds__(BS) void k_pr_dd(PrDdArgs a)
                     mm &= mm - 1;
                     const uint32_t u = shfl(dsc.x, t), beg = shfl(dsc.y, t);
                     const uint32_t end = min(g.row[u + 1], beg + kHugeChunk);
-                    const double pu = a.p[u];
+                    const float pu = a.p[u];
                     for (uint32_t e0 = beg; e0 < end; e0 += kWave) {
                         const uint32_t e = e0 + ln;
                         const bool act = e < end;
-                        dd_accumulate(R, act, act ? g.col[e] : 0u, pu, true);
+                        dd_accumulate(R, act, act ? g.col[e] : 0u, CSum{pu, 0.f}, true);
                     }
                 }
             }
@@ -683,15 +781,15 @@ __global__ __launch_bounds__(BS) void k_pr_dd(PrDdArgs a)
             for (uint32_t c = gwave; c < g.nhugein; c += nwaves) {
                 const uint2 dsc = g.hugein[c];
                 const uint32_t end = min(g.cstart[dsc.x + 1], dsc.y + kPullChunk);
-                double s = 0.0;
+                CSum s{0.f, 0.f};
                 for (uint32_t e0 = dsc.y; e0 < end; e0 += kWave) {
                     const uint32_t e = e0 + ln;
                     if (e < end) {
                         const uint32_t u = g.csrc[e];
-                        if (active(u)) s += a.p[u];
+                        if (active(u)) csum_add(s, a.p[u]);
                     }
                 }
-                s = wave_sum_f64(s);
+                s = wave_csum(s);
                 dd_accumulate(R, ln == 0, dsc.x, s, true);
             }
             for (uint32_t w = gwave; w < W; w += nwaves) {
@@ -704,9 +802,9 @@ __global__ __launch_bounds__(BS) void k_pr_dd(PrDdArgs a)
                     own = e - b <= kPullHubDegree;
                     if (!own) b = e;
                 }
-                const double s = wave_gather_sum(b, e, [&](uint32_t ed) {
+                const CSum s = wave_gather_csum(b, e, [&](uint32_t ed) {
                     const uint32_t u = g.csrc[ed];
-                    return active(u) ? a.p[u] : 0.0;
+                    return active(u) ? a.p[u] : 0.f;
                 });
                 dd_accumulate(R, own, v, s, false);
             }
@@ -718,24 +816,30 @@ __global__ __launch_bounds__(BS) void k_pr_dd(PrDdArgs a)
     }
 
     // ---- absorb what residual is left, then normalise
-    double ysum = 0.0, rsum = 0.0;
+    // the sum of y (compensated FP32) normalises; the residual mass absorbed
+    // here is reduced in FP64
+    CSum ysum{0.f, 0.f};
+    double rsum = 0.0;
     if (ok) {
         for (uint32_t v = gtid; v < n; v += nthreads) {
-            const double rv = a.r[v];
-            const double yv = a.y[v] + rv;
+            CSum yv = a.y[v];
+            const float rv = a.r[v], rl = a.rlo[v];
+            csum_add(yv, rv);
+            csum_add(yv, rl);
             a.y[v] = yv;
-            ysum += yv;
-            rsum += rv;
+            csum_add(ysum, csum_value(yv));
+            rsum += (double)rv + (double)rl;
         }
-        const double2 t = block_sum2<WPB>(ysum, rsum, s_red2);
+        const PrPart t = block_part<WPB>(rsum, ysum, s_red2);
         if (threadIdx.x == 0) a.part[blockIdx.x] = t;
         ok = grid_sync(a.bar, xcc, gen, a.abort);
     }
-    double2 tot = make_double2(1.0, 0.0);
+    PrPart tot{0.0, 1.f, 0.f};
     if (ok) {
-        tot = grid_sum2<BS>(a.part, gridDim.x, s_red2, &s_glob);
-        const double inv = tot.x > 0.0 ? 1.0 / tot.x : 0.0;
-        for (uint32_t v = gtid; v < n; v += nthreads) a.x[v] = (float)(a.y[v] * inv);
+        tot = grid_part<BS>(a.part, gridDim.x, s_red2, &s_glob);
+        const float ys = csum_value(CSum{tot.dhi, tot.dlo});
+        const float inv = ys > 0.f ? 1.0f / ys : 0.f;
+        for (uint32_t v = gtid; v < n; v += nthreads) a.x[v] = csum_value(a.y[v]) * inv;
     }
     if (gtid == 0) {
         DevStats s{};
@@ -744,7 +848,7 @@ __global__ __launch_bounds__(BS) void k_pr_dd(PrDdArgs a)
         s.pull_rounds = pulls;
         s.trace = ptrace;
         s.iterations = k;
-        s.residual = tot.y;  // residual mass absorbed at the end (unnormalised)
+        s.residual = tot.res;  // residual mass absorbed at the end (unnormalised)
         s.status = ok ? SG_OK : SG_ERR_TIMEOUT;
         *a.stats = s;
         if (k < kTraceRows) {
@@ -834,8 +938,8 @@ extern "C" int sg_pagerank(sg_engine *e, double damping, double tol, int32_t max
         a.g = d;
         a.x = (float *)res;
         a.contrib = (float *)base;
-        a.hubpart = (double *)(base + off_hub);
-        a.part = (double2 *)(base + off_part);
+        a.hubpart = (CSum *)(base + off_hub);
+        a.part = (PrPart *)(base + off_part);
         a.bar.spin_limit = 1u << 22;
         a.abort = e->abort_flag;
         a.stats = e->stats;
@@ -851,17 +955,18 @@ extern "C" int sg_pagerank(sg_engine *e, double damping, double tol, int32_t max
     } else {
         const int force = mode == SG_PR_ITERATIVE ? -1 : forced_direction(o);
         if (force < -1) SG_FAIL(SG_ERR_UNSUPPORTED, "sg_pagerank: data-driven mode takes push/pull DD variants only");
-        // y | r | p (n doubles each) | partials
-        const uint64_t off_part = 3 * n * 8;
+        // y (n compensated floats) | r | rlo | p (n floats each) | partials
+        const uint64_t off_part = (n * 8 + 3 * n * 4 + 15) / 16 * 16;
         if ((rc = engine_ensure_aux(e, off_part + part_bytes))) return rc;
         char *base = (char *)e->aux;
         PrDdArgs a{};
         a.g = d;
         a.x = (float *)res;
-        a.y = (double *)base;
-        a.r = (double *)(base + n * 8);
-        a.p = (double *)(base + 2 * n * 8);
-        a.part = (double2 *)(base + off_part);
+        a.y = (CSum *)base;
+        a.r = (float *)(base + n * 8);
+        a.rlo = (float *)(base + n * 12);
+        a.p = (float *)(base + n * 16);
+        a.part = (PrPart *)(base + off_part);
         a.abit = e->bitmaps;
         a.queue = e->queues;
         a.qcap = e->qcap;

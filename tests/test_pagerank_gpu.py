@@ -45,8 +45,11 @@ def run_pr(sg, hg, mode, tol=1e-6, max_iter=0, **opts):
 
 def check_td(orc, hg, x, st, tol=1e-6):
     ref, it, res = orc.pagerank(hg.row, hg.col, D, tol, 100000)
-    # same stopping rule: the iteration counts agree up to a tie at the threshold
-    assert abs(st["iterations"] - it) <= 1, (st["iterations"], it, res)
+    # same stopping rule; the ranks are FP32, so the L1 change near the
+    # threshold carries rounding noise of about n ulp(x), which can move the
+    # stop by a few iterations where the change decays slowly (bipartite
+    # graphs oscillate, shrinking by the damping factor per iteration)
+    assert abs(st["iterations"] - it) <= 3, (st["iterations"], it, res)
     same, _, _ = orc.pagerank(hg.row, hg.col, D, 0.0, st["iterations"])
     assert rel_l1(x, same) <= 1e-6
     if st["iterations"] == it:
