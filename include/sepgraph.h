@@ -268,7 +268,12 @@ typedef struct sg_engine_options {
     int32_t grid_blocks;        /* cap on the persistent grid's workgroups (0: fill the
                                    GPU); small grids make rounds cheaper on high-diameter
                                    graphs whose frontiers stay small */
-    int32_t reserved[7];
+    int32_t stage_hubs;         /* sg_bfs push rounds: 1 stages the neighbour lists of
+                                   high-degree frontier vertices in LDS per workgroup (the
+                                   reference's CTA-level expansion of large vertices,
+                                   cta_scheduler_hybrid.cuh:86-142); 0 (default) expands
+                                   their 64-edge chunks wave by wave. Results are the same */
+    int32_t reserved[6];
 } sg_engine_options;
 
 void sg_engine_options_default(sg_engine_options *opt);

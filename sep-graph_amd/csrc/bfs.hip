@@ -7,7 +7,7 @@ using namespace bfsk;
 
 namespace {
 
-template <int BS>
+template <int BS, bool kStage>
 __global__ SG_BFS_BOUNDS(BS) void k_bfs(BfsArgs a)
 {
     if (a.probe) {
@@ -16,11 +16,11 @@ __global__ SG_BFS_BOUNDS(BS) void k_bfs(BfsArgs a)
     }
     const uint32_t xcc = xcc_id();
     BarState gen = bar_begin(a.bar, xcc);
-    bfs_run<BS>(a, blockIdx.x, gridDim.x, xcc, gen);
+    bfs_run<BS, kStage>(a, blockIdx.x, gridDim.x, xcc, gen);
 }
 
 
-template <int BS>
+template <int BS, bool kStage>
 int launch_bfs(sg_engine *e, BfsArgs &args, bool timed)
 {
     sg_engine::PGrid *pg = nullptr;
@@ -28,16 +28,16 @@ int launch_bfs(sg_engine *e, BfsArgs &args, bool timed)
         BfsArgs p = args;
         p.probe = 1;
         p.bar.words = words;
-        hipLaunchKernelGGL(k_bfs<BS>, dim3(blocks), dim3(BS), 0, e->stream, p);
+        hipLaunchKernelGGL((k_bfs<BS, kStage>), dim3(blocks), dim3(BS), 0, e->stream, p);
         SG_HIP(hipGetLastError());
         return SG_OK;
     };
-    int rc = engine_persistent_grid(e, (const void *)k_bfs<BS>, BS, probe, &pg);
+    int rc = engine_persistent_grid(e, (const void *)k_bfs<BS, kStage>, BS, probe, &pg);
     if (rc) return rc;
     bind_barrier(pg, args.bar);
     return engine_launch_persistent(e, [&]() -> int {
         if (timed) SG_HIP(hipEventRecord(e->ev0, e->stream));
-        hipLaunchKernelGGL(k_bfs<BS>, dim3(pg->blocks), dim3(BS), 0, e->stream, args);
+        hipLaunchKernelGGL((k_bfs<BS, kStage>), dim3(pg->blocks), dim3(BS), 0, e->stream, args);
         SG_HIP(hipGetLastError());
         pg->launches++;
         if (timed) SG_HIP(hipEventRecord(e->ev1, e->stream));
@@ -91,9 +91,9 @@ int bfs_launch(sg_engine *e, uint32_t source, int32_t *out, bool traced)
     e->last_algo = 0;
     const int bs = o.block_size ? o.block_size : 1024;
     switch (bs) {
-    case 256: rc = launch_bfs<256>(e, a, traced); break;
-    case 512: rc = launch_bfs<512>(e, a, traced); break;
-    case 1024: rc = launch_bfs<1024>(e, a, traced); break;
+    case 256: rc = o.stage_hubs ? launch_bfs<256, true>(e, a, traced) : launch_bfs<256, false>(e, a, traced); break;
+    case 512: rc = o.stage_hubs ? launch_bfs<512, true>(e, a, traced) : launch_bfs<512, false>(e, a, traced); break;
+    case 1024: rc = o.stage_hubs ? launch_bfs<1024, true>(e, a, traced) : launch_bfs<1024, false>(e, a, traced); break;
     default: SG_FAIL(SG_ERR_UNSUPPORTED, "sg_bfs: block_size must be 256, 512 or 1024");
     }
     if (rc) return rc;
@@ -156,7 +156,7 @@ constexpr int kNaMaxDevices = 64;
 
 struct NaDevice {
     char *ptr = nullptr;   // device address of `scratch` on this device
-    int blocks = 0;        // co-resident grid of k_bfs<kNaBlock>
+    int blocks = 0;        // co-resident grid of k_bfs<kNaBlock, false>
     uint64_t launches = 0; // runs on the barrier words (registration parity)
 };
 std::mutex g_na_mu[kNaMaxDevices];
@@ -175,7 +175,7 @@ int na_zero(hipStream_t s, char *p, size_t bytes)
 }
 
 // Once per device: the array's address, cleared counters, and the largest
-// blocks-per-CU grid of k_bfs<kNaBlock> whose blocks are all resident.
+// blocks-per-CU grid of k_bfs<kNaBlock, false> whose blocks are all resident.
 int na_setup(NaDevice &d, const sg_graph *g, hipStream_t s, const BfsArgs &proto)
 {
     void *p = nullptr;
@@ -185,7 +185,7 @@ int na_setup(NaDevice &d, const sg_graph *g, hipStream_t s, const BfsArgs &proto
     int rc = na_zero(s, d.ptr + kNaOffCtr, kNaBytes - kNaOffCtr);
     if (rc) return rc;
     int per_cu = 0;
-    SG_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void *)k_bfs<kNaBlock>, kNaBlock, 0));
+    SG_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void *)k_bfs<kNaBlock, false>, kNaBlock, 0));
     if (per_cu < 1) SG_FAIL(SG_ERR_HIP, "sg_bfs_noalloc: kernel cannot be resident");
     uint32_t *words = (uint32_t *)(d.ptr + kNaOffProbe);
     for (int pc = per_cu; pc >= 1; pc--) {
@@ -194,7 +194,7 @@ int na_setup(NaDevice &d, const sg_graph *g, hipStream_t s, const BfsArgs &proto
         BfsArgs pa = proto;
         pa.probe = 1;
         pa.bar.words = words;
-        hipLaunchKernelGGL(k_bfs<kNaBlock>, dim3(blocks), dim3(kNaBlock), 0, s, pa);
+        hipLaunchKernelGGL((k_bfs<kNaBlock, false>), dim3(blocks), dim3(kNaBlock), 0, s, pa);
         SG_HIP(hipGetLastError());
         uint32_t counts[kMaxXcd + 2];
         SG_HIP(hipMemcpyAsync(counts, words + bar_count(0), sizeof(counts), hipMemcpyDeviceToHost, s));
@@ -267,7 +267,7 @@ extern "C" int sg_bfs_noalloc(const sg_graph *g, uint32_t source, int32_t *level
     a.bar.rset = (uint32_t)(d.launches & 1u);
     a.abort = (uint32_t *)(d.ptr + kNaOffAbort);
     a.stats = (DevStats *)(d.ptr + kNaOffStats);
-    hipLaunchKernelGGL(k_bfs<kNaBlock>, dim3(d.blocks), dim3(kNaBlock), 0, s, a);
+    hipLaunchKernelGGL((k_bfs<kNaBlock, false>), dim3(d.blocks), dim3(kNaBlock), 0, s, a);
     SG_HIP(hipGetLastError());
     d.launches++;
     int32_t status = 0;

@@ -88,6 +88,14 @@ struct BfsArgs {
     uint32_t lb_push_none, lb_pull_none;
 };
 
+// Hub chunks whose heads a block stages in LDS at a time (with 64-edge
+// chunks: 4096 heads, 16 KiB), and the mark of a staged slot past a chunk's end.
+constexpr uint32_t kStageChunks = 64;
+// Member chunks a block lists before staging them (16 KiB of LDS).
+constexpr uint32_t kHubList = 2048;
+constexpr uint32_t kNoHead = 0xFFFFFFFFu;
+static_assert(kHugeChunk <= kWave, "a staged hub chunk is expanded by one wave in one step");
+
 // Largest source out-degree whose expansion (round 0) block 0 does alone.
 constexpr uint32_t kLocalRound0 = 4096;
 
@@ -310,7 +318,10 @@ __device__ __forceinline__ void pull_group(Round &R, const uint32_t (&wg)[G], co
 // when a launch serves one query per XCD. `gen` is the block's barrier state
 // for a.bar (from bar_begin or the caller's census), advanced by every
 // barrier. Returns false when a bounded wait expired (the run then stopped).
-template <int BS>
+// kStage: push rounds stage hub neighbour lists in LDS (the engine option
+// stage_hubs; a separate instantiation, so the default path keeps its
+// register budget).
+template <int BS, bool kStage = false>
 __device__ __forceinline__ bool bfs_run(const BfsArgs &a, uint32_t brank, uint32_t nblocks, uint32_t xcc,
                                         BarState &gen)
 {
@@ -329,6 +340,10 @@ __device__ __forceinline__ bool bfs_run(const BfsArgs &a, uint32_t brank, uint32
     __shared__ unsigned long long s_lvl[5];
     __shared__ uint32_t s_q[kLdsQ];
     __shared__ uint32_t s_qn, s_flush[2];
+    // hub round: the block's member chunks (first edge, end) and their staged heads
+    __shared__ uint2 s_hc[kStage ? kHubList : 1];
+    __shared__ uint32_t s_he[kStage ? kStageChunks * kHugeChunk : 1];
+    __shared__ uint32_t s_hn;
 
     // ---- prologue: reset the levels (caller order) and seed the source.
     // The previous query left the frontier bitmaps and counters zeroed, and
@@ -444,27 +459,75 @@ __device__ __forceinline__ bool bfs_run(const BfsArgs &a, uint32_t brank, uint32
         const uint32_t *qin = a.queue + (size_t)(L % 3) * a.qcap;
 
         if (!pull) {
-            // (1) high-degree frontier vertices through the static chunk list;
-            //     lane ln of wave gw takes chunk base + ln*nwaves + gw, so the
-            //     chunks of one hub land on different waves.
+            // (1) high-degree frontier vertices through the static chunk list.
+            //     With kStage their neighbour lists are staged in LDS
+            //     (the reference's CTA-level expansion of large vertices,
+            //     cta_scheduler_hybrid.cuh:86-142): thread t of block b
+            //     checks descriptors k * nblocks * BS + t * nblocks + b (the
+            //     chunks of one hub land in different blocks) and lists the
+            //     chunks whose vertex is in the frontier (first edge, end),
+            //     up to kHubList at a time; the block then loads their heads
+            //     into LDS, kStageChunks chunks per pass with every thread's
+            //     loads in flight together, and its waves expand one staged
+            //     chunk per step.
             const uint32_t nhuge = rc.has_huge && !a.lb_push_none ? g.nhuge : 0u;
-            for (uint32_t base = 0; base < nhuge; base += nwaves * kWave) {
+            const uint32_t per = !kStage ? 0u : (nhuge + nblocks * BS - 1) / (nblocks * BS);  // descriptors per thread
+            for (uint32_t k0 = 0; k0 < per;) {
+                if (threadIdx.x == 0) s_hn = 0;
+                __syncthreads();
+                // each pass lists at most kHubList chunks: BS threads x kHubList / BS descriptors
+                const uint32_t k1 = min(per, k0 + kHubList / BS);
+                for (uint32_t k = k0; k < k1; k++) {
+                    const uint32_t c = k * nblocks * BS + threadIdx.x * nblocks + brank;
+                    if (c < nhuge) {
+                        const uint2 dsc = g.huge[c];
+                        const uint32_t end = g.huge_end[c];
+                        if (in_frontier(R, dsc.x)) s_hc[atomicAdd(&s_hn, 1u)] = make_uint2(dsc.y, end);
+                    }
+                }
+                k0 = k1;
+                __syncthreads();
+                const uint32_t nm = s_hn;
+                for (uint32_t m0 = 0; m0 < nm; m0 += kStageChunks) {
+                    const uint32_t mc = min(kStageChunks, nm - m0);
+                    for (uint32_t i = threadIdx.x; i < mc * kHugeChunk; i += BS) {
+                        const uint2 ch = s_hc[m0 + i / kHugeChunk];
+                        const uint32_t e = ch.x + i % kHugeChunk;
+                        s_he[i] = e < ch.y ? g.col[e] : kNoHead;
+                    }
+                    __syncthreads();
+                    for (uint32_t q = threadIdx.x / kWave; q < mc; q += WPB) {
+                        const uint32_t v = s_he[q * kHugeChunk + (uint32_t)ln];
+                        push_visit(R, v != kNoHead, v != kNoHead ? v : 0u);
+                    }
+                    __syncthreads();
+                }
+                __syncthreads();  // every thread has read the count before it is reset
+            }
+            //     Otherwise (the default: on RMAT-22 staging measured 6 %
+            //     slower for the whole query, its block barriers outweighing
+            //     the overlapped loads) lane ln of wave gw takes chunk
+            //     base + ln * nwaves + gw, so the chunks of one hub land on
+            //     different waves, and each wave expands its member chunks
+            //     one after another.
+            for (uint32_t base = 0; !kStage && base < nhuge; base += nwaves * kWave) {
                 const uint32_t c = base + (uint32_t)ln * nwaves + gwave;
                 uint2 dsc = make_uint2(0u, 0u);
+                uint32_t end = 0;
                 bool mem = false;
                 if (c < nhuge) {
                     dsc = g.huge[c];
+                    end = g.huge_end[c];
                     mem = in_frontier(R, dsc.x);
                 }
                 uint64_t mm = ballot(mem);
                 while (mm) {
                     const int t = first_lane(mm);
                     mm &= mm - 1;
-                    const uint32_t u = shfl(dsc.x, t), beg = shfl(dsc.y, t);
-                    const uint32_t end = min(g.row[u + 1], beg + kHugeChunk);
-                    for (uint32_t e0 = beg; e0 < end; e0 += kWave) {
+                    const uint32_t beg = shfl(dsc.y, t), stop = shfl(end, t);
+                    for (uint32_t e0 = beg; e0 < stop; e0 += kWave) {
                         const uint32_t e = e0 + ln;
-                        const bool act = e < end;
+                        const bool act = e < stop;
                         push_visit(R, act, act ? g.col[e] : 0u);
                     }
                 }

@@ -349,6 +349,12 @@ extern "C" int sg_graph_create_ex(uint32_t n, uint64_t m, const uint32_t *row_st
         chunk_list(row_start, kHugeDegree, kHugeChunk, out_chunks, nullptr);
         chunk_list(undirected ? row_start : cs, kPullHubDegree, kPullChunk, in_chunks, &in_hubs);
         if (!rc) rc = upload2(out_chunks, &d.huge, &d.nhuge);
+        if (!rc) {
+            std::vector<uint32_t> ends(out_chunks.size() ? out_chunks.size() : 1);
+            for (size_t c = 0; c < out_chunks.size(); c++)
+                ends[c] = std::min(row_start[out_chunks[c].x + 1], out_chunks[c].y + kHugeChunk);
+            rc = up(ends.data(), out_chunks.size(), &d.huge_end);
+        }
         if (!rc) rc = upload2(in_chunks, &d.hugein, &d.nhugein);
         if (!rc) rc = upload2(in_hubs, &d.hubin, &d.nhubin);
     }

@@ -67,6 +67,7 @@ struct DevGraph {
     // (vertex, first edge), so a push round can spread its edges over the grid.
     const uint2 *huge;
     uint32_t nhuge;           // number of chunk descriptors
+    const uint32_t *huge_end; // end (one past the last edge) of each chunk
     // Split of the in-edges of every vertex with in-degree > kPullHubDegree
     // over the CSC (vertex, first in-edge), for pull rounds that visit all
     // in-edges (SSSP), so no wave walks a long in-edge list alone.

@@ -75,7 +75,8 @@ class EngineOptions(C.Structure):
         ("block_size", C.c_int32),
         ("large_degree", C.c_int32),
         ("grid_blocks", C.c_int32),
-        ("reserved", C.c_int32 * 7),
+        ("stage_hubs", C.c_int32),
+        ("reserved", C.c_int32 * 6),
     ]
 
 

@@ -9,7 +9,7 @@ import pytest
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 STRUCTS = {
-    "sg_engine_options": ("EngineOptions", ["alpha", "block_size", "grid_blocks", "reserved"]),
+    "sg_engine_options": ("EngineOptions", ["alpha", "block_size", "grid_blocks", "stage_hubs", "reserved"]),
     "sg_stats": ("Stats", ["kernel_ms", "residual", "status", "reserved"]),
     "sg_round_trace": ("RoundTrace", ["in_edges", "time_us"]),
     "sg_server_config": ("ServerConfig", ["max_rows", "max_batch", "bfs_packed_min", "sssp_packed_min", "mode", "reserved"]),

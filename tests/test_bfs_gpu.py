@@ -267,3 +267,15 @@ def test_options_outside_the_implementation_are_rejected(sg, gpu, rmat16_sym):
     with pytest.raises(sg.SepGraphError) as ei:
         e.bc_dependency([0, 1], 2)
     assert ei.value.code == -8
+
+
+@pytest.mark.parametrize("block_size", [256, 1024])
+def test_bfs_hub_lists_staged_in_lds(sg, orc, gpu, block_size):
+    """stage_hubs=1: push rounds stage the neighbour lists of high-degree
+    frontier vertices in LDS per workgroup; levels stay bit-exact."""
+    for hg in (sg.rmat_graph(18, 16, symmetrize=False, seed=21), sg.rmat_graph(16, 16, symmetrize=True, seed=22)):
+        e = sg.Engine(sg.Graph(hg), stage_hubs=1, block_size=block_size)
+        for s in sg.pick_sources(hg.row, 6, seed=23):
+            e.bfs(int(s))
+            assert np.array_equal(e.gather(np.int32), orc.bfs(hg.row, hg.col, int(s))), (hg.name, int(s))
+            assert e.stats()["status"] == 0
