@@ -173,7 +173,7 @@ def config5(sg, args, rates):
     share = np.array(live, np.float64) / sum(live)
     engines = [sg.Engine(g24), sg.Engine(ggrid)]
     out = serving_bench(sg, [g24, ggrid], [h24, hgrid], engines, share, "rmat24-directed+grid2048", rates,
-                        args.serve_count, args.serve_plain, seed=40)
+                        args.config5_count, args.serve_plain, seed=40)
     for e in engines:
         e.close()
     g24.close()
@@ -216,9 +216,13 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--sssp-sources", type=int, default=16)
     ap.add_argument("--no-serving", action="store_true")
-    ap.add_argument("--serve-rates", type=str, default="1000,2000,3000,4000,5000,7000,10000,20000,50000",
-                    help="Poisson arrival rates (queries/s) for the serving runs")
+    ap.add_argument("--serve-rates", type=str, default="500,1000,1500,2000,3000,5000",
+                    help="Poisson arrival rates (queries/s) for the RMAT serving run (stops once saturated)")
+    ap.add_argument("--config5-rates", type=str, default="30,60,100,1000",
+                    help="Poisson arrival rates (queries/s) for the RMAT-24 + grid serving run (stops once "
+                         "saturated; its configuration names 1k-50k q/s, above what one GPU serves)")
     ap.add_argument("--serve-count", type=int, default=2000, help="queries per arrival rate")
+    ap.add_argument("--config5-count", type=int, default=600, help="queries per arrival rate, RMAT-24 + grid")
     ap.add_argument("--serve-plain", type=int, default=200, help="queries timed one at a time (plain baseline)")
     ap.add_argument("--config5", type=int, default=1,
                     help="also serve RMAT-24 and the 2048x2048 grid resident together (0: skip)")
@@ -325,7 +329,7 @@ def main():
                                 args.serve_count, args.serve_plain, seed=4)
     serving5 = None
     if args.config5:
-        serving5 = config5(sg, args, rates)
+        serving5 = config5(sg, args, [float(r) for r in args.config5_rates.split(",")])
 
     cpu = None
     if not args.no_cpu:

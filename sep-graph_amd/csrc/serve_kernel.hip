@@ -71,6 +71,7 @@ struct ServeArgs {
     sg::ServeLayout L;
     uint32_t qcap;
     GridBarrier bar;        // the launch's census area, then one area per XCD
+    uint32_t claim_limit;   // polls a slot's blocks wait for its leader's claim
     uint32_t *abort;
     // options
     double alpha, beta;
@@ -265,9 +266,15 @@ __global__ __launch_bounds__(BS) void k_serve(ServeArgs a)
     char *base = a.slots + (size_t)a.slot_of[xcc] * a.L.total;
     DevStats *stats = (DevStats *)(base + a.L.stats);
     const bool leader = brank == 0 && threadIdx.x == 0;
+    // The slot's other blocks wait for the leader's claim as long as no slot
+    // has work for it, i.e. as long as the longest query running in another
+    // slot: that wait gets a bound far above any query (the leader itself
+    // always returns: on a claim, a stop, an abort or the idle vote).
+    GridBarrier xclaim = xb;
+    xclaim.spin_limit = a.claim_limit;
     for (;;) {
         if (leader) a.mail[xcc] = claim_next(a, xcc, s_live);
-        if (!grid_sync(xb, xcc, gen, a.abort)) break;
+        if (!grid_sync(xclaim, xcc, gen, a.abort)) break;
         if (threadIdx.x == 0) s_mail = a.mail[xcc];
         __syncthreads();
         Mail m;
@@ -490,6 +497,7 @@ int serve_create(sg_graph *const *graphs, int ngraphs, const sg_engine_options &
     a.qcap = qcap;
     a.bar.words = (uint32_t *)(d->dev + o_bar);
     a.bar.spin_limit = 1u << 22;
+    a.claim_limit = 1u << 31;  // about a minute of polling
     a.alpha = o.alpha == 0.0 ? 1.0 : o.alpha;
     a.beta = o.beta;
     a.force = force;

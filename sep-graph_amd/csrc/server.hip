@@ -107,6 +107,7 @@ struct sg_server {
     std::thread worker;
     // counters
     uint64_t submitted = 0, completed = 0, failed = 0, batches = 0, batched_queries = 0;
+    int last_failure = SG_OK;  // status of the last failed query
     double busy_s = 0.0;
 
     bool has_work() const
@@ -117,10 +118,12 @@ struct sg_server {
         return false;
     }
 
-    void finish_locked(uint64_t t, Req &r, bool ok, double t_done)
+    // status: the query's error when !ok (SG_ERR_HIP when none is known)
+    void finish_locked(uint64_t t, Req &r, bool ok, double t_done, int status = SG_ERR_HIP)
     {
         r.state = ok ? kDone : kFailed;
-        r.status = ok ? SG_OK : SG_ERR_HIP;
+        r.status = ok ? SG_OK : status;
+        if (!ok) last_failure = status;
         r.t_done = t_done;
         if (ok) completed++;
         else failed++;
@@ -181,7 +184,7 @@ struct sg_server {
                 if (serve_poll(sd, inflight[i].first, &st)) {
                     const uint64_t t = inflight[i].second;
                     batched_queries++;
-                    finish_locked(t, reqs[t], st == SG_OK, tn);
+                    finish_locked(t, reqs[t], st == SG_OK, tn, st);
                     inflight[i] = inflight.back();
                     inflight.pop_back();
                     changed = true;
@@ -196,7 +199,7 @@ struct sg_server {
                 for (size_t i = 0; i < inflight.size();) {
                     if ((int32_t)(head - inflight[i].first) > 0) {
                         const uint64_t t = inflight[i].second;
-                        finish_locked(t, reqs[t], false, tn);
+                        finish_locked(t, reqs[t], false, tn, rc);
                         inflight[i] = inflight.back();
                         inflight.pop_back();
                         changed = true;
@@ -493,7 +496,8 @@ extern "C" int sg_server_wait(sg_server *s, uint64_t ticket, int32_t timeout_ms)
     else if (!s->cv_done.wait_for(lk, std::chrono::milliseconds(timeout_ms), done))
         SG_FAIL(SG_ERR_TIMEOUT, "sg_server_wait: not finished");
     auto jt = s->reqs.find(ticket);
-    if (jt != s->reqs.end() && jt->second.state == kFailed) SG_FAIL(jt->second.status, "sg_server_wait: batch failed");
+    if (jt != s->reqs.end() && jt->second.state == kFailed)
+        SG_FAIL(jt->second.status, "sg_server_wait: query failed with status " + std::to_string(jt->second.status));
     return SG_OK;
 }
 
@@ -600,7 +604,9 @@ extern "C" int sg_server_replay_multi(sg_server *s, const int32_t *graphs, const
     }
     std::unique_lock<std::mutex> lk(s->mu);
     s->cv_done.wait(lk, [&] { return s->completed + s->failed - base_completed - base_failed >= (uint64_t)count; });
-    if (s->failed != base_failed) SG_FAIL(SG_ERR_HIP, "sg_server_replay: a batch failed");
+    if (s->failed != base_failed)
+        SG_FAIL(s->last_failure, "sg_server_replay: " + std::to_string(s->failed - base_failed) +
+                                     " queries failed; the last with status " + std::to_string(s->last_failure));
     return SG_OK;
 }
 
