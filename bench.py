@@ -157,6 +157,31 @@ def serving_bench(sg, graphs, hgs, engines, share, label, rates, count, plain_n,
     }
 
 
+def lone_times(sg, e, hg, k):
+    """Mean milliseconds of single BFS, SSSP (with and without the near/far
+    priority schedule) and BC-dependency queries from k seeded non-isolated
+    sources, each synchronised, after one untimed query of each kind."""
+    src = [int(s) for s in sg.pick_sources(hg.row, k, seed=77)]
+    runs = [
+        ("bfs_ms", lambda s: e.bfs(s), {}),
+        ("sssp_ms", lambda s: e.sssp(s), {}),
+        ("sssp_priority_ms", lambda s: e.sssp(s), {"priority": 1}),
+        ("bc_dependency_ms", lambda s: e.bc_dependency([s], 1), {}),
+    ]
+    out = {}
+    for name, run, opts in runs:
+        e.set_options(**opts)
+        run(src[0])
+        e.sync()
+        t0 = time.perf_counter()
+        for s in src:
+            run(s)
+            e.sync()
+        out[name] = round((time.perf_counter() - t0) * 1e3 / len(src), 3)
+    e.set_options()
+    return out
+
+
 def config5(sg, args, rates):
     """Serving on a directed RMAT-24 (edge factor 16, weights in [1,64)) and
     the 2048x2048 grid (10% of edges deleted, weights in [1,1000)), resident
@@ -172,8 +197,11 @@ def config5(sg, args, rates):
     live = [int((np.diff(h.row.astype(np.int64)) > 0).sum()) for h in (h24, hgrid)]
     share = np.array(live, np.float64) / sum(live)
     engines = [sg.Engine(g24), sg.Engine(ggrid)]
+    lone = {"rmat24-directed": lone_times(sg, engines[0], h24, 8), "grid2048": lone_times(sg, engines[1], hgrid, 8)}
+    log(f"[config5] one query at a time: {lone}")
     out = serving_bench(sg, [g24, ggrid], [h24, hgrid], engines, share, "rmat24-directed+grid2048", rates,
                         args.config5_count, args.serve_plain, seed=40)
+    out["lone"] = lone
     for e in engines:
         e.close()
     g24.close()
@@ -218,11 +246,11 @@ def main():
     ap.add_argument("--no-serving", action="store_true")
     ap.add_argument("--serve-rates", type=str, default="500,1000,1500,2000,3000,5000",
                     help="Poisson arrival rates (queries/s) for the RMAT serving run (stops once saturated)")
-    ap.add_argument("--config5-rates", type=str, default="30,60,100,1000",
+    ap.add_argument("--config5-rates", type=str, default="50,80,1000",
                     help="Poisson arrival rates (queries/s) for the RMAT-24 + grid serving run (stops once "
                          "saturated; its configuration names 1k-50k q/s, above what one GPU serves)")
     ap.add_argument("--serve-count", type=int, default=2000, help="queries per arrival rate")
-    ap.add_argument("--config5-count", type=int, default=600, help="queries per arrival rate, RMAT-24 + grid")
+    ap.add_argument("--config5-count", type=int, default=2000, help="queries per arrival rate, RMAT-24 + grid")
     ap.add_argument("--serve-plain", type=int, default=200, help="queries timed one at a time (plain baseline)")
     ap.add_argument("--config5", type=int, default=1,
                     help="also serve RMAT-24 and the 2048x2048 grid resident together (0: skip)")
