@@ -142,7 +142,14 @@ struct sg_server {
         std::vector<std::pair<uint32_t, uint64_t>> inflight;  // (ring sequence, ticket)
         std::unique_lock<std::mutex> lk(mu);
         for (;;) {
-            if (inflight.empty()) cv_work.wait(lk, [&] { return stop || has_work(); });
+            if (inflight.empty()) {
+                // while a launch drains its idle time, poll it every
+                // millisecond, so its end (busy time, launch count) is seen
+                if (running_)
+                    cv_work.wait_for(lk, std::chrono::milliseconds(1), [&] { return stop || has_work(); });
+                else
+                    cv_work.wait(lk, [&] { return stop || has_work(); });
+            }
             if (stop) return;
             const double tp = now_s();
             while (inflight.size() < kServeRing && !free_rows.empty()) {

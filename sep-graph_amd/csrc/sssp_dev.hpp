@@ -385,12 +385,17 @@ __device__ __forceinline__ void pull_slice(Round &R, uint32_t w)
         for (uint32_t j0 = 0; j0 < rows; j0 += U) {
             uint32_t u[U], du[U], wt[U];
 #pragma unroll
-            for (int s = 0; s < U; s++) u[s] = (want && j0 + s < rows) ? S[(size_t)(j0 + s) * kWave] : kInf;
+            for (int s = 0; s < U; s++) {
+                const bool in = want && j0 + s < rows;
+                u[s] = in ? S[(size_t)(j0 + s) * kWave] : kInf;
+                wt[s] = in ? (Wt ? Wt[(size_t)(j0 + s) * kWave] : 1u) : 0u;
+            }
+            // an in-edge whose weight alone cannot beat the lane's best (every
+            // frontier distance is at least fmin) skips its distance read
 #pragma unroll
             for (int s = 0; s < U; s++) {
-                const bool f = u[s] != kInf && in_frontier(R, u[s]);
+                const bool f = u[s] != kInf && sat_add(R.fmin, wt[s]) < best && in_frontier(R, u[s]);
                 du[s] = f ? a.dist[u[s]] : kInf;
-                wt[s] = f ? (Wt ? Wt[(size_t)(j0 + s) * kWave] : 1u) : 0u;
             }
 #pragma unroll
             for (int s = 0; s < U; s++)
@@ -423,12 +428,13 @@ __device__ __forceinline__ void pull_hub_chunks(Round &R, const uint2 (&dsc)[NC]
     const DevGraph &g = a.g;
     const int ln = lane();
     constexpr int U = kPullChunk / kWave;
-    uint32_t end[NC], u[NC][U], du[NC][U], wt[NC][U];
+    uint32_t end[NC], dv[NC], u[NC][U], du[NC][U], wt[NC][U];
 #pragma unroll
     for (int c = 0; c < NC; c++) {
         end[c] = c < nc ? min(g.cstart[dsc[c].x + 1], dsc[c].y + kPullChunk) : 0u;
+        dv[c] = c < nc ? a.dist[dsc[c].x] : 0u;
         // a hub no frontier in-neighbour can lower skips its chunks
-        if (c < nc && settled(R, dsc[c].x, a.dist[dsc[c].x])) end[c] = 0u;
+        if (c < nc && settled(R, dsc[c].x, dv[c])) end[c] = 0u;
     }
 #pragma unroll
     for (int c = 0; c < NC; c++)
@@ -436,15 +442,16 @@ __device__ __forceinline__ void pull_hub_chunks(Round &R, const uint2 (&dsc)[NC]
         for (int s = 0; s < U; s++) {
             const uint32_t ed = dsc[c].y + (uint32_t)(s * kWave + ln);
             u[c][s] = ed < end[c] ? g.csrc[ed] : kInf;
+            wt[c][s] = ed < end[c] ? edge_w(g.cw, ed) : 0u;
         }
+    // an in-edge whose weight alone cannot beat the hub's distance (every
+    // frontier distance is at least fmin) skips its distance read
 #pragma unroll
     for (int c = 0; c < NC; c++)
 #pragma unroll
         for (int s = 0; s < U; s++) {
-            const uint32_t ed = dsc[c].y + (uint32_t)(s * kWave + ln);
-            const bool f = u[c][s] != kInf && in_frontier(R, u[c][s]);
+            const bool f = u[c][s] != kInf && sat_add(R.fmin, wt[c][s]) < dv[c] && in_frontier(R, u[c][s]);
             du[c][s] = f ? a.dist[u[c][s]] : kInf;
-            wt[c][s] = f ? edge_w(g.cw, ed) : 0u;
         }
 #pragma unroll
     for (int c = 0; c < NC; c++) {
