@@ -418,66 +418,114 @@ __device__ __forceinline__ void pull_slice(Round &R, uint32_t w)
     }
 }
 
-// Pull for up to NC chunks of hubs' in-edges at once (each load of a stage
-// issued for every chunk before any is used). A chunk's minimum candidate is
-// merged into its hub with atomicMin, as several waves share a hub.
+// The merges of a group of hub chunks issued by pull_hubs, finished: a
+// chunk whose atomicMin found the hub above its candidate lowered it, and the
+// chunk that sets the hub's next-frontier bit counts it (several may lower
+// it). pv[i] is kInf where no atomic was issued; only lane 0 issued them.
 template <int NC>
-__device__ __forceinline__ void pull_hub_chunks(Round &R, const uint2 (&dsc)[NC], int nc)
+__device__ __forceinline__ void finish_merges(Round &R, const uint32_t (&pv)[NC], const uint32_t (&pm)[NC],
+                                              const uint32_t (&pold)[NC])
+{
+    const DevGraph &g = R.a->g;
+    if (lane() != 0) return;
+#pragma unroll
+    for (int i = 0; i < NC; i++) {
+        const uint32_t v = pv[i];
+        if (v == kInf || pm[i] >= pold[i]) continue;
+        R.minv = min(R.minv, pm[i]);
+        const uint64_t bit = 1ull << (v & 63);
+        if (!(atomicOr((unsigned long long *)&R.fnext[v >> 6], (unsigned long long)bit) & bit)) {
+            const uint32_t dv = g.row[v + 1] - g.row[v];
+            R.scout += dv;
+            R.count++;
+            R.huge |= dv > kHugeDegree;
+        }
+    }
+}
+
+// Pull for the hub in-edge chunks dealt to this wave (first, first +
+// stride, ...), kHubChunks at a time; each chunk's minimum candidate is merged
+// into its hub with atomicMin, as several waves share a hub. The loads are
+// staged so that each round trip serves as much as it can: the next group's
+// descriptors are read while this group is worked; a chunk's hub distance,
+// hub end, source ids and weights are read together (ids read past the hub's
+// end, still inside the edge array, are dropped afterwards); and a chunk's
+// atomicMin is not waited for: whether it lowered the hub is taken once the
+// next group's loads are in flight.
+__device__ __forceinline__ void pull_hubs(Round &R, uint32_t first, uint32_t stride, uint32_t nchunks)
 {
     const SsspArgs &a = *R.a;
     const DevGraph &g = a.g;
     const int ln = lane();
     constexpr int U = kPullChunk / kWave;
-    uint32_t end[NC], dv[NC], u[NC][U], du[NC][U], wt[NC][U];
+    constexpr int NC = kHubChunks;
+    uint2 nxt[NC];
 #pragma unroll
-    for (int c = 0; c < NC; c++) {
-        end[c] = c < nc ? min(g.cstart[dsc[c].x + 1], dsc[c].y + kPullChunk) : 0u;
-        dv[c] = c < nc ? a.dist[dsc[c].x] : 0u;
-        // a hub no frontier in-neighbour can lower skips its chunks
-        if (c < nc && settled(R, dsc[c].x, dv[c])) end[c] = 0u;
+    for (int i = 0; i < NC; i++) {
+        const uint32_t c = first + (uint32_t)i * stride;
+        nxt[i] = c < nchunks ? g.hugein[c] : make_uint2(kInf, 0u);
     }
+    uint32_t pv[NC], pm[NC], pold[NC];
 #pragma unroll
-    for (int c = 0; c < NC; c++)
+    for (int i = 0; i < NC; i++) {
+        pv[i] = kInf;
+        pm[i] = kInf;
+        pold[i] = 0u;
+    }
+    for (uint32_t c0 = first; c0 < nchunks; c0 += NC * stride) {
+        uint2 dsc[NC];
 #pragma unroll
-        for (int s = 0; s < U; s++) {
-            const uint32_t ed = dsc[c].y + (uint32_t)(s * kWave + ln);
-            u[c][s] = ed < end[c] ? g.csrc[ed] : kInf;
-            wt[c][s] = ed < end[c] ? edge_w(g.cw, ed) : 0u;
+        for (int i = 0; i < NC; i++) {
+            dsc[i] = nxt[i];
+            const uint32_t c = c0 + (uint32_t)(NC + i) * stride;
+            nxt[i] = c < nchunks ? g.hugein[c] : make_uint2(kInf, 0u);
         }
-    // an in-edge whose weight alone cannot beat the hub's distance (every
-    // frontier distance is at least fmin) skips its distance read
+        uint32_t hend[NC], dv[NC], u[NC][U], wt[NC][U], du[NC][U];
 #pragma unroll
-    for (int c = 0; c < NC; c++)
+        for (int i = 0; i < NC; i++) {
+            const bool ok = dsc[i].x != kInf;
+            hend[i] = ok ? g.cstart[dsc[i].x + 1] : 0u;
+            dv[i] = ok ? a.dist[dsc[i].x] : 0u;
 #pragma unroll
-        for (int s = 0; s < U; s++) {
-            const bool f = u[c][s] != kInf && sat_add(R.fmin, wt[c][s]) < dv[c] && in_frontier(R, u[c][s]);
-            du[c][s] = f ? a.dist[u[c][s]] : kInf;
-        }
-#pragma unroll
-    for (int c = 0; c < NC; c++) {
-        if (c >= nc) break;
-        const uint32_t v = dsc[c].x;
-        uint32_t m = kInf;
-#pragma unroll
-        for (int s = 0; s < U; s++)
-            if (du[c][s] != kInf) m = min(m, sat_add(du[c][s], wt[c][s]));
-#pragma unroll
-        for (int d = kWave / 2; d > 0; d >>= 1) m = min(m, (uint32_t)__shfl_xor(m, d, kWave));
-        if (ln == 0 && m < a.dist[v]) {
-            const uint32_t old = atomicMin(&a.dist[v], m);
-            if (m < old) {
-                R.minv = min(R.minv, m);
-                // several chunks may lower the hub: the one setting its bit counts it
-                const uint64_t bit = 1ull << (v & 63);
-                if (!(atomicOr((unsigned long long *)&R.fnext[v >> 6], (unsigned long long)bit) & bit)) {
-                    const uint32_t dv = g.row[v + 1] - g.row[v];
-                    R.scout += dv;
-                    R.count++;
-                    R.huge |= dv > kHugeDegree;
-                }
+            for (int s = 0; s < U; s++) {
+                const uint32_t ed = dsc[i].y + (uint32_t)(s * kWave + ln);
+                const bool in = ok && ed < g.m;
+                u[i][s] = in ? g.csrc[ed] : kInf;
+                wt[i][s] = in ? edge_w(g.cw, ed) : 0u;
             }
         }
+        finish_merges<NC>(R, pv, pm, pold);
+        // an in-edge whose weight alone cannot beat the hub's distance (every
+        // frontier distance is at least fmin) skips its distance read; so do
+        // all in-edges of a hub no frontier vertex can lower
+#pragma unroll
+        for (int i = 0; i < NC; i++) {
+            const uint32_t end = min(hend[i], dsc[i].y + kPullChunk);
+#pragma unroll
+            for (int s = 0; s < U; s++) {
+                const uint32_t ed = dsc[i].y + (uint32_t)(s * kWave + ln);
+                const bool f = ed < end && u[i][s] != kInf && sat_add(R.fmin, wt[i][s]) < dv[i] &&
+                               in_frontier(R, u[i][s]);
+                du[i][s] = f ? a.dist[u[i][s]] : kInf;
+            }
+        }
+#pragma unroll
+        for (int i = 0; i < NC; i++) {
+            uint32_t m = kInf;
+#pragma unroll
+            for (int s = 0; s < U; s++)
+                if (du[i][s] != kInf) m = min(m, sat_add(du[i][s], wt[i][s]));
+#pragma unroll
+            for (int d = kWave / 2; d > 0; d >>= 1) m = min(m, (uint32_t)__shfl_xor(m, d, kWave));
+            // dv is this chunk's earlier read of the hub: a candidate not
+            // below it cannot lower the hub
+            const bool issue = dsc[i].x != kInf && m < dv[i];
+            pv[i] = issue ? dsc[i].x : kInf;
+            pm[i] = m;
+            if (issue && ln == 0) pold[i] = atomicMin(&a.dist[dsc[i].x], m);
+        }
     }
+    finish_merges<NC>(R, pv, pm, pold);
 }
 
 // `gen`: the block's barrier state (from bar_begin, or the census of a
@@ -658,17 +706,7 @@ __device__ __forceinline__ bool sssp_run(const SsspArgs &a, uint32_t brank, uint
             // over all waves, kHubChunks chunks per wave at a time; (2) every
             // other vertex with in-edges, a word at a time.
             const uint32_t nhugein = a.lb_pull_none ? 0u : g.nhugein;
-            for (uint32_t c0 = gwave; c0 < nhugein; c0 += kHubChunks * nwaves) {
-                uint2 dsc[kHubChunks];
-                int nc = 0;
-#pragma unroll
-                for (int i = 0; i < kHubChunks; i++) {
-                    const uint32_t c = c0 + (uint32_t)i * nwaves;
-                    dsc[i] = c < nhugein ? g.hugein[c] : make_uint2(0u, 0u);
-                    nc += c < nhugein ? 1 : 0;
-                }
-                pull_hub_chunks<kHubChunks>(R, dsc, nc);
-            }
+            if (gwave < nhugein) pull_hubs(R, gwave, nwaves, nhugein);
             // words interleaved over waves: heavy vertices cluster at low ids
             // (RMAT, degree-ordered graphs), so contiguous ranges would leave a
             // few waves with most of the in-edges
