@@ -83,7 +83,7 @@ def draw_mix(rng, pools, share, count):
     return gi.astype(np.int32), kinds.astype(np.int32), srcs
 
 
-def serving_bench(sg, graphs, hgs, engines, share, label, rates, count, plain_n, seed, mode=0):
+def serving_bench(sg, graphs, hgs, engines, share, label, rates, count, plain_n, seed, mode=0, prio_graphs=0):
     """Queries/s and latency of the server against plain one-at-a-time
     execution on the same resident graphs: a seeded Poisson stream of the
     60/30/10 BFS/SSSP/BC-dependency mix, sources uniform over non-isolated
@@ -104,7 +104,7 @@ def serving_bench(sg, graphs, hgs, engines, share, label, rates, count, plain_n,
         e.sync()
     plain_qps = plain_n / (time.perf_counter() - t0)
     log(f"[{label}] plain one-at-a-time: {plain_qps:.1f} queries/s")
-    srv = sg.Server(graphs, memory_budget=16 << 30, max_rows=1024, mode=mode)
+    srv = sg.Server(graphs, memory_budget=16 << 30, max_rows=1024, mode=mode, sssp_prio_graphs=prio_graphs)
     # warm the serving kernel (residency probe done at creation; first launch)
     wg, wk, ws = draw_mix(rng, pools, share, 16)
     srv.replay(wk, ws, [0.0] * 16, graphs=wg)
@@ -157,10 +157,11 @@ def serving_bench(sg, graphs, hgs, engines, share, label, rates, count, plain_n,
     }
 
 
-def lone_times(sg, e, hg, k):
+def lone_times(sg, e, hg, k, base):
     """Mean milliseconds of single BFS, SSSP (with and without the near/far
     priority schedule) and BC-dependency queries from k seeded non-isolated
-    sources, each synchronised, after one untimed query of each kind."""
+    sources, each synchronised, after one untimed query of each kind; the
+    engine is left on the options `base`."""
     src = [int(s) for s in sg.pick_sources(hg.row, k, seed=77)]
     runs = [
         ("bfs_ms", lambda s: e.bfs(s), {}),
@@ -178,7 +179,7 @@ def lone_times(sg, e, hg, k):
             run(s)
             e.sync()
         out[name] = round((time.perf_counter() - t0) * 1e3 / len(src), 3)
-    e.set_options()
+    e.set_options(**base)
     return out
 
 
@@ -196,11 +197,17 @@ def config5(sg, args, rates):
         f"({(g24.info()[2] + ggrid.info()[2]) / 1e9:.2f} GB)")
     live = [int((np.diff(h.row.astype(np.int64)) > 0).sum()) for h in (h24, hgrid)]
     share = np.array(live, np.float64) / sum(live)
-    engines = [sg.Engine(g24), sg.Engine(ggrid)]
-    lone = {"rmat24-directed": lone_times(sg, engines[0], h24, 8), "grid2048": lone_times(sg, engines[1], hgrid, 8)}
+    # the grid's SSSP runs the two-level priority schedule, the reference's
+    # setting for road-network-like graphs (hybrid_sssp --sparse); RMAT-24's
+    # does not (slower there), in the server and in the plain baseline alike
+    opts = [{}, {"priority": 1}]
+    engines = [sg.Engine(g24, **opts[0]), sg.Engine(ggrid, **opts[1])]
+    lone = {"rmat24-directed": lone_times(sg, engines[0], h24, 8, opts[0]),
+            "grid2048": lone_times(sg, engines[1], hgrid, 8, opts[1])}
     log(f"[config5] one query at a time: {lone}")
     out = serving_bench(sg, [g24, ggrid], [h24, hgrid], engines, share, "rmat24-directed+grid2048", rates,
-                        args.config5_count, args.serve_plain, seed=40)
+                        args.config5_count, args.serve_plain, seed=40, prio_graphs=0b10)
+    out["sssp_priority"] = {"rmat24-directed": False, "grid2048": True}
     out["lone"] = lone
     for e in engines:
         e.close()
@@ -246,7 +253,7 @@ def main():
     ap.add_argument("--no-serving", action="store_true")
     ap.add_argument("--serve-rates", type=str, default="500,1000,1500,2000,3000,5000",
                     help="Poisson arrival rates (queries/s) for the RMAT serving run (stops once saturated)")
-    ap.add_argument("--config5-rates", type=str, default="50,80,1000",
+    ap.add_argument("--config5-rates", type=str, default="50,80,120,1000",
                     help="Poisson arrival rates (queries/s) for the RMAT-24 + grid serving run (stops once "
                          "saturated; its configuration names 1k-50k q/s, above what one GPU serves)")
     ap.add_argument("--serve-count", type=int, default=2000, help="queries per arrival rate")

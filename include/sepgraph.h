@@ -472,7 +472,13 @@ typedef struct sg_server_config {
      * SG_SERVE_BATCHES only */
     int32_t bfs_packed_min, sssp_packed_min;
     int32_t mode;            /* SG_SERVE_SLOTS (default) or SG_SERVE_BATCHES */
-    int32_t reserved[5];
+    /* bit i: SSSP queries on graph i use the two-level priority schedule
+     * (bucket width options.priority_threshold, or the heuristic when 0), as
+     * options.priority = SG_PRIO_LOW_HIGH does for every graph; the
+     * reference's hybrid_sssp sets it per dataset with --sparse
+     * (samples/hybrid_sssp/hybrid_sssp.cu:201-206). SG_SERVE_SLOTS only. */
+    uint32_t sssp_prio_graphs;
+    int32_t reserved[4];
 } sg_server_config;
 
 typedef struct sg_server_stats {

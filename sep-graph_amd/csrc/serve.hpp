@@ -42,9 +42,11 @@ struct ServeDev;
 // Device state, the pinned ring and the co-resident grid of the serving
 // kernel for up to kServeGraphs graphs on one device, writing results into
 // `pool` (rows `row_bytes` apart, at least 4 n of the largest graph; the
-// caller's vertex order). Launches go to `stream`.
-int serve_create(sg_graph *const *graphs, int ngraphs, const sg_engine_options &opt, hipStream_t stream, void *pool,
-                 uint64_t row_bytes, ServeDev **out);
+// caller's vertex order). Launches go to `stream`. SSSP on graph i uses the
+// two-level priority schedule when opt asks for it or bit i of prio_graphs
+// is set.
+int serve_create(sg_graph *const *graphs, int ngraphs, const sg_engine_options &opt, uint32_t prio_graphs,
+                 hipStream_t stream, void *pool, uint64_t row_bytes, ServeDev **out);
 // Asks a running launch to finish once the ring is drained, waits, frees.
 void serve_destroy(ServeDev *d);
 // Publish a query (at most kServeRing may be unfinished); returns its

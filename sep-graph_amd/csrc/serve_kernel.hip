@@ -414,8 +414,8 @@ int size_serve_grid(ServeDev *d)
 
 }  // namespace
 
-int serve_create(sg_graph *const *graphs, int ngraphs, const sg_engine_options &o, hipStream_t stream, void *pool,
-                 uint64_t row_bytes, ServeDev **out)
+int serve_create(sg_graph *const *graphs, int ngraphs, const sg_engine_options &o, uint32_t prio_graphs,
+                 hipStream_t stream, void *pool, uint64_t row_bytes, ServeDev **out)
 {
     *out = nullptr;
     if (ngraphs < 1 || ngraphs > kServeGraphs) SG_FAIL(SG_ERR_RANGE, "server: 1..4 graphs");
@@ -471,7 +471,7 @@ int serve_create(sg_graph *const *graphs, int ngraphs, const sg_engine_options &
         a.inv[i] = gi->relabeled ? gi->d_inv : nullptr;
         a.total_wl[i] = (uint32_t)((double)dg.m * o.edge_factor);
         a.delta[i] = 0;
-        if (o.priority == SG_PRIO_LOW_HIGH) {
+        if (o.priority == SG_PRIO_LOW_HIGH || ((prio_graphs >> i) & 1u)) {
             // the SSSP two-level bucket width, as for sg_sssp
             double dl = o.priority_threshold;
             if (!(dl >= 1.0)) {

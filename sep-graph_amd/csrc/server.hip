@@ -351,6 +351,9 @@ int server_create(sg_graph *const *graphs, int32_t ngraphs, const sg_server_conf
     if (c.mode != SG_SERVE_SLOTS && c.mode != SG_SERVE_BATCHES) SG_FAIL(SG_ERR_INVALID, "sg_server_create: unknown mode");
     if (c.mode == SG_SERVE_BATCHES && ngraphs > 1)
         SG_FAIL(SG_ERR_UNSUPPORTED, "sg_server_create: several graphs need SG_SERVE_SLOTS");
+    if (c.sssp_prio_graphs >> ngraphs) SG_FAIL(SG_ERR_RANGE, "sg_server_create: sssp_prio_graphs names a graph not given");
+    if (c.mode == SG_SERVE_BATCHES && c.sssp_prio_graphs)
+        SG_FAIL(SG_ERR_UNSUPPORTED, "sg_server_create: per-graph SSSP priority needs SG_SERVE_SLOTS");
     if (!c.bfs_packed_min) c.bfs_packed_min = (int32_t)kMsBfsMin;
     if (!c.sssp_packed_min) c.sssp_packed_min = (int32_t)kMsSsspMin;
     // a result row holds one query's values on the largest graph
@@ -388,7 +391,7 @@ int server_create(sg_graph *const *graphs, int32_t ngraphs, const sg_server_conf
     }
     if ((rc = engine_alloc(&s->pool, (uint64_t)s->nrows * row_bytes))) return fail(rc);
     if (c.mode == SG_SERVE_SLOTS &&
-        (rc = serve_create(graphs, ngraphs, s->e->opt, s->e->stream, s->pool, row_bytes, &s->sd)))
+        (rc = serve_create(graphs, ngraphs, s->e->opt, c.sssp_prio_graphs, s->e->stream, s->pool, row_bytes, &s->sd)))
         return fail(rc);
     if ((rc = engine_alloc((void **)&s->d_src, 256))) return fail(rc);
     if ((rc = engine_alloc((void **)&s->d_rows, 256))) return fail(rc);

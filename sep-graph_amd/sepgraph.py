@@ -527,7 +527,8 @@ class ServerConfig(C.Structure):
         ("bfs_packed_min", C.c_int32),
         ("sssp_packed_min", C.c_int32),
         ("mode", C.c_int32),
-        ("reserved", C.c_int32 * 5),
+        ("sssp_prio_graphs", C.c_uint32),
+        ("reserved", C.c_int32 * 4),
     ]
 
 
@@ -586,12 +587,14 @@ class Server:
     """sg_server: batched BFS / SSSP / BC-dependency queries on a resident graph."""
 
     def __init__(self, graph, memory_budget: int = 4 << 30, max_rows: int = 4096, max_batch=(64, 64, 64),
-                 bfs_packed_min: int = 0, sssp_packed_min: int = 0, mode: int = 0, **opts):
+                 bfs_packed_min: int = 0, sssp_packed_min: int = 0, mode: int = 0, sssp_prio_graphs: int = 0,
+                 **opts):
         """graph: a Graph, or a list of up to four Graphs (queries then name
         theirs by index). mode: SERVE_SLOTS (0, the serving kernel with one
         query slot per XCD) or SERVE_BATCHES (1, bit-packed batches formed
         when the GPU frees up; max_batch and the packed_min thresholds apply
-        to it; one graph only)."""
+        to it; one graph only). sssp_prio_graphs: bit i turns on the two-level
+        SSSP priority schedule for graph i (slots only)."""
         L = _server_lib()
         graphs = list(graph) if isinstance(graph, (list, tuple)) else [graph]
         self.graphs = graphs
@@ -606,6 +609,7 @@ class Server:
         cfg.bfs_packed_min = bfs_packed_min
         cfg.sssp_packed_min = sssp_packed_min
         cfg.mode = mode
+        cfg.sssp_prio_graphs = sssp_prio_graphs
         o = default_options(**opts)
         h = _P()
         arr = (_P * len(graphs))(*[gr.h for gr in graphs])

@@ -12,7 +12,7 @@ STRUCTS = {
     "sg_engine_options": ("EngineOptions", ["alpha", "block_size", "grid_blocks", "stage_hubs", "reserved"]),
     "sg_stats": ("Stats", ["kernel_ms", "residual", "status", "reserved"]),
     "sg_round_trace": ("RoundTrace", ["in_edges", "time_us"]),
-    "sg_server_config": ("ServerConfig", ["max_rows", "max_batch", "bfs_packed_min", "sssp_packed_min", "mode", "reserved"]),
+    "sg_server_config": ("ServerConfig", ["max_rows", "max_batch", "bfs_packed_min", "sssp_packed_min", "mode", "sssp_prio_graphs", "reserved"]),
     "sg_server_stats": ("ServerStats", ["device_bytes", "mean_batch", "rows", "reserved"]),
     "sg_tune_config": ("TuneConfig", ["tune_bfs", "reserved"]),
     "sg_tune_result": ("TuneResult", ["sssp_alpha", "sssp_priority", "bfs_ms", "max_batch", "batch_ms", "reserved"]),

@@ -151,14 +151,16 @@ def test_server_replay(sg, gpu, wgraph, mode):
     srv.close()
 
 
-def test_server_two_graphs(sg, gpu, wgraph):
+@pytest.mark.parametrize("prio", [0, 2])
+def test_server_two_graphs(sg, gpu, wgraph, prio):
     """One serving kernel over an RMAT graph and a grid resident together:
     queries on either graph, mixed in one arrival stream, equal lone runs on
-    their own graph."""
-    print("[server] two graphs", flush=True)
+    their own graph; with prio = 2 the grid's SSSP queries run the two-level
+    priority schedule (sssp_prio_graphs), the RMAT graph's do not."""
+    print(f"[server] two graphs, sssp_prio_graphs {prio}", flush=True)
     grid = sg.grid_graph(48, 40, p_delete=0.1, seed=3, weights=(1, 1000))
     ga, gb = sg.Graph(wgraph), sg.Graph(grid)
-    srv = sg.Server([ga, gb], max_rows=128)
+    srv = sg.Server([ga, gb], max_rows=128, sssp_prio_graphs=prio)
     rng = np.random.default_rng(12)
     sa = sg.pick_sources(wgraph.row, 80, seed=13)
     sb = sg.pick_sources(grid.row, 80, seed=14)
@@ -187,3 +189,7 @@ def test_server_two_graphs(sg, gpu, wgraph):
         assert same(sg, k, got, alone(sg, engines[gi], k, s)), (gi, k, s)
     with pytest.raises(sg.SepGraphError):
         sg.Server([ga, gb], mode=sg.SERVE_BATCHES)
+    with pytest.raises(sg.SepGraphError):
+        sg.Server([ga, gb], sssp_prio_graphs=4)  # no graph 2
+    with pytest.raises(sg.SepGraphError):
+        sg.Server(ga, mode=sg.SERVE_BATCHES, sssp_prio_graphs=1)
