@@ -120,14 +120,15 @@ def test_bc_rmat18_directed(sg, orc, gpu):
 
 def test_bc_rmat20_undirected_full_size(sg, orc, gpu):
     """Config: undirected, deduplicated RMAT-20, 256 seeded sources, 64 per
-    launch. A 16-source prefix is checked against the oracle; all 256
-    through the dependency-sum identity; batch 32 must agree with batch 64."""
+    launch. The first 64 sources (a whole launch) are checked against the
+    oracle; all 256 through the dependency-sum identity; batch 32 must agree
+    with batch 64 (one source per launch against 64: test_configs_gpu)."""
     hg = sg.rmat_graph(20, 16, symmetrize=True, seed=1)
     srcs = sg.pick_sources(hg.row, 256, seed=3)
     g = sg.Graph(hg)
     e = sg.Engine(g)
-    e.bc(srcs[:16], 64)
-    ok, err = close(e.gather(np.float32), orc.bc(hg.row, hg.col, srcs[:16]))
+    e.bc(srcs[:64], 64)
+    ok, err = close(e.gather(np.float32), orc.bc(hg.row, hg.col, srcs[:64]))
     assert ok, err
     e.bc(srcs, 64)
     x64 = e.gather(np.float32)
