@@ -540,15 +540,10 @@ __device__ __forceinline__ bool bfs_run(const BfsArgs &a, uint32_t brank, uint32
                 for (uint32_t wb = 0; wb < W; wb += nwaves * kWave) {
                     const uint32_t w = wb + (uint32_t)ln * nwaves + gwave;
                     const uint64_t fl = w < W ? R.fcur[w] : 0ull;
-                    uint64_t todo = ballot(fl != 0);
-                    while (todo) {
-                        const int t = first_lane(todo);
-                        todo &= todo - 1;
-                        const uint64_t fw = shfl(fl, t);
-                        const uint32_t u = (wb + (uint32_t)t * nwaves + gwave) * 64 + ln;
-                        if (a.lb_push_none) push_expand_serial(R, (fw >> ln) & 1, u);
-                        else push_expand(R, (fw >> ln) & 1, u);
-                    }
+                    wave_word_vertices(fl, w, [&](bool valid, uint32_t u) {
+                        if (a.lb_push_none) push_expand_serial(R, valid, u);
+                        else push_expand(R, valid, u);
+                    });
                 }
             } else {
                 // entries dealt round-robin, lane ln of wave gw taking entry

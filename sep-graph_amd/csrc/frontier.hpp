@@ -67,6 +67,53 @@ __device__ __forceinline__ int owner_lane(uint32_t incl, uint32_t j)
     return k;
 }
 
+// Position of the r-th set bit of x (r < popcount(x)), by halving.
+__device__ __forceinline__ uint32_t nth_set_bit(uint64_t x, uint32_t r)
+{
+    uint32_t pos = 0;
+    const uint32_t lo = (uint32_t)__popc((uint32_t)x);
+    if (r >= lo) {
+        r -= lo;
+        x >>= 32;
+        pos += 32;
+    }
+    uint32_t y = (uint32_t)x;
+#pragma unroll
+    for (int h = 16; h >= 2; h >>= 1) {
+        const uint32_t c = (uint32_t)__popc(y & ((1u << h) - 1u));
+        if (r >= c) {
+            r -= c;
+            y >>= h;
+            pos += (uint32_t)h;
+        }
+    }
+    if (r >= (y & 1u)) pos += 1;
+    return pos;
+}
+
+// The vertices of a wave's bitmap words, up to 64 at a time: lane ln holds
+// word fl of index wi (vertices wi * 64 + bit); f(valid, v) is called by the
+// whole wave once per group of up to 64 of them. A frontier read from a
+// bitmap is sparse after a pull round (mostly one or two vertices per word),
+// and expanding word by word had left most lanes of every step idle.
+template <class F>
+__device__ __forceinline__ void wave_word_vertices(uint64_t fl, uint32_t wi, F &&f)
+{
+    const uint32_t c = (uint32_t)__popcll(fl);
+    const uint32_t incl = wave_incl_scan(c);
+    const uint32_t total = shfl(incl, kWave - 1);
+    const int ln = lane();
+    for (uint32_t j0 = 0; j0 < total; j0 += kWave) {
+        const uint32_t j = j0 + (uint32_t)ln;
+        const bool valid = j < total;
+        const int k = owner_lane(incl, valid ? j : total - 1);
+        const uint64_t fk = shfl(fl, k);
+        const uint32_t first = shfl(incl, k) - shfl(c, k);  // this word's first position
+        const uint32_t wk = shfl(wi, k);
+        f(valid, valid ? wk * 64 + nth_set_bit(fk, j - first) : 0u);
+    }
+}
+
 // Snapshot of a counter slot, read by wave 0 of each block after a barrier.
 struct RoundCounts {
     unsigned long long count, scout;
