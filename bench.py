@@ -71,6 +71,51 @@ def cpu_baseline(hg, sources, gpu_levels, budget_s):
         "sample": f"{done} of {len(sources)} BFS sources on the same graph, oracle FIFO BFS "
                   f"(restating BFSHost), {tot_t:.1f} s",
         "bit_exact_vs_gpu": f"{exact}/{min(done, len(gpu_levels))}",
+        "host": host_info(),
+    }
+
+
+def host_info():
+    """The host CPU the baseline ran on: model name and hardware threads."""
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"cpu_model": model, "threads": os.cpu_count(), "threads_usable": len(os.sched_getaffinity(0))}
+
+
+def cpu_sssp_baseline(hg, sources, gpu_dist, budget_s):
+    """Oracle label-correcting SSSP (single thread, the reference's
+    SSSPHostNaive) over as many sources as fit in budget_s, in the same
+    GTEPS convention as the GPU figure (out-edges of reached vertices)."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle
+
+    deg = np.diff(hg.row.astype(np.int64))
+    tot_edges, tot_t, done, exact = 0, 0.0, 0, 0
+    for i, s in enumerate(sources):
+        t0 = time.perf_counter()
+        d = pyoracle.sssp_fifo(hg.row, hg.col, hg.w, int(s))
+        tot_t += time.perf_counter() - t0
+        tot_edges += int(deg[d != 0xFFFFFFFF].sum())
+        done += 1
+        if i in gpu_dist:
+            exact += int(np.array_equal(d, gpu_dist[i]))
+        if tot_t > budget_s:
+            break
+    return {
+        "value": tot_edges / tot_t / 1e9,
+        "unit": "GTEPS",
+        "cores": 1,
+        "kind": "port",
+        "sample": f"{done} of {len(sources)} SSSP sources on the same graph, oracle FIFO label-correcting "
+                  f"SSSP (restating SSSPHostNaive), {tot_t:.1f} s",
+        "bit_exact_vs_gpu": f"{exact}/{min(done, len(gpu_dist))}",
     }
 
 
@@ -347,13 +392,16 @@ def main():
     # SSSP on the same graph and sources (a subset: one query is ~20x a BFS)
     ns = min(args.sssp_sources, nq)
     sssp_m, sssp_ms = 0.0, 0.0
-    for s in sources[:ns]:
+    sssp_keep = {}
+    for i, s in enumerate(sources[:ns]):
         e.sssp(int(s))
         st = e.stats()
         if st["status"] != 0:
             raise RuntimeError(f"SSSP status {st['status']} for source {s}")
         sssp_m += st["reached_edges"]
         sssp_ms += st["kernel_ms"]
+        if i < 2 and not args.no_cpu:
+            sssp_keep[i] = e.gather(np.uint32)  # checked against the CPU baseline below
     sssp_gteps = sssp_m / (sssp_ms / 1e3) / 1e9 if sssp_ms else 0.0
     log(f"SSSP: {ns} queries, mean {sssp_ms / max(ns, 1):.2f} ms, {sssp_gteps:.2f} GTEPS")
 
@@ -367,9 +415,14 @@ def main():
         serving5 = config5(sg, args, [float(r) for r in args.config5_rates.split(",")])
 
     cpu = None
+    cpu_sssp = None
     if not args.no_cpu:
         cpu = cpu_baseline(hg, sources, keep, args.cpu_seconds)
         log(f"cpu baseline {cpu['value']:.4f} GTEPS ({cpu['sample']}), exact {cpu['bit_exact_vs_gpu']}")
+        if ns:
+            cpu_sssp = cpu_sssp_baseline(hg, sources[:ns], sssp_keep, args.cpu_seconds / 2)
+            log(f"cpu SSSP baseline {cpu_sssp['value']:.4f} GTEPS ({cpu_sssp['sample']}), "
+                f"exact {cpu_sssp['bit_exact_vs_gpu']}")
 
     out = {
         "metric": f"BFS GTEPS, RMAT-{args.scale} directed, {nq} sources (bit-exact levels)",
@@ -406,7 +459,8 @@ def main():
             "mean_launch_us": round(float(durs_ms.mean()) * 1e3, 2),
         },
         "cpu_baseline": cpu,
-        "sssp": {"gteps": round(sssp_gteps, 3), "queries": ns, "mean_ms": round(sssp_ms / max(ns, 1), 3)},
+        "sssp": {"gteps": round(sssp_gteps, 3), "queries": ns, "mean_ms": round(sssp_ms / max(ns, 1), 3),
+                 "cpu_baseline": cpu_sssp},
         "serving": serving,
         "serving_config5": serving5,
     }
