@@ -390,10 +390,18 @@ extern "C" int sg_graph_create_ex(uint32_t n, uint64_t m, const uint32_t *row_st
                 for (uint64_t v = w * 64; v < std::min<uint64_t>(n, w * 64 + 64); v++) {
                     const uint32_t deg = hcs[v + 1] - hcs[v];
                     if (deg > kPullHubDegree) continue;
+                    // a weighted vertex's in-edges by increasing weight (then
+                    // source): an SSSP pull round's test "this weight alone
+                    // cannot beat the best so far" then holds for every later
+                    // row once it holds for one
+                    uint64_t e[kPullHubDegree];
+                    for (uint32_t j = 0; j < deg; j++)
+                        e[j] = ((uint64_t)(hcw ? hcw[hcs[v] + j] : 0u) << 32) | hsrc[hcs[v] + j];
+                    if (hcw) std::sort(e, e + deg);
                     for (uint32_t j = 0; j < deg; j++) {
                         const uint64_t at = off[w] + (uint64_t)j * 64 + (v - w * 64);
-                        ssrc[at] = hsrc[hcs[v] + j];
-                        if (hcw) sw[at] = hcw[hcs[v] + j];
+                        ssrc[at] = (uint32_t)e[j];
+                        if (hcw) sw[at] = (uint32_t)(e[j] >> 32);
                     }
                 }
         });

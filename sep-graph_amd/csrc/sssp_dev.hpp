@@ -361,7 +361,10 @@ __device__ __forceinline__ void pull_word(Round &R, uint32_t w)
 // its own vertex in one coalesced load, so a lane keeps its vertex's running
 // minimum in a register, with no edge scheduling or cross-lane reduction.
 // kPullSteps rows are loaded before any is used. A slice whose vertices are
-// all settled, hubs or without in-edges is skipped whole.
+// all settled, hubs or without in-edges is skipped whole. A vertex's rows
+// hold its in-edges by increasing weight, so once the last weight read
+// cannot beat the lane's best (every frontier distance is at least fmin), no
+// later row can: the lane stops, and the wave when all its lanes have.
 __device__ __forceinline__ void pull_slice(Round &R, uint32_t w)
 {
     const SsspArgs &a = *R.a;
@@ -370,8 +373,9 @@ __device__ __forceinline__ void pull_slice(Round &R, uint32_t w)
     const uint32_t v = w * 64 + ln;
     uint32_t best = kInf;
     bool want = false;
+    uint32_t deg = 0;
     if (v < g.n) {
-        const uint32_t deg = g.cstart[v + 1] - g.cstart[v];
+        deg = g.cstart[v + 1] - g.cstart[v];
         best = a.dist[v];
         want = deg != 0 && deg <= kPullHubDegree && !settled(R, v, best);
     }
@@ -400,6 +404,8 @@ __device__ __forceinline__ void pull_slice(Round &R, uint32_t w)
 #pragma unroll
             for (int s = 0; s < U; s++)
                 if (du[s] != kInf) best = min(best, sat_add(du[s], wt[s]));
+            want = want && j0 + U < deg && sat_add(R.fmin, wt[U - 1]) < best;
+            if (!ballot(want)) break;
         }
     }
     const bool improved = v < g.n && best < dv0;
