@@ -73,7 +73,7 @@ __global__ void k_reach_stats(const uint32_t *value, const uint32_t *row, const 
 
 int dev_alloc(sg_graph *g, void **p, uint64_t bytes)
 {
-    if (g->nallocs >= 24) SG_FAIL(SG_ERR_STATE, "sg_graph_create: too many allocations");
+    if (g->nallocs >= 32) SG_FAIL(SG_ERR_STATE, "sg_graph_create: too many allocations");
     hipError_t e = hipMalloc(p, bytes ? bytes : 4);
     if (e != hipSuccess) {
         (void)hipGetLastError();
@@ -357,6 +357,36 @@ extern "C" int sg_graph_create_ex(uint32_t n, uint64_t m, const uint32_t *row_st
         }
         if (!rc) rc = upload2(in_chunks, &d.hugein, &d.nhugein);
         if (!rc) rc = upload2(in_hubs, &d.hubin, &d.nhubin);
+        const uint32_t *hcw = undirected ? weights : cw;
+        if (!rc && hcw && !in_chunks.empty() && (uint64_t)in_chunks.size() * kPullChunk < 0xFFFFFFFFull) {
+            // weighted graphs: every in-hub's in-edges by increasing weight
+            // (then source), chunk c at [c * kPullChunk, (c + 1) * kPullChunk),
+            // the hub's last chunk padded with 0xFFFFFFFF sources; and each
+            // chunk's smallest weight
+            const uint32_t *hcs = undirected ? row_start : cs;
+            const uint32_t *hsrc = undirected ? edge_dst : src;
+            const uint64_t total = (uint64_t)in_chunks.size() * kPullChunk;
+            std::vector<uint32_t> hs(total, 0xFFFFFFFFu), hw(total, 0xFFFFFFFFu), hmin(in_chunks.size());
+            sg::parallel_for_dynamic(in_hubs.size(), 64, [&](uint64_t h0, uint64_t h1) {
+                std::vector<uint64_t> e;
+                for (uint64_t h = h0; h < h1; h++) {
+                    const uint32_t c0 = in_hubs[h].x, nc = in_hubs[h].y;
+                    const uint32_t v = in_chunks[c0].x, b = hcs[v], deg = hcs[v + 1] - b;
+                    e.resize(deg);
+                    for (uint32_t j = 0; j < deg; j++) e[j] = ((uint64_t)hcw[b + j] << 32) | hsrc[b + j];
+                    std::sort(e.begin(), e.end());
+                    const uint64_t at = (uint64_t)c0 * kPullChunk;
+                    for (uint32_t j = 0; j < deg; j++) {
+                        hs[at + j] = (uint32_t)e[j];
+                        hw[at + j] = (uint32_t)(e[j] >> 32);
+                    }
+                    for (uint32_t k = 0; k < nc; k++) hmin[c0 + k] = hw[at + (uint64_t)k * kPullChunk];
+                }
+            });
+            rc = up(hs.data(), total, &d.hub_src);
+            if (!rc) rc = up(hw.data(), total, &d.hub_w);
+            if (!rc) rc = up(hmin.data(), in_chunks.size(), &d.hub_minw);
+        }
     }
     if (!rc) {
         // sliced in-edges (64 vertices per slice, in-hubs left to the chunks)

@@ -77,6 +77,14 @@ struct DevGraph {
     // number of descriptors); its chunks are contiguous there.
     const uint2 *hubin;
     uint32_t nhubin;
+    // Weighted graphs: the same in-hub in-edges by increasing weight (then
+    // source), chunk c of hugein at [c * kPullChunk, (c + 1) * kPullChunk)
+    // (0xFFFFFFFF sources past the hub's end), and each chunk's smallest
+    // weight, so an SSSP pull round skips every chunk whose lightest in-edge
+    // cannot lower its hub. Null otherwise.
+    const uint32_t *hub_src;
+    const uint32_t *hub_w;
+    const uint32_t *hub_minw;
 };
 
 // A vertex whose out-degree exceeds this is expanded through the static chunk
@@ -103,7 +111,7 @@ struct sg_graph {
     bool undirected = false;
     bool weighted = false;
     uint64_t device_bytes = 0;
-    void *allocs[24] = {};
+    void *allocs[32] = {};
     int nallocs = 0;
     int device = 0;
     int num_cus = 0;

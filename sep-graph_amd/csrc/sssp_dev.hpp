@@ -534,6 +534,86 @@ __device__ __forceinline__ void pull_hubs(Round &R, uint32_t first, uint32_t str
     finish_merges<NC>(R, pv, pm, pold);
 }
 
+// pull_hubs over the weight-ordered copy of the in-hub edges (weighted
+// graphs, DevGraph::hub_src): every frontier distance is at least fmin, so a
+// chunk whose lightest in-edge cannot lower its hub (fmin + minw >= the hub's
+// distance) is skipped without reading it -- with the edges ordered, that is
+// every chunk after the first few of a hub that already has a distance. The
+// hub's distance is read first (one round trip more per group of chunks
+// than pull_hubs); the chunk descriptors and smallest weights of the next
+// group are read while this group is worked, and merges are finished after
+// the next group's first loads are in flight, as in pull_hubs.
+__device__ __forceinline__ void pull_hubs_sorted(Round &R, uint32_t first, uint32_t stride, uint32_t nchunks)
+{
+    const SsspArgs &a = *R.a;
+    const DevGraph &g = a.g;
+    const int ln = lane();
+    constexpr int U = kPullChunk / kWave;
+    constexpr int NC = kHubChunks;
+    uint32_t nv[NC], nmw[NC];
+#pragma unroll
+    for (int i = 0; i < NC; i++) {
+        const uint32_t c = first + (uint32_t)i * stride;
+        nv[i] = c < nchunks ? g.hugein[c].x : kInf;
+        nmw[i] = c < nchunks ? g.hub_minw[c] : kInf;
+    }
+    uint32_t pv[NC], pm[NC], pold[NC];
+#pragma unroll
+    for (int i = 0; i < NC; i++) {
+        pv[i] = kInf;
+        pm[i] = kInf;
+        pold[i] = 0u;
+    }
+    for (uint32_t c0 = first; c0 < nchunks; c0 += NC * stride) {
+        uint32_t hv[NC], mw[NC], dv[NC];
+#pragma unroll
+        for (int i = 0; i < NC; i++) {
+            hv[i] = nv[i];
+            mw[i] = nmw[i];
+            const uint32_t c = c0 + (uint32_t)(NC + i) * stride;
+            nv[i] = c < nchunks ? g.hugein[c].x : kInf;
+            nmw[i] = c < nchunks ? g.hub_minw[c] : kInf;
+        }
+#pragma unroll
+        for (int i = 0; i < NC; i++) dv[i] = hv[i] != kInf ? a.dist[hv[i]] : 0u;
+        finish_merges<NC>(R, pv, pm, pold);
+        uint32_t u[NC][U], wt[NC][U], du[NC][U];
+#pragma unroll
+        for (int i = 0; i < NC; i++) {
+            const bool live = hv[i] != kInf && sat_add(R.fmin, mw[i]) < dv[i];
+            const uint64_t at = (uint64_t)(c0 + (uint32_t)i * stride) * kPullChunk + (uint32_t)ln;
+#pragma unroll
+            for (int s = 0; s < U; s++) {
+                u[i][s] = live ? g.hub_src[at + (uint64_t)s * kWave] : kInf;
+                wt[i][s] = live ? g.hub_w[at + (uint64_t)s * kWave] : 0u;
+            }
+        }
+        // an in-edge whose weight alone cannot beat the hub's distance skips
+        // its distance read (past the hub's end the source is 0xFFFFFFFF)
+#pragma unroll
+        for (int i = 0; i < NC; i++)
+#pragma unroll
+            for (int s = 0; s < U; s++) {
+                const bool f = u[i][s] != kInf && sat_add(R.fmin, wt[i][s]) < dv[i] && in_frontier(R, u[i][s]);
+                du[i][s] = f ? a.dist[u[i][s]] : kInf;
+            }
+#pragma unroll
+        for (int i = 0; i < NC; i++) {
+            uint32_t m = kInf;
+#pragma unroll
+            for (int s = 0; s < U; s++)
+                if (du[i][s] != kInf) m = min(m, sat_add(du[i][s], wt[i][s]));
+#pragma unroll
+            for (int d = kWave / 2; d > 0; d >>= 1) m = min(m, (uint32_t)__shfl_xor(m, d, kWave));
+            const bool issue = hv[i] != kInf && m < dv[i];
+            pv[i] = issue ? hv[i] : kInf;
+            pm[i] = m;
+            if (issue && ln == 0) pold[i] = atomicMin(&a.dist[hv[i]], m);
+        }
+    }
+    finish_merges<NC>(R, pv, pm, pold);
+}
+
 // `gen`: the block's barrier state (from bar_begin, or the census of a
 // grouped launch); `brank` of `nblocks` ranks the block in the query's grid.
 // Returns false when a bounded wait expired (the run then stopped). `gen`
@@ -707,7 +787,10 @@ __device__ __forceinline__ bool sssp_run(const SsspArgs &a, uint32_t brank, uint
             // over all waves, kHubChunks chunks per wave at a time; (2) every
             // other vertex with in-edges, a word at a time.
             const uint32_t nhugein = a.lb_pull_none ? 0u : g.nhugein;
-            if (gwave < nhugein) pull_hubs(R, gwave, nwaves, nhugein);
+            if (gwave < nhugein) {
+                if (g.hub_src) pull_hubs_sorted(R, gwave, nwaves, nhugein);
+                else pull_hubs(R, gwave, nwaves, nhugein);
+            }
             // words interleaved over waves: heavy vertices cluster at low ids
             // (RMAT, degree-ordered graphs), so contiguous ranges would leave a
             // few waves with most of the in-edges
