@@ -383,9 +383,26 @@ extern "C" int sg_graph_create_ex(uint32_t n, uint64_t m, const uint32_t *row_st
                     for (uint32_t k = 0; k < nc; k++) hmin[c0 + k] = hw[at + (uint64_t)k * kPullChunk];
                 }
             });
+            // the order an SSSP pull round visits them in: every hub's first
+            // (lightest) chunk, then the others, each with its smallest weight
+            std::vector<uint2> ord;
+            std::vector<uint32_t> omin;
+            ord.reserve(in_chunks.size());
+            omin.reserve(in_chunks.size());
+            for (const uint2 &h : in_hubs) {
+                ord.push_back(make_uint2(in_chunks[h.x].x, h.x));
+                omin.push_back(hmin[h.x]);
+            }
+            for (const uint2 &h : in_hubs)
+                for (uint32_t k = 1; k < h.y; k++) {
+                    ord.push_back(make_uint2(in_chunks[h.x + k].x, h.x + k));
+                    omin.push_back(hmin[h.x + k]);
+                }
+            uint32_t nord = 0;
             rc = up(hs.data(), total, &d.hub_src);
             if (!rc) rc = up(hw.data(), total, &d.hub_w);
-            if (!rc) rc = up(hmin.data(), in_chunks.size(), &d.hub_minw);
+            if (!rc) rc = up(omin.data(), omin.size(), &d.hub_minw);
+            if (!rc) rc = upload2(ord, &d.hub_ord, &nord);
         }
     }
     if (!rc) {

@@ -79,11 +79,14 @@ struct DevGraph {
     uint32_t nhubin;
     // Weighted graphs: the same in-hub in-edges by increasing weight (then
     // source), chunk c of hugein at [c * kPullChunk, (c + 1) * kPullChunk)
-    // (0xFFFFFFFF sources past the hub's end), and each chunk's smallest
-    // weight, so an SSSP pull round skips every chunk whose lightest in-edge
-    // cannot lower its hub. Null otherwise.
+    // (0xFFFFFFFF sources past the hub's end); the chunks as (hub, chunk) in
+    // the order an SSSP pull round visits them -- every hub's first chunk
+    // (nhubin of them), then the others -- with each one's smallest weight
+    // beside it, so a round skips every chunk whose lightest in-edge cannot
+    // lower its hub. Null otherwise.
     const uint32_t *hub_src;
     const uint32_t *hub_w;
+    const uint2 *hub_ord;
     const uint32_t *hub_minw;
 };
 

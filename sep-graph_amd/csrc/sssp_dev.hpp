@@ -545,16 +545,18 @@ __device__ __forceinline__ void pull_hubs(Round &R, uint32_t first, uint32_t str
 // the next group's first loads are in flight, as in pull_hubs.
 __device__ __forceinline__ void pull_hubs_sorted(Round &R, uint32_t first, uint32_t stride, uint32_t nchunks)
 {
+    // positions first, first + stride, ... < nchunks of DevGraph::hub_ord
     const SsspArgs &a = *R.a;
     const DevGraph &g = a.g;
     const int ln = lane();
     constexpr int U = kPullChunk / kWave;
     constexpr int NC = kHubChunks;
-    uint32_t nv[NC], nmw[NC];
+    uint2 nv[NC];
+    uint32_t nmw[NC];
 #pragma unroll
     for (int i = 0; i < NC; i++) {
         const uint32_t c = first + (uint32_t)i * stride;
-        nv[i] = c < nchunks ? g.hugein[c].x : kInf;
+        nv[i] = c < nchunks ? g.hub_ord[c] : make_uint2(kInf, 0u);
         nmw[i] = c < nchunks ? g.hub_minw[c] : kInf;
     }
     uint32_t pv[NC], pm[NC], pold[NC];
@@ -565,13 +567,14 @@ __device__ __forceinline__ void pull_hubs_sorted(Round &R, uint32_t first, uint3
         pold[i] = 0u;
     }
     for (uint32_t c0 = first; c0 < nchunks; c0 += NC * stride) {
-        uint32_t hv[NC], mw[NC], dv[NC];
+        uint32_t hv[NC], hc[NC], mw[NC], dv[NC];
 #pragma unroll
         for (int i = 0; i < NC; i++) {
-            hv[i] = nv[i];
+            hv[i] = nv[i].x;
+            hc[i] = nv[i].y;
             mw[i] = nmw[i];
             const uint32_t c = c0 + (uint32_t)(NC + i) * stride;
-            nv[i] = c < nchunks ? g.hugein[c].x : kInf;
+            nv[i] = c < nchunks ? g.hub_ord[c] : make_uint2(kInf, 0u);
             nmw[i] = c < nchunks ? g.hub_minw[c] : kInf;
         }
 #pragma unroll
@@ -581,7 +584,7 @@ __device__ __forceinline__ void pull_hubs_sorted(Round &R, uint32_t first, uint3
 #pragma unroll
         for (int i = 0; i < NC; i++) {
             const bool live = hv[i] != kInf && sat_add(R.fmin, mw[i]) < dv[i];
-            const uint64_t at = (uint64_t)(c0 + (uint32_t)i * stride) * kPullChunk + (uint32_t)ln;
+            const uint64_t at = (uint64_t)hc[i] * kPullChunk + (uint32_t)ln;
 #pragma unroll
             for (int s = 0; s < U; s++) {
                 u[i][s] = live ? g.hub_src[at + (uint64_t)s * kWave] : kInf;
@@ -787,9 +790,16 @@ __device__ __forceinline__ bool sssp_run(const SsspArgs &a, uint32_t brank, uint
             // over all waves, kHubChunks chunks per wave at a time; (2) every
             // other vertex with in-edges, a word at a time.
             const uint32_t nhugein = a.lb_pull_none ? 0u : g.nhugein;
-            if (gwave < nhugein) {
-                if (g.hub_src) pull_hubs_sorted(R, gwave, nwaves, nhugein);
-                else pull_hubs(R, gwave, nwaves, nhugein);
+            if (nhugein && g.hub_src) {
+                // every hub's first (lightest) chunk before the others: a hub
+                // lowered by it lets its heavier chunks be skipped even in the
+                // first pull round, when every hub starts unreached (any
+                // distance read is an upper bound, so the order is only a
+                // matter of how much is skipped)
+                if (gwave < g.nhubin) pull_hubs_sorted(R, gwave, nwaves, g.nhubin);
+                if (g.nhubin + gwave < nhugein) pull_hubs_sorted(R, g.nhubin + gwave, nwaves, nhugein);
+            } else if (gwave < nhugein) {
+                pull_hubs(R, gwave, nwaves, nhugein);
             }
             // words interleaved over waves: heavy vertices cluster at low ids
             // (RMAT, degree-ordered graphs), so contiguous ranges would leave a
