@@ -296,9 +296,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--sssp-sources", type=int, default=16)
     ap.add_argument("--no-serving", action="store_true")
-    ap.add_argument("--serve-rates", type=str, default="500,1000,1500,2000,3000,5000",
+    ap.add_argument("--serve-rates", type=str, default="500,1000,1500,2000,2250,2500,2750,3000,5000",
                     help="Poisson arrival rates (queries/s) for the RMAT serving run (stops once saturated)")
-    ap.add_argument("--config5-rates", type=str, default="50,80,120,1000",
+    ap.add_argument("--config5-rates", type=str, default="50,80,120,160,200,1000",
                     help="Poisson arrival rates (queries/s) for the RMAT-24 + grid serving run (stops once "
                          "saturated; its configuration names 1k-50k q/s, above what one GPU serves)")
     ap.add_argument("--serve-count", type=int, default=2000, help="queries per arrival rate")
