@@ -16,7 +16,8 @@
 //   2. level lists: a counting sort of the reached vertices by level (block
 //      histograms in LDS, one atomic per block and level, positions handed out
 //      per wave and level), so each later phase visits a level's vertices
-//      without scanning all n;
+//      without scanning all n; the same pass sums the out- and in-degrees of
+//      each of the first kBcVolLevels levels (edge volumes);
 //   3. sigma, level by level: every vertex gathers sigma(u) over in-neighbours
 //      one level up (pull over the CSC, wave prefix-sum scheduling), testing
 //      "u is one level up" in a bitmap of that level (n/8 bytes, L2-resident)
@@ -26,14 +27,22 @@
 //      with in-degree above kPullHubDegree are summed per 256-edge chunk of
 //      the graph's static in-hub list and merged with an FP64 atomicAdd --
 //      path counts are integers below 2^53, so the order of the additions
-//      cannot change them;
+//      cannot change them; a level holding a sixteenth of the vertices or
+//      more is pulled through the sliced in-edges (a lane per vertex), and a
+//      level whose upper neighbour has at most a quarter of its in-edges as
+//      out-edges is pushed instead: sigma(u) added along u's out-edges into
+//      the level with FP64 atomics (the BFS direction rule applied to the
+//      sigma pass; the reference's stage 1 pushes or pulls with its BFS);
 //   4. dependencies, level by level from the deepest (the same rotating
 //      level bitmaps): acc(u) = sum over
 //      successors w one level down of psi(w) = 1/sigma(w) + acc(w), which is
 //      (1 + delta(w)) / sigma(w) with delta(w) = sigma(w) acc(w); sigma and
 //      acc sit side by side (one 16-byte load per successor); vertices with
 //      out-degree above kHugeDegree are summed per 64-edge chunk of the
-//      static out-hub list with an FP64 atomicAdd;
+//      static out-hub list with an FP64 atomicAdd; a level whose lower
+//      neighbour has at most a quarter of its out-edges as in-edges is pushed
+//      instead, psi(w) added along w's in-edges into the level (the
+//      reference's stage 2 always pushes, hybrid_bc.cu:134-225);
 //   5. 0.5 delta(v) = 0.5 sigma(v) acc(v) written out (0 for the source and
 //      for unreached vertices, as BetweennessCentralityHost,
 //      hybrid_bc_host.cpp:16-92).
@@ -51,6 +60,13 @@ using namespace bfsk;
 // counts in LDS (deeper ones use global atomics).
 constexpr uint32_t kBcMaxDepth = 1u << 16;
 constexpr uint32_t kBcLdsLevels = 2048;
+// Levels whose edge volumes (sum of out- and of in-degrees) are counted; the
+// sigma and dependency passes of these levels pick pull or push by them,
+// deeper levels always pull.
+constexpr uint32_t kBcVolLevels = 64;
+// A pass pushes when the other side's edges number at most 1/kBcPushRatio of
+// this side's (a push adds one FP64 atomic per edge inside the DAG).
+constexpr uint32_t kBcPushRatio = 4;
 
 struct Bc1Args {
     BfsArgs f;          // forward BFS: f.level (device order, f.perm null), f.stats
@@ -60,6 +76,7 @@ struct Bc1Args {
     uint32_t *loff;     // [kBcMaxDepth + 2]: first position of each level in order
     uint32_t *lfill;    // [kBcMaxDepth + 1]: positions handed out per level
     uint32_t *ctl;      // [0]: deepest level reached, [1]: a level beyond kBcMaxDepth
+    uint32_t *lvol;     // [2 * kBcVolLevels]: out-edge, then in-edge volume per level
     uint64_t *lbm;      // 3 x words: bitmaps of three consecutive levels (level j in slot j % 3)
     // outputs (any may be null):
     float *dep;         // 0.5 delta(v) at dep[perm ? perm[v] : v]
@@ -150,6 +167,66 @@ __device__ __forceinline__ double wave_level_sum(const uint32_t *adj, const uint
     return acc;
 }
 
+// For every edge [b, e) of each lane's vertex (edge ids into `adj`), in the
+// scheduling of wave_level_sum: when `to(neighbour)` holds, atomically adds the
+// owning lane's x into dst[neighbour] (FP64). The push form of a level pass,
+// for a level whose neighbouring level has far fewer edges.
+template <typename To>
+__device__ __forceinline__ void wave_level_push(const uint32_t *adj, uint32_t b, uint32_t e, double x, double *dst,
+                                                size_t stride, To to)
+{
+    constexpr int U = kLevelSteps;
+    const int ln = lane();
+    const uint32_t r = e - b;
+    const uint32_t incl = wave_incl_scan(r);
+    const uint32_t total = shfl(incl, kWave - 1);
+    const uint32_t excl = incl - r;
+    for (uint32_t jb = 0; jb < total; jb += U * kWave) {
+        uint32_t u[U];
+        double xs[U];
+        bool act[U];
+#pragma unroll
+        for (int s = 0; s < U; s++) {
+            const uint32_t j = jb + (uint32_t)(s * kWave + ln);
+            act[s] = j < total;
+            const int k = owner_lane(incl, act[s] ? j : 0u);
+            const uint32_t ed = shfl(b, k) + (j - shfl(excl, k));
+            xs[s] = __shfl(x, k, kWave);
+            u[s] = act[s] ? adj[ed] : 0u;
+        }
+        bool on[U];
+#pragma unroll
+        for (int s = 0; s < U; s++) on[s] = act[s] && to(u[s]);
+#pragma unroll
+        for (int s = 0; s < U; s++)
+            if (on[s]) atomicAdd(dst + (size_t)u[s] * stride, xs[s]);
+    }
+}
+
+// Out-edge and in-edge volumes of the wave's `valid` lanes' levels (below
+// kBcVolLevels) added into vol[level] and vol[kBcVolLevels + level] (LDS), one
+// add per distinct level. Wave-convergent.
+__device__ __forceinline__ void wave_level_vol(uint32_t *vol, bool valid, uint32_t l, uint32_t od, uint32_t id)
+{
+    uint64_t todo = ballot(valid);
+    while (todo) {
+        const uint32_t l0 = shfl(l, first_lane(todo));
+        const bool in = valid && l == l0;
+        const uint64_t m = ballot(in);
+        todo &= ~m;
+        uint32_t so = in ? od : 0u, si = in ? id : 0u;
+#pragma unroll
+        for (int d = kWave / 2; d > 0; d >>= 1) {
+            so += (uint32_t)__shfl_xor(so, d, kWave);
+            si += (uint32_t)__shfl_xor(si, d, kWave);
+        }
+        if (lane() == first_lane(m)) {
+            atomicAdd(&vol[l0], so);
+            atomicAdd(&vol[kBcVolLevels + l0], si);
+        }
+    }
+}
+
 template <int BS>
 __device__ __forceinline__ bool bc1_run(const Bc1Args &a, uint32_t brank, uint32_t nblocks, uint32_t xcc,
                                         BarState &gen)
@@ -166,12 +243,14 @@ __device__ __forceinline__ bool bc1_run(const Bc1Args &a, uint32_t brank, uint32
 
     __shared__ uint32_t s_hist[kBcLdsLevels];
     __shared__ uint32_t s_max, s_scan[BS / kWave], s_D;
+    __shared__ uint32_t s_vol[2 * kBcVolLevels];
 
     // level counters start at zero (the BFS's first barrier publishes this)
     for (uint32_t i = gtid; i <= kBcMaxDepth; i += nthreads) {
         a.lcnt[i] = 0;
         a.lfill[i] = 0;
     }
+    for (uint32_t i = gtid; i < 2 * kBcVolLevels; i += nthreads) a.lvol[i] = 0;
     if (gtid == 0) {
         a.ctl[0] = 0;
         a.ctl[1] = 0;
@@ -182,6 +261,7 @@ __device__ __forceinline__ bool bc1_run(const Bc1Args &a, uint32_t brank, uint32
     // ---- per-level counts: block histograms, one atomic per block and level
     if (ok) {
         for (uint32_t i = threadIdx.x; i < kBcLdsLevels; i += BS) s_hist[i] = 0;
+        for (uint32_t i = threadIdx.x; i < 2 * kBcVolLevels; i += BS) s_vol[i] = 0;
         if (threadIdx.x == 0) s_max = 0;
         __syncthreads();
         uint32_t lmax = 0;
@@ -192,6 +272,14 @@ __device__ __forceinline__ bool bc1_run(const Bc1Args &a, uint32_t brank, uint32
             if (valid) lmax = max(lmax, (uint32_t)l);
             if (l >= 0 && !valid) a.ctl[1] = 1u;
             wave_level_add(s_hist, a.lcnt, valid, valid ? (uint32_t)l : 0u, false);
+            // edge volumes of the shallow levels (a level's edges number
+            // below 2^32: edge offsets are 32-bit)
+            const bool vv = valid && (uint32_t)l < kBcVolLevels;
+            if (ballot(vv)) {
+                const uint32_t od = vv ? g.row[v + 1] - g.row[v] : 0u;
+                const uint32_t id = vv ? g.cstart[v + 1] - g.cstart[v] : 0u;
+                wave_level_vol(s_vol, vv, vv ? (uint32_t)l : 0u, od, id);
+            }
         }
 #pragma unroll
         for (int d = kWave / 2; d > 0; d >>= 1) lmax = max(lmax, (uint32_t)__shfl_xor(lmax, d, kWave));
@@ -200,6 +288,8 @@ __device__ __forceinline__ bool bc1_run(const Bc1Args &a, uint32_t brank, uint32
         const uint32_t top = min(s_max, kBcLdsLevels - 1);
         for (uint32_t l = threadIdx.x; l <= top; l += BS)
             if (s_hist[l]) atomicAdd(&a.lcnt[l], s_hist[l]);
+        for (uint32_t i = threadIdx.x; i < 2 * kBcVolLevels; i += BS)
+            if (s_vol[i]) atomicAdd(&a.lvol[i], s_vol[i]);
         if (threadIdx.x == 0) atomicMax(&a.ctl[0], s_max);
         ok = grid_sync(a.f.bar, xcc, gen, a.f.abort);
     }
@@ -282,42 +372,110 @@ __device__ __forceinline__ bool bc1_run(const Bc1Args &a, uint32_t brank, uint32
         uint64_t *old = a.lbm + (size_t)((L + 1) % 3) * W;       // level L-2, cleared now
         if (L >= 2)
             for (uint32_t i = a.loff[L - 2] + gtid; i < a.loff[L - 1]; i += nthreads) old[a.order[i] >> 6] = 0ull;
-        for (uint32_t i0 = b0 + gwave * kWave; i0 < e0; i0 += nwaves * kWave) {
-            const uint32_t i = i0 + (uint32_t)ln;
-            uint32_t v = 0, b = 0, e = 0;
-            if (i < e0) {
-                v = a.order[i];
-                b = g.cstart[v];
-                e = g.cstart[v + 1];
-                if (e - b > kPullHubDegree) e = b;  // in-hubs: the chunk pass below
+        // push when level L-1's out-edges are far fewer than level L's in-edges
+        const bool push = L < kBcVolLevels && (uint64_t)a.lvol[L - 1] * kBcPushRatio <= a.lvol[kBcVolLevels + L];
+        if (push) {
+            auto at_level = [&](uint32_t w) { return lvl[w] == (int32_t)L; };
+            for (uint32_t i0 = b0 + gwave * kWave; i0 < e0; i0 += nwaves * kWave) {
+                const uint32_t i = i0 + (uint32_t)ln;
+                wave_set_bits(cur, i < e0, i < e0 ? a.order[i] : 0u);
             }
-            wave_set_bits(cur, i < e0, v);
-            const double s = wave_level_sum(g.csrc, up, b, e, [&](uint32_t u) { return a.sa[u].x; });
-            if (i < e0 && e > b) a.sa[v].x = s;
-        }
-        // in-hub chunks of this level, kPullChunk in-edges each
-        for (uint32_t c0 = 0; c0 < g.nhugein; c0 += nwaves * kWave) {
-            const uint32_t c = c0 + (uint32_t)ln * nwaves + gwave;
-            uint2 dsc = make_uint2(0u, 0u);
-            bool mine = false;
-            if (c < g.nhugein) {
-                dsc = g.hugein[c];
-                mine = lvl[dsc.x] == (int32_t)L;
-            }
-            uint64_t mm = ballot(mine);
-            while (mm) {
-                const int t = first_lane(mm);
-                mm &= mm - 1;
-                const uint32_t v = shfl(dsc.x, t), beg = shfl(dsc.y, t);
-                const uint32_t end = min(g.cstart[v + 1], beg + kPullChunk);
+            // sigma(u) of level L-1 along its out-edges into level L
+            const uint32_t p0 = a.loff[L - 1];
+            for (uint32_t i0 = p0 + gwave * kWave; i0 < b0; i0 += nwaves * kWave) {
+                const uint32_t i = i0 + (uint32_t)ln;
+                uint32_t u = 0, b = 0, e = 0;
                 double x = 0.0;
-                for (uint32_t e = beg + (uint32_t)ln; e < end; e += kWave) {
-                    const uint32_t u = g.csrc[e];
-                    if (test_bit(up, u)) x += a.sa[u].x;
+                if (i < b0) {
+                    u = a.order[i];
+                    b = g.row[u];
+                    e = g.row[u + 1];
+                    if (e - b > kHugeDegree) e = b;  // out-hubs: the chunks below
+                    x = a.sa[u].x;
                 }
-                x = wave_sum_f64(x);
-                if (ln == 0 && x != 0.0) atomicAdd(&a.sa[v].x, x);
+                wave_level_push(g.col, b, e, x, &a.sa[0].x, 2, at_level);
             }
+            for (uint32_t c0 = 0; c0 < g.nhuge; c0 += nwaves * kWave) {
+                const uint32_t c = c0 + (uint32_t)ln * nwaves + gwave;
+                uint32_t u = 0, b = 0, e = 0;
+                if (c < g.nhuge) {
+                    const uint2 dsc = g.huge[c];
+                    if (lvl[dsc.x] == (int32_t)(L - 1)) {
+                        u = dsc.x;
+                        b = dsc.y;
+                        e = min(g.row[u + 1], b + kHugeChunk);
+                    }
+                }
+                if (!ballot(e > b)) continue;
+                const double x = e > b ? a.sa[u].x : 0.0;
+                wave_level_push(g.col, b, e, x, &a.sa[0].x, 2, at_level);
+            }
+        } else if (g.sell_src && (uint64_t)(e0 - b0) * 16 >= n) {
+            // a level holding a sixteenth of the vertices or more: by the
+            // 64-vertex slices of the sliced in-edges, one lane per vertex and
+            // one coalesced row per in-edge index, instead of prefix-sum
+            // scheduling over the level's list (path counts are integers below
+            // 2^53, so the order of the additions cannot change them)
+            for (uint32_t w = gwave; w < W; w += nwaves) {
+                const uint32_t v = w * 64 + (uint32_t)ln;
+                const bool at = v < n && lvl[v] == (int32_t)L;
+                // one wave owns word w of this level's bitmap, clear until now
+                const uint64_t m = ballot(at);
+                if (!m) continue;
+                if (ln == 0) cur[w] = m;
+                const uint32_t deg = at ? g.cstart[v + 1] - g.cstart[v] : 0u;
+                const bool light = at && deg <= kPullHubDegree;  // in-hubs: the chunk pass below
+                uint32_t rows = light ? deg : 0u;
+#pragma unroll
+                for (int d = kWave / 2; d > 0; d >>= 1) rows = max(rows, (uint32_t)__shfl_xor(rows, d, kWave));
+                if (!rows) continue;
+                const uint32_t *S = g.sell_src + g.sell_off[w] + ln;
+                constexpr int U = kLevelSteps;
+                double acc = 0.0;
+                for (uint32_t j0 = 0; j0 < rows; j0 += U) {
+                    uint32_t u[U];
+#pragma unroll
+                    for (int s = 0; s < U; s++) u[s] = light && j0 + s < deg ? S[(size_t)(j0 + s) * kWave] : 0xFFFFFFFFu;
+                    double x[U];
+#pragma unroll
+                    for (int s = 0; s < U; s++) x[s] = u[s] != 0xFFFFFFFFu && test_bit(up, u[s]) ? a.sa[u[s]].x : 0.0;
+#pragma unroll
+                    for (int s = 0; s < U; s++) acc += x[s];
+                }
+                if (light) a.sa[v].x = acc;
+            }
+        } else {
+            for (uint32_t i0 = b0 + gwave * kWave; i0 < e0; i0 += nwaves * kWave) {
+                const uint32_t i = i0 + (uint32_t)ln;
+                uint32_t v = 0, b = 0, e = 0;
+                if (i < e0) {
+                    v = a.order[i];
+                    b = g.cstart[v];
+                    e = g.cstart[v + 1];
+                    if (e - b > kPullHubDegree) e = b;  // in-hubs: the chunk pass below
+                }
+                wave_set_bits(cur, i < e0, v);
+                const double s = wave_level_sum(g.csrc, up, b, e, [&](uint32_t u) { return a.sa[u].x; });
+                if (i < e0 && e > b) a.sa[v].x = s;
+            }
+        }
+        // in-hub chunks of this level, kPullChunk in-edges each: a lane per
+        // chunk, the wave's chunks walked together like a level list (every
+        // load of a step in flight at once), one atomic per chunk
+        for (uint32_t c0 = 0; !push && c0 < g.nhugein; c0 += nwaves * kWave) {
+            const uint32_t c = c0 + (uint32_t)ln * nwaves + gwave;
+            uint32_t v = 0, b = 0, e = 0;
+            if (c < g.nhugein) {
+                const uint2 dsc = g.hugein[c];
+                if (lvl[dsc.x] == (int32_t)L) {
+                    v = dsc.x;
+                    b = dsc.y;
+                    e = min(g.cstart[v + 1], b + kPullChunk);
+                }
+            }
+            if (!ballot(e > b)) continue;
+            const double x = wave_level_sum(g.csrc, up, b, e, [&](uint32_t u) { return a.sa[u].x; });
+            if (e > b && x != 0.0) atomicAdd(&a.sa[v].x, x);
         }
         ok = grid_sync(a.f.bar, xcc, gen, a.f.abort);
     }
@@ -334,6 +492,48 @@ __device__ __forceinline__ bool bc1_run(const Bc1Args &a, uint32_t brank, uint32
             const double2 x = a.sa[w];
             return 1.0 / x.x + x.y;
         };
+        // push when level L+1's in-edges are far fewer than level L's out-edges
+        const bool push =
+            L + 1 < kBcVolLevels && (uint64_t)a.lvol[kBcVolLevels + L + 1] * kBcPushRatio <= a.lvol[L];
+        if (push) {
+            auto at_level = [&](uint32_t u) { return lvl[u] == (int32_t)L; };
+            for (uint32_t i0 = b0 + gwave * kWave; i0 < e0; i0 += nwaves * kWave) {
+                const uint32_t i = i0 + (uint32_t)ln;
+                wave_set_bits(cur, i < e0, i < e0 ? a.order[i] : 0u);
+            }
+            // psi(w) of level L+1 along its in-edges into level L
+            const uint32_t p1 = a.loff[L + 2];
+            for (uint32_t i0 = e0 + gwave * kWave; i0 < p1; i0 += nwaves * kWave) {
+                const uint32_t i = i0 + (uint32_t)ln;
+                uint32_t w = 0, b = 0, e = 0;
+                double x = 0.0;
+                if (i < p1) {
+                    w = a.order[i];
+                    b = g.cstart[w];
+                    e = g.cstart[w + 1];
+                    if (e - b > kPullHubDegree) e = b;  // in-hubs: the chunks below
+                    x = psi(w);
+                }
+                wave_level_push(g.csrc, b, e, x, &a.sa[0].y, 2, at_level);
+            }
+            for (uint32_t c0 = 0; c0 < g.nhugein; c0 += nwaves * kWave) {
+                const uint32_t c = c0 + (uint32_t)ln * nwaves + gwave;
+                uint32_t w = 0, b = 0, e = 0;
+                if (c < g.nhugein) {
+                    const uint2 dsc = g.hugein[c];
+                    if (lvl[dsc.x] == (int32_t)(L + 1)) {
+                        w = dsc.x;
+                        b = dsc.y;
+                        e = min(g.cstart[w + 1], b + kPullChunk);
+                    }
+                }
+                if (!ballot(e > b)) continue;
+                const double x = e > b ? psi(w) : 0.0;
+                wave_level_push(g.csrc, b, e, x, &a.sa[0].y, 2, at_level);
+            }
+            ok = grid_sync(a.f.bar, xcc, gen, a.f.abort);
+            continue;
+        }
         for (uint32_t i0 = b0 + gwave * kWave; i0 < e0; i0 += nwaves * kWave) {
             const uint32_t i = i0 + (uint32_t)ln;
             uint32_t u = 0, b = 0, e = 0;
@@ -347,28 +547,22 @@ __device__ __forceinline__ bool bc1_run(const Bc1Args &a, uint32_t brank, uint32
             const double s = wave_level_sum(g.col, down, b, e, psi);
             if (i < e0 && e > b) a.sa[u].y = s;
         }
+        // out-hub chunks of this level, kHugeChunk out-edges each, walked
+        // like the in-hub chunks of the sigma pass
         for (uint32_t c0 = 0; c0 < g.nhuge; c0 += nwaves * kWave) {
             const uint32_t c = c0 + (uint32_t)ln * nwaves + gwave;
-            uint2 dsc = make_uint2(0u, 0u);
-            bool mine = false;
+            uint32_t u = 0, b = 0, e = 0;
             if (c < g.nhuge) {
-                dsc = g.huge[c];
-                mine = lvl[dsc.x] == (int32_t)L;
-            }
-            uint64_t mm = ballot(mine);
-            while (mm) {
-                const int t = first_lane(mm);
-                mm &= mm - 1;
-                const uint32_t u = shfl(dsc.x, t), beg = shfl(dsc.y, t);
-                const uint32_t end = min(g.row[u + 1], beg + kHugeChunk);
-                double x = 0.0;
-                for (uint32_t e = beg + (uint32_t)ln; e < end; e += kWave) {
-                    const uint32_t w = g.col[e];
-                    if (test_bit(down, w)) x += psi(w);
+                const uint2 dsc = g.huge[c];
+                if (lvl[dsc.x] == (int32_t)L) {
+                    u = dsc.x;
+                    b = dsc.y;
+                    e = min(g.row[u + 1], b + kHugeChunk);
                 }
-                x = wave_sum_f64(x);
-                if (ln == 0 && x != 0.0) atomicAdd(&a.sa[u].y, x);
             }
+            if (!ballot(e > b)) continue;
+            const double x = wave_level_sum(g.col, down, b, e, psi);
+            if (e > b && x != 0.0) atomicAdd(&a.sa[u].y, x);
         }
         ok = grid_sync(a.f.bar, xcc, gen, a.f.abort);
     }
@@ -410,6 +604,7 @@ inline Bc1Layout bc1_layout(uint64_t n)
     put(L.loff, (uint64_t)(kBcMaxDepth + 2) * 4);
     put(L.lfill, (uint64_t)(kBcMaxDepth + 2) * 4);
     put(L.ctl, 256);
+    put(L.lvol, 2 * kBcVolLevels * 4);
     put(L.lbm, 3 * ((n + 63) / 64) * 8);
     put(L.fstats, sizeof(DevStats));
     put(L.bcsum, n * 8);
@@ -448,6 +643,7 @@ inline int bc1_fill(const sg_engine *e, char *base, const Bc1Layout &L, uint32_t
     b.loff = (uint32_t *)(base + L.loff);
     b.lfill = (uint32_t *)(base + L.lfill);
     b.ctl = (uint32_t *)(base + L.ctl);
+    b.lvol = (uint32_t *)(base + L.lvol);
     b.lbm = (uint64_t *)(base + L.lbm);
     return SG_OK;
 }

@@ -224,6 +224,7 @@ __device__ __noinline__ bool serve_bc(const ServeArgs &a, char *base, const Mail
     c.loff = (uint32_t *)(bb + a.L.bc1.loff);
     c.lfill = (uint32_t *)(bb + a.L.bc1.lfill);
     c.ctl = (uint32_t *)(bb + a.L.bc1.ctl);
+    c.lvol = (uint32_t *)(bb + a.L.bc1.lvol);
     c.lbm = (uint64_t *)(bb + a.L.bc1.lbm);
     c.dep = (float *)(a.pool + (size_t)m.row * a.row_bytes);
     c.perm = a.perm[m.graph];

@@ -257,7 +257,7 @@ int ms_launch(sg_engine *e, int kind, const uint32_t *dev_src, uint32_t nsrc, vo
 
 // Working arena of one single-source BC run (byte offsets).
 struct Bc1Layout {
-    uint64_t sa, lvl, order, lcnt, loff, lfill, ctl, lbm, fstats, bcsum, total;
+    uint64_t sa, lvl, order, lcnt, loff, lfill, ctl, lvol, lbm, fstats, bcsum, total;
 };
 
 // Single-source BC from device vertex `dev_source` over the whole grid
