@@ -42,7 +42,8 @@
 //      static out-hub list with an FP64 atomicAdd; a level whose lower
 //      neighbour has at most a quarter of its out-edges as in-edges is pushed
 //      instead, psi(w) added along w's in-edges into the level (the
-//      reference's stage 2 always pushes, hybrid_bc.cu:134-225);
+//      reference's stage 2 always gathers over each vertex's out-edges,
+//      hybrid_bc.cu:155-167);
 //   5. 0.5 delta(v) = 0.5 sigma(v) acc(v) written out (0 for the source and
 //      for unreached vertices, as BetweennessCentralityHost,
 //      hybrid_bc_host.cpp:16-92).
