@@ -29,8 +29,8 @@
 //      path counts are integers below 2^53, so the order of the additions
 //      cannot change them; a level holding a sixteenth of the vertices or
 //      more is pulled through the sliced in-edges (a lane per vertex), and a
-//      level whose upper neighbour has at most a quarter of its in-edges as
-//      out-edges is pushed instead: sigma(u) added along u's out-edges into
+//      level whose upper neighbour has at most 1/kBcPushRatio of its in-edges
+//      as out-edges is pushed instead: sigma(u) added along u's out-edges into
 //      the level with FP64 atomics (the BFS direction rule applied to the
 //      sigma pass; the reference's stage 1 pushes or pulls with its BFS);
 //   4. dependencies, level by level from the deepest (the same rotating
@@ -40,7 +40,7 @@
 //      acc sit side by side (one 16-byte load per successor); vertices with
 //      out-degree above kHugeDegree are summed per 64-edge chunk of the
 //      static out-hub list with an FP64 atomicAdd; a level whose lower
-//      neighbour has at most a quarter of its out-edges as in-edges is pushed
+//      neighbour has at most 1/kBcPushRatio of its out-edges as in-edges is pushed
 //      instead, psi(w) added along w's in-edges into the level (the
 //      reference's stage 2 always gathers over each vertex's out-edges,
 //      hybrid_bc.cu:155-167);
@@ -66,8 +66,9 @@ constexpr uint32_t kBcLdsLevels = 2048;
 // deeper levels always pull.
 constexpr uint32_t kBcVolLevels = 64;
 // A pass pushes when the other side's edges number at most 1/kBcPushRatio of
-// this side's (a push adds one FP64 atomic per edge inside the DAG).
-constexpr uint32_t kBcPushRatio = 4;
+// this side's: a push adds one device-scope FP64 atomic per edge inside the
+// DAG, which costs about as much as scanning a dozen edges in a pull.
+constexpr uint32_t kBcPushRatio = 16;
 
 struct Bc1Args {
     BfsArgs f;          // forward BFS: f.level (device order, f.perm null), f.stats
@@ -77,7 +78,7 @@ struct Bc1Args {
     uint32_t *loff;     // [kBcMaxDepth + 2]: first position of each level in order
     uint32_t *lfill;    // [kBcMaxDepth + 1]: positions handed out per level
     uint32_t *ctl;      // [0]: deepest level reached, [1]: a level beyond kBcMaxDepth
-    uint32_t *lvol;     // [2 * kBcVolLevels]: out-edge, then in-edge volume per level
+    uint32_t *lvol;     // [4 * kBcVolLevels]: out-edge volume, in-edge volume, out-hubs, in-hubs per level
     uint64_t *lbm;      // 3 x words: bitmaps of three consecutive levels (level j in slot j % 3)
     // outputs (any may be null):
     float *dep;         // 0.5 delta(v) at dep[perm ? perm[v] : v]
@@ -168,6 +169,61 @@ __device__ __forceinline__ double wave_level_sum(const uint32_t *adj, const uint
     return acc;
 }
 
+// Sum over the edges [b, e) of each lane's hub chunk (at most kChunk edges,
+// a multiple of 64) of val(neighbour) for neighbours whose bit is set in
+// `inlevel`, returned to the chunk's lane. All 64 lanes walk one chunk's edges;
+// Q chunks go together, every load of a step issued for all of them before
+// any is used, and each chunk costs one wave reduction rather than the
+// per-step owner search and segmented scan of wave_level_sum. FP64.
+template <uint32_t kChunk, int Q, typename Val>
+__device__ __forceinline__ double wave_chunk_sum(const uint32_t *adj, const uint64_t *inlevel, uint32_t b, uint32_t e,
+                                                 Val val)
+{
+    static_assert(kChunk % kWave == 0, "chunks are whole wave steps");
+    const int ln = lane();
+    uint64_t todo = ballot(e > b);
+    double mine = 0.0;
+    while (todo) {
+        int t[Q];
+        uint32_t cb[Q], ce[Q];
+#pragma unroll
+        for (int q = 0; q < Q; q++) {
+            t[q] = todo ? first_lane(todo) : -1;
+            todo &= todo - 1;
+            cb[q] = t[q] >= 0 ? shfl(b, t[q]) : 0u;
+            ce[q] = t[q] >= 0 ? shfl(e, t[q]) : 0u;
+        }
+        double acc[Q];
+#pragma unroll
+        for (int q = 0; q < Q; q++) acc[q] = 0.0;
+#pragma unroll
+        for (uint32_t o = 0; o < kChunk; o += kWave) {
+            uint32_t u[Q];
+            bool on[Q];
+#pragma unroll
+            for (int q = 0; q < Q; q++) {
+                const uint32_t pos = cb[q] + o + (uint32_t)ln;
+                on[q] = pos < ce[q];
+                u[q] = on[q] ? adj[pos] : 0u;
+            }
+#pragma unroll
+            for (int q = 0; q < Q; q++) on[q] = on[q] && test_bit(inlevel, u[q]);
+            double x[Q];
+#pragma unroll
+            for (int q = 0; q < Q; q++) x[q] = on[q] ? val(u[q]) : 0.0;
+#pragma unroll
+            for (int q = 0; q < Q; q++) acc[q] += x[q];
+        }
+#pragma unroll
+        for (int q = 0; q < Q; q++) {
+            if (t[q] < 0) break;  // wave-uniform
+            const double sum = wave_sum_f64(acc[q]);
+            if (ln == t[q]) mine = sum;
+        }
+    }
+    return mine;
+}
+
 // For every edge [b, e) of each lane's vertex (edge ids into `adj`), in the
 // scheduling of wave_level_sum: when `to(neighbour)` holds, atomically adds the
 // owning lane's x into dst[neighbour] (FP64). The push form of a level pass,
@@ -204,9 +260,9 @@ __device__ __forceinline__ void wave_level_push(const uint32_t *adj, uint32_t b,
     }
 }
 
-// Out-edge and in-edge volumes of the wave's `valid` lanes' levels (below
-// kBcVolLevels) added into vol[level] and vol[kBcVolLevels + level] (LDS), one
-// add per distinct level. Wave-convergent.
+// Out-edge and in-edge volumes, out-hub and in-hub counts of the wave's
+// `valid` lanes' levels (below kBcVolLevels) added into vol[k * kBcVolLevels +
+// level] for k = 0..3 (LDS), one add each per distinct level. Wave-convergent.
 __device__ __forceinline__ void wave_level_vol(uint32_t *vol, bool valid, uint32_t l, uint32_t od, uint32_t id)
 {
     uint64_t todo = ballot(valid);
@@ -221,9 +277,13 @@ __device__ __forceinline__ void wave_level_vol(uint32_t *vol, bool valid, uint32
             so += (uint32_t)__shfl_xor(so, d, kWave);
             si += (uint32_t)__shfl_xor(si, d, kWave);
         }
+        const uint32_t ho = (uint32_t)__popcll(ballot(in && od > kHugeDegree));
+        const uint32_t hi = (uint32_t)__popcll(ballot(in && id > kPullHubDegree));
         if (lane() == first_lane(m)) {
             atomicAdd(&vol[l0], so);
             atomicAdd(&vol[kBcVolLevels + l0], si);
+            if (ho) atomicAdd(&vol[2 * kBcVolLevels + l0], ho);
+            if (hi) atomicAdd(&vol[3 * kBcVolLevels + l0], hi);
         }
     }
 }
@@ -244,14 +304,14 @@ __device__ __forceinline__ bool bc1_run(const Bc1Args &a, uint32_t brank, uint32
 
     __shared__ uint32_t s_hist[kBcLdsLevels];
     __shared__ uint32_t s_max, s_scan[BS / kWave], s_D;
-    __shared__ uint32_t s_vol[2 * kBcVolLevels];
+    __shared__ uint32_t s_vol[4 * kBcVolLevels];
 
     // level counters start at zero (the BFS's first barrier publishes this)
     for (uint32_t i = gtid; i <= kBcMaxDepth; i += nthreads) {
         a.lcnt[i] = 0;
         a.lfill[i] = 0;
     }
-    for (uint32_t i = gtid; i < 2 * kBcVolLevels; i += nthreads) a.lvol[i] = 0;
+    for (uint32_t i = gtid; i < 4 * kBcVolLevels; i += nthreads) a.lvol[i] = 0;
     if (gtid == 0) {
         a.ctl[0] = 0;
         a.ctl[1] = 0;
@@ -262,7 +322,7 @@ __device__ __forceinline__ bool bc1_run(const Bc1Args &a, uint32_t brank, uint32
     // ---- per-level counts: block histograms, one atomic per block and level
     if (ok) {
         for (uint32_t i = threadIdx.x; i < kBcLdsLevels; i += BS) s_hist[i] = 0;
-        for (uint32_t i = threadIdx.x; i < 2 * kBcVolLevels; i += BS) s_vol[i] = 0;
+        for (uint32_t i = threadIdx.x; i < 4 * kBcVolLevels; i += BS) s_vol[i] = 0;
         if (threadIdx.x == 0) s_max = 0;
         __syncthreads();
         uint32_t lmax = 0;
@@ -289,7 +349,7 @@ __device__ __forceinline__ bool bc1_run(const Bc1Args &a, uint32_t brank, uint32
         const uint32_t top = min(s_max, kBcLdsLevels - 1);
         for (uint32_t l = threadIdx.x; l <= top; l += BS)
             if (s_hist[l]) atomicAdd(&a.lcnt[l], s_hist[l]);
-        for (uint32_t i = threadIdx.x; i < 2 * kBcVolLevels; i += BS)
+        for (uint32_t i = threadIdx.x; i < 4 * kBcVolLevels; i += BS)
             if (s_vol[i]) atomicAdd(&a.lvol[i], s_vol[i]);
         if (threadIdx.x == 0) atomicMax(&a.ctl[0], s_max);
         ok = grid_sync(a.f.bar, xcc, gen, a.f.abort);
@@ -383,8 +443,8 @@ __device__ __forceinline__ bool bc1_run(const Bc1Args &a, uint32_t brank, uint32
             }
             // sigma(u) of level L-1 along its out-edges into level L
             const uint32_t p0 = a.loff[L - 1];
-            for (uint32_t i0 = p0 + gwave * kWave; i0 < b0; i0 += nwaves * kWave) {
-                const uint32_t i = i0 + (uint32_t)ln;
+            for (uint32_t i0 = p0; i0 < b0; i0 += nwaves * kWave) {
+                const uint32_t i = i0 + (uint32_t)ln * nwaves + gwave;
                 uint32_t u = 0, b = 0, e = 0;
                 double x = 0.0;
                 if (i < b0) {
@@ -396,7 +456,9 @@ __device__ __forceinline__ bool bc1_run(const Bc1Args &a, uint32_t brank, uint32
                 }
                 wave_level_push(g.col, b, e, x, &a.sa[0].x, 2, at_level);
             }
-            for (uint32_t c0 = 0; c0 < g.nhuge; c0 += nwaves * kWave) {
+            // (push implies L - 1 < kBcVolLevels: the out-hub count is known)
+            const uint32_t nch = a.lvol[2 * kBcVolLevels + L - 1] ? g.nhuge : 0u;
+            for (uint32_t c0 = 0; c0 < nch; c0 += nwaves * kWave) {
                 const uint32_t c = c0 + (uint32_t)ln * nwaves + gwave;
                 uint32_t u = 0, b = 0, e = 0;
                 if (c < g.nhuge) {
@@ -448,6 +510,13 @@ __device__ __forceinline__ bool bc1_run(const Bc1Args &a, uint32_t brank, uint32
         } else {
             for (uint32_t i0 = b0 + gwave * kWave; i0 < e0; i0 += nwaves * kWave) {
                 const uint32_t i = i0 + (uint32_t)ln;
+                wave_set_bits(cur, i < e0, i < e0 ? a.order[i] : 0u);
+            }
+            // a wave's 64 vertices strided over the list, so the waves' edge
+            // totals even out (neighbouring positions hold vertices of close
+            // ids, hence of close degrees)
+            for (uint32_t i0 = b0; i0 < e0; i0 += nwaves * kWave) {
+                const uint32_t i = i0 + (uint32_t)ln * nwaves + gwave;
                 uint32_t v = 0, b = 0, e = 0;
                 if (i < e0) {
                     v = a.order[i];
@@ -455,7 +524,6 @@ __device__ __forceinline__ bool bc1_run(const Bc1Args &a, uint32_t brank, uint32
                     e = g.cstart[v + 1];
                     if (e - b > kPullHubDegree) e = b;  // in-hubs: the chunk pass below
                 }
-                wave_set_bits(cur, i < e0, v);
                 const double s = wave_level_sum(g.csrc, up, b, e, [&](uint32_t u) { return a.sa[u].x; });
                 if (i < e0 && e > b) a.sa[v].x = s;
             }
@@ -463,7 +531,9 @@ __device__ __forceinline__ bool bc1_run(const Bc1Args &a, uint32_t brank, uint32
         // in-hub chunks of this level, kPullChunk in-edges each: a lane per
         // chunk, the wave's chunks walked together like a level list (every
         // load of a step in flight at once), one atomic per chunk
-        for (uint32_t c0 = 0; !push && c0 < g.nhugein; c0 += nwaves * kWave) {
+        // (skipped for a level known to hold no in-hub)
+        const uint32_t nchin = push || (L < kBcVolLevels && !a.lvol[3 * kBcVolLevels + L]) ? 0u : g.nhugein;
+        for (uint32_t c0 = 0; c0 < nchin; c0 += nwaves * kWave) {
             const uint32_t c = c0 + (uint32_t)ln * nwaves + gwave;
             uint32_t v = 0, b = 0, e = 0;
             if (c < g.nhugein) {
@@ -475,7 +545,7 @@ __device__ __forceinline__ bool bc1_run(const Bc1Args &a, uint32_t brank, uint32
                 }
             }
             if (!ballot(e > b)) continue;
-            const double x = wave_level_sum(g.csrc, up, b, e, [&](uint32_t u) { return a.sa[u].x; });
+            const double x = wave_chunk_sum<kPullChunk, 4>(g.csrc, up, b, e, [&](uint32_t u) { return a.sa[u].x; });
             if (e > b && x != 0.0) atomicAdd(&a.sa[v].x, x);
         }
         ok = grid_sync(a.f.bar, xcc, gen, a.f.abort);
@@ -504,8 +574,8 @@ __device__ __forceinline__ bool bc1_run(const Bc1Args &a, uint32_t brank, uint32
             }
             // psi(w) of level L+1 along its in-edges into level L
             const uint32_t p1 = a.loff[L + 2];
-            for (uint32_t i0 = e0 + gwave * kWave; i0 < p1; i0 += nwaves * kWave) {
-                const uint32_t i = i0 + (uint32_t)ln;
+            for (uint32_t i0 = e0; i0 < p1; i0 += nwaves * kWave) {
+                const uint32_t i = i0 + (uint32_t)ln * nwaves + gwave;
                 uint32_t w = 0, b = 0, e = 0;
                 double x = 0.0;
                 if (i < p1) {
@@ -517,7 +587,8 @@ __device__ __forceinline__ bool bc1_run(const Bc1Args &a, uint32_t brank, uint32
                 }
                 wave_level_push(g.csrc, b, e, x, &a.sa[0].y, 2, at_level);
             }
-            for (uint32_t c0 = 0; c0 < g.nhugein; c0 += nwaves * kWave) {
+            const uint32_t nch = a.lvol[3 * kBcVolLevels + L + 1] ? g.nhugein : 0u;
+            for (uint32_t c0 = 0; c0 < nch; c0 += nwaves * kWave) {
                 const uint32_t c = c0 + (uint32_t)ln * nwaves + gwave;
                 uint32_t w = 0, b = 0, e = 0;
                 if (c < g.nhugein) {
@@ -537,6 +608,11 @@ __device__ __forceinline__ bool bc1_run(const Bc1Args &a, uint32_t brank, uint32
         }
         for (uint32_t i0 = b0 + gwave * kWave; i0 < e0; i0 += nwaves * kWave) {
             const uint32_t i = i0 + (uint32_t)ln;
+            wave_set_bits(cur, i < e0, i < e0 ? a.order[i] : 0u);
+        }
+        // strided over the list like the sigma pass
+        for (uint32_t i0 = b0; i0 < e0; i0 += nwaves * kWave) {
+            const uint32_t i = i0 + (uint32_t)ln * nwaves + gwave;
             uint32_t u = 0, b = 0, e = 0;
             if (i < e0) {
                 u = a.order[i];
@@ -544,13 +620,13 @@ __device__ __forceinline__ bool bc1_run(const Bc1Args &a, uint32_t brank, uint32
                 e = g.row[u + 1];
                 if (e - b > kHugeDegree) e = b;  // out-hubs: the chunk pass below
             }
-            wave_set_bits(cur, i < e0, u);
             const double s = wave_level_sum(g.col, down, b, e, psi);
             if (i < e0 && e > b) a.sa[u].y = s;
         }
         // out-hub chunks of this level, kHugeChunk out-edges each, walked
         // like the in-hub chunks of the sigma pass
-        for (uint32_t c0 = 0; c0 < g.nhuge; c0 += nwaves * kWave) {
+        const uint32_t nch = L < kBcVolLevels && !a.lvol[2 * kBcVolLevels + L] ? 0u : g.nhuge;
+        for (uint32_t c0 = 0; c0 < nch; c0 += nwaves * kWave) {
             const uint32_t c = c0 + (uint32_t)ln * nwaves + gwave;
             uint32_t u = 0, b = 0, e = 0;
             if (c < g.nhuge) {
@@ -562,7 +638,7 @@ __device__ __forceinline__ bool bc1_run(const Bc1Args &a, uint32_t brank, uint32
                 }
             }
             if (!ballot(e > b)) continue;
-            const double x = wave_level_sum(g.col, down, b, e, psi);
+            const double x = wave_chunk_sum<kHugeChunk, 8>(g.col, down, b, e, psi);
             if (e > b && x != 0.0) atomicAdd(&a.sa[u].y, x);
         }
         ok = grid_sync(a.f.bar, xcc, gen, a.f.abort);
@@ -605,7 +681,7 @@ inline Bc1Layout bc1_layout(uint64_t n)
     put(L.loff, (uint64_t)(kBcMaxDepth + 2) * 4);
     put(L.lfill, (uint64_t)(kBcMaxDepth + 2) * 4);
     put(L.ctl, 256);
-    put(L.lvol, 2 * kBcVolLevels * 4);
+    put(L.lvol, 4 * kBcVolLevels * 4);
     put(L.lbm, 3 * ((n + 63) / 64) * 8);
     put(L.fstats, sizeof(DevStats));
     put(L.bcsum, n * 8);
