@@ -32,7 +32,8 @@
 //      level whose upper neighbour has at most 1/kBcPushRatio of its in-edges
 //      as out-edges is pushed instead: sigma(u) added along u's out-edges into
 //      the level with FP64 atomics (the BFS direction rule applied to the
-//      sigma pass; the reference's stage 1 pushes or pulls with its BFS);
+//      sigma pass; the reference's stage 1 pushes or pulls with its BFS); a
+//      forced push or pull variant fixes the direction of every level pass;
 //   4. dependencies, level by level from the deepest (the same rotating
 //      level bitmaps): acc(u) = sum over
 //      successors w one level down of psi(w) = 1/sigma(w) + acc(w), which is
@@ -434,7 +435,9 @@ __device__ __forceinline__ bool bc1_run(const Bc1Args &a, uint32_t brank, uint32
         if (L >= 2)
             for (uint32_t i = a.loff[L - 2] + gtid; i < a.loff[L - 1]; i += nthreads) old[a.order[i] >> 6] = 0ull;
         // push when level L-1's out-edges are far fewer than level L's in-edges
-        const bool push = L < kBcVolLevels && (uint64_t)a.lvol[L - 1] * kBcPushRatio <= a.lvol[kBcVolLevels + L];
+        // (or when a push variant is forced; a forced pull never pushes)
+        const bool push = a.f.force == 0 || (a.f.force < 0 && L < kBcVolLevels &&
+                                             (uint64_t)a.lvol[L - 1] * kBcPushRatio <= a.lvol[kBcVolLevels + L]);
         if (push) {
             auto at_level = [&](uint32_t w) { return lvl[w] == (int32_t)L; };
             for (uint32_t i0 = b0 + gwave * kWave; i0 < e0; i0 += nwaves * kWave) {
@@ -456,8 +459,8 @@ __device__ __forceinline__ bool bc1_run(const Bc1Args &a, uint32_t brank, uint32
                 }
                 wave_level_push(g.col, b, e, x, &a.sa[0].x, 2, at_level);
             }
-            // (push implies L - 1 < kBcVolLevels: the out-hub count is known)
-            const uint32_t nch = a.lvol[2 * kBcVolLevels + L - 1] ? g.nhuge : 0u;
+            // (skipped for a level known to hold no out-hub)
+            const uint32_t nch = L - 1 < kBcVolLevels && !a.lvol[2 * kBcVolLevels + L - 1] ? 0u : g.nhuge;
             for (uint32_t c0 = 0; c0 < nch; c0 += nwaves * kWave) {
                 const uint32_t c = c0 + (uint32_t)ln * nwaves + gwave;
                 uint32_t u = 0, b = 0, e = 0;
@@ -564,8 +567,8 @@ __device__ __forceinline__ bool bc1_run(const Bc1Args &a, uint32_t brank, uint32
             return 1.0 / x.x + x.y;
         };
         // push when level L+1's in-edges are far fewer than level L's out-edges
-        const bool push =
-            L + 1 < kBcVolLevels && (uint64_t)a.lvol[kBcVolLevels + L + 1] * kBcPushRatio <= a.lvol[L];
+        const bool push = a.f.force == 0 || (a.f.force < 0 && L + 1 < kBcVolLevels &&
+                                             (uint64_t)a.lvol[kBcVolLevels + L + 1] * kBcPushRatio <= a.lvol[L]);
         if (push) {
             auto at_level = [&](uint32_t u) { return lvl[u] == (int32_t)L; };
             for (uint32_t i0 = b0 + gwave * kWave; i0 < e0; i0 += nwaves * kWave) {
@@ -587,7 +590,7 @@ __device__ __forceinline__ bool bc1_run(const Bc1Args &a, uint32_t brank, uint32
                 }
                 wave_level_push(g.csrc, b, e, x, &a.sa[0].y, 2, at_level);
             }
-            const uint32_t nch = a.lvol[3 * kBcVolLevels + L + 1] ? g.nhugein : 0u;
+            const uint32_t nch = L + 1 < kBcVolLevels && !a.lvol[3 * kBcVolLevels + L + 1] ? 0u : g.nhugein;
             for (uint32_t c0 = 0; c0 < nch; c0 += nwaves * kWave) {
                 const uint32_t c = c0 + (uint32_t)ln * nwaves + gwave;
                 uint32_t w = 0, b = 0, e = 0;

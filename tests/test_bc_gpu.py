@@ -110,6 +110,31 @@ def test_bc_rmat16_batches(sg, orc, gpu, rmat16_sym, variant):
         assert ok, (variant, batch, err, st)
 
 
+@pytest.mark.parametrize("variant", ["auto", "async_push_dd", "async_pull_dd"])
+@pytest.mark.parametrize("name", ["long_path", "grid", "dir_random", "two_parts", "rmat14_dir", "rmat14_und"])
+def test_bc_single_source_directions(sg, orc, gpu, name, variant):
+    """One source per launch (the single-source kernel): its sigma and
+    dependency level passes pull or push by the levels' edge volumes, and a
+    forced push or pull variant fixes every pass's direction. All three must
+    match the oracle, including levels past the 64 whose volumes are counted
+    (the 150-vertex path) and hub chunk lists (RMAT-14)."""
+    if name == "long_path":
+        row, col, und = *G.path(150), True
+    elif name == "rmat14_dir" or name == "rmat14_und":
+        h = sg.rmat_graph(14, 16, symmetrize=name == "rmat14_und", seed=5)
+        row, col, und = h.row, h.col, name == "rmat14_und"
+    else:
+        row, col = SMALL[name]()
+        und = name != "dir_random"
+    n = len(row) - 1
+    hg = sg.HostGraph(n, row, col, None, undirected=und)
+    srcs = np.asarray(sg.pick_sources(hg.row, 12, seed=7), np.uint32)
+    x, st, _ = run_bc(sg, hg, srcs, 1, force_variant=variant)
+    ok, err = close(x, orc.bc(row, col, srcs))
+    assert ok, (name, variant, err)
+    assert st["iterations"] == len(srcs)
+
+
 def test_bc_rmat18_directed(sg, orc, gpu):
     hg = sg.rmat_graph(18, 16, symmetrize=False, seed=1)
     srcs = sg.pick_sources(hg.row, 40, seed=3)
