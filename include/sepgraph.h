@@ -398,7 +398,10 @@ int sg_pagerank(sg_engine *e, double damping, double tol, int32_t max_iter, int3
  * sigma in FP64). With k = 1 this is the reference's single-source BC
  * (samples/hybrid_bc/hybrid_bc.cu:303-376, BetweennessCentralityHost,
  * hybrid_bc_host.cpp:16-92). Graphs may be directed (forward over
- * out-edges, dependencies over successors). Asynchronous; sg_stats.rounds
+ * out-edges, dependencies over successors). With batch 1 each level's
+ * sigma and dependency pass pulls or pushes by the level's edge volumes; a
+ * forced push or pull variant (options.force_variant) fixes that direction
+ * as it fixes the forward traversal's. Asynchronous; sg_stats.rounds
  * counts forward levels over all batches, .max_depth the deepest level,
  * .iterations the batches. */
 int sg_bc(sg_engine *e, const uint32_t *sources, int32_t k, int32_t batch);
