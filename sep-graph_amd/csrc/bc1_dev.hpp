@@ -99,7 +99,8 @@ __device__ __forceinline__ uint32_t wave_level_add(uint32_t *hist, uint32_t *glo
     uint64_t todo = ballot(valid);
     uint32_t got = 0;
     while (todo) {
-        const uint32_t l0 = shfl(l, first_lane(todo));
+        // the lane indices are wave-uniform: read lanes, no cross-lane shuffle
+        const uint32_t l0 = __builtin_amdgcn_readfirstlane(__builtin_amdgcn_readlane(l, first_lane(todo)));
         const uint64_t m = ballot(valid && l == l0);
         todo &= ~m;
         uint32_t base = 0;
@@ -107,10 +108,12 @@ __device__ __forceinline__ uint32_t wave_level_add(uint32_t *hist, uint32_t *glo
             const uint32_t c = (uint32_t)__popcll(m);
             base = l0 < kBcLdsLevels ? atomicAdd(&hist[l0], c) : atomicAdd(&glob[l0], c);
         }
-        base = shfl(base, first_lane(m));
-        if (valid && l == l0) got = base + (uint32_t)rank_in(m);
+        if (take) {
+            base = __builtin_amdgcn_readlane(base, first_lane(m));
+            if (valid && l == l0) got = base + (uint32_t)rank_in(m);
+        }
     }
-    return take ? got : 0u;
+    return got;
 }
 
 // Sum over the edges [b, e) of each lane's vertex (edge ids into `adj`) of
@@ -261,32 +264,17 @@ __device__ __forceinline__ void wave_level_push(const uint32_t *adj, uint32_t b,
     }
 }
 
-// Out-edge and in-edge volumes, out-hub and in-hub counts of the wave's
-// `valid` lanes' levels (below kBcVolLevels) added into vol[k * kBcVolLevels +
-// level] for k = 0..3 (LDS), one add each per distinct level. Wave-convergent.
-__device__ __forceinline__ void wave_level_vol(uint32_t *vol, bool valid, uint32_t l, uint32_t od, uint32_t id)
+// Out-edge and in-edge volume, out-hub and in-hub count of a lane's vertex
+// at level l (below kBcVolLevels) added into vol[k * kBcVolLevels + l] for
+// k = 0..3 (LDS). Per-lane LDS atomics: lanes of one level serialise on one
+// bank, which costs less than wave reductions' chains of cross-lane shuffles.
+__device__ __forceinline__ void lane_level_vol(uint32_t *vol, bool valid, uint32_t l, uint32_t od, uint32_t id)
 {
-    uint64_t todo = ballot(valid);
-    while (todo) {
-        const uint32_t l0 = shfl(l, first_lane(todo));
-        const bool in = valid && l == l0;
-        const uint64_t m = ballot(in);
-        todo &= ~m;
-        uint32_t so = in ? od : 0u, si = in ? id : 0u;
-#pragma unroll
-        for (int d = kWave / 2; d > 0; d >>= 1) {
-            so += (uint32_t)__shfl_xor(so, d, kWave);
-            si += (uint32_t)__shfl_xor(si, d, kWave);
-        }
-        const uint32_t ho = (uint32_t)__popcll(ballot(in && od > kHugeDegree));
-        const uint32_t hi = (uint32_t)__popcll(ballot(in && id > kPullHubDegree));
-        if (lane() == first_lane(m)) {
-            atomicAdd(&vol[l0], so);
-            atomicAdd(&vol[kBcVolLevels + l0], si);
-            if (ho) atomicAdd(&vol[2 * kBcVolLevels + l0], ho);
-            if (hi) atomicAdd(&vol[3 * kBcVolLevels + l0], hi);
-        }
-    }
+    if (!valid) return;
+    atomicAdd(&vol[l], od);
+    atomicAdd(&vol[kBcVolLevels + l], id);
+    if (od > kHugeDegree) atomicAdd(&vol[2 * kBcVolLevels + l], 1u);
+    if (id > kPullHubDegree) atomicAdd(&vol[3 * kBcVolLevels + l], 1u);
 }
 
 template <int BS>
@@ -337,11 +325,8 @@ __device__ __forceinline__ bool bc1_run(const Bc1Args &a, uint32_t brank, uint32
             // edge volumes of the shallow levels (a level's edges number
             // below 2^32: edge offsets are 32-bit)
             const bool vv = valid && (uint32_t)l < kBcVolLevels;
-            if (ballot(vv)) {
-                const uint32_t od = vv ? g.row[v + 1] - g.row[v] : 0u;
-                const uint32_t id = vv ? g.cstart[v + 1] - g.cstart[v] : 0u;
-                wave_level_vol(s_vol, vv, vv ? (uint32_t)l : 0u, od, id);
-            }
+            if (vv)
+                lane_level_vol(s_vol, true, (uint32_t)l, g.row[v + 1] - g.row[v], g.cstart[v + 1] - g.cstart[v]);
         }
 #pragma unroll
         for (int d = kWave / 2; d > 0; d >>= 1) lmax = max(lmax, (uint32_t)__shfl_xor(lmax, d, kWave));
