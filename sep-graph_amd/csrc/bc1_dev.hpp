@@ -13,15 +13,11 @@
 // Here:
 //   1. levels: bfs_run (push/pull by the alpha/beta rule, exact claims),
 //      levels kept in device vertex order;
-//   2. level lists: a counting sort of the reached vertices by level, so
-//      each later phase visits a level's vertices without scanning all n.
-//      The level sizes come from the traversal's own per-round counts (every
-//      block holds them, so each block scans them into offsets without a
-//      census pass or a barrier; traversals deeper than kBcLdsLevels count
-//      levels with block histograms and one atomic per block and level); the
-//      scatter hands out positions per block, wave and level, and sums the
-//      out- and in-degrees and hub counts of each of the first kBcVolLevels
-//      levels (edge volumes);
+//   2. level lists: a counting sort of the reached vertices by level (block
+//      histograms in LDS, one atomic per block and level, positions handed out
+//      per wave and level), so each later phase visits a level's vertices
+//      without scanning all n; the same pass sums the out- and in-degrees of
+//      each of the first kBcVolLevels levels (edge volumes);
 //   3. sigma, level by level: every vertex gathers sigma(u) over in-neighbours
 //      one level up (pull over the CSC, wave prefix-sum scheduling), testing
 //      "u is one level up" in a bitmap of that level (n/8 bytes, L2-resident)
@@ -296,8 +292,7 @@ __device__ __forceinline__ bool bc1_run(const Bc1Args &a, uint32_t brank, uint32
     const int32_t *lvl = a.f.level;
 
     __shared__ uint32_t s_hist[kBcLdsLevels];
-    __shared__ uint32_t s_max, s_scan[BS / kWave], s_D, s_nlev;
-    __shared__ uint32_t s_off[kBcLdsLevels + 1];
+    __shared__ uint32_t s_max, s_scan[BS / kWave], s_D;
     __shared__ uint32_t s_vol[4 * kBcVolLevels];
 
     // level counters start at zero (the BFS's first barrier publishes this)
@@ -310,47 +305,11 @@ __device__ __forceinline__ bool bc1_run(const Bc1Args &a, uint32_t brank, uint32
         a.ctl[0] = 0;
         a.ctl[1] = 0;
     }
-    if (threadIdx.x == 0) s_nlev = 0;
-    bool ok = bfs_run<BS>(a.f, brank, nblocks, xcc, gen, s_hist, kBcLdsLevels, &s_nlev);
+    bool ok = bfs_run<BS>(a.f, brank, nblocks, xcc, gen);
     int32_t status = ok ? SG_OK : SG_ERR_TIMEOUT;
 
-    // Level offsets from the traversal's own per-round counts, which every
-    // block read alike (s_hist[0..nl), the empty last round at nl): each
-    // block scans them into s_off itself and block 0 publishes them, so a
-    // traversal of at most kBcLdsLevels levels needs neither a census pass
-    // over the vertices nor a barrier before the scatter.
-    auto scan_levels = [&](const uint32_t *cnt, uint32_t nl, uint32_t *off) {
-        // thread t sums a contiguous run of levels, then a block scan
-        const uint32_t per = (nl + BS - 1) / BS;
-        const uint32_t l0 = threadIdx.x * per, l1 = min(l0 + per, nl);
-        uint32_t sum = 0;
-        for (uint32_t l = l0; l < l1; l++) sum += cnt[l];
-        const uint32_t incl = wave_incl_scan(sum);
-        if (ln == kWave - 1) s_scan[threadIdx.x / kWave] = incl;
-        __syncthreads();
-        uint32_t wbase = 0;
-        for (uint32_t w = 0; w < threadIdx.x / kWave; w++) wbase += s_scan[w];
-        uint32_t pos = wbase + incl - sum;
-        for (uint32_t l = l0; l < l1; l++) {
-            off[l] = pos;
-            pos += cnt[l];
-        }
-        if (threadIdx.x == BS - 1) off[nl] = pos;
-    };
-    __syncthreads();
-    const uint32_t nl = s_nlev - 1u;  // levels; the last round begun found none
-    const bool fast = ok && s_nlev >= 2u && nl <= kBcLdsLevels;
-    if (fast) {
-        scan_levels(s_hist, nl, s_off);
-        if (threadIdx.x == 0) s_D = nl - 1;
-        __syncthreads();
-        if (brank == 0)
-            for (uint32_t l = threadIdx.x; l <= nl; l += BS) a.loff[l] = s_off[l];
-    }
-
-    // ---- deeper traversals: per-level counts by block histograms, one
-    // atomic per block and level, then one block scans them
-    if (ok && !fast) {
+    // ---- per-level counts: block histograms, one atomic per block and level
+    if (ok) {
         for (uint32_t i = threadIdx.x; i < kBcLdsLevels; i += BS) s_hist[i] = 0;
         for (uint32_t i = threadIdx.x; i < 4 * kBcVolLevels; i += BS) s_vol[i] = 0;
         if (threadIdx.x == 0) s_max = 0;
@@ -382,7 +341,7 @@ __device__ __forceinline__ bool bc1_run(const Bc1Args &a, uint32_t brank, uint32
         ok = grid_sync(a.f.bar, xcc, gen, a.f.abort);
     }
     // ---- level offsets: one block scans the counts
-    if (ok && !fast) {
+    if (ok) {
         if (threadIdx.x == 0) {
             s_D = __hip_atomic_load(&a.ctl[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             s_max = __hip_atomic_load(&a.ctl[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -394,34 +353,44 @@ __device__ __forceinline__ bool bc1_run(const Bc1Args &a, uint32_t brank, uint32
             status = SG_ERR_CAPACITY;
         }
     }
-    if (ok && !fast) {
-        if (brank == 0) scan_levels(a.lcnt, s_D + 1, a.loff);
+    if (ok) {
+        const uint32_t D = s_D;
+        if (brank == 0) {
+            // thread t sums a contiguous run of levels, then a block scan
+            const uint32_t per = (D + 1 + BS - 1) / BS;
+            const uint32_t l0 = threadIdx.x * per, l1 = min(l0 + per, D + 1);
+            uint32_t sum = 0;
+            for (uint32_t l = l0; l < l1; l++) sum += a.lcnt[l];
+            const uint32_t incl = wave_incl_scan(sum);
+            if (ln == kWave - 1) s_scan[threadIdx.x / kWave] = incl;
+            __syncthreads();
+            uint32_t wbase = 0;
+            for (uint32_t w = 0; w < threadIdx.x / kWave; w++) wbase += s_scan[w];
+            uint32_t pos = wbase + incl - sum;
+            for (uint32_t l = l0; l < l1; l++) {
+                a.loff[l] = pos;
+                pos += a.lcnt[l];
+            }
+            if (threadIdx.x == BS - 1) a.loff[D + 1] = pos;
+        }
         ok = grid_sync(a.f.bar, xcc, gen, a.f.abort);
     }
     // ---- scatter the reached vertices into their levels' ranges; (sigma,
     // acc) start at (1, 0) for the source and (0, 0) elsewhere
     if (ok) {
         for (uint32_t i = threadIdx.x; i < kBcLdsLevels; i += BS) s_hist[i] = 0;
-        if (fast)
-            for (uint32_t i = threadIdx.x; i < 4 * kBcVolLevels; i += BS) s_vol[i] = 0;
         __syncthreads();
-        // the block histogram of the vertices this block places (and, with
-        // the counts taken from the traversal, the edge volumes)
+        // the same vertices as the count pass, so the same block histogram
         for (uint32_t v0 = brank * BS; v0 < n; v0 += nthreads) {
             const uint32_t v = v0 + threadIdx.x;
             const int32_t l = v < n ? lvl[v] : -1;
             const bool valid = l >= 0 && (uint32_t)l < kBcLdsLevels;
             wave_level_add(s_hist, nullptr, valid, valid ? (uint32_t)l : 0u, false);
-            if (fast && valid && (uint32_t)l < kBcVolLevels)
-                lane_level_vol(s_vol, true, (uint32_t)l, g.row[v + 1] - g.row[v], g.cstart[v + 1] - g.cstart[v]);
         }
         __syncthreads();
         const uint32_t top = min(s_D, kBcLdsLevels - 1);
         for (uint32_t l = threadIdx.x; l <= top; l += BS)
-            if (s_hist[l]) s_hist[l] = (fast ? s_off[l] : a.loff[l]) + atomicAdd(&a.lfill[l], s_hist[l]);
-        if (fast)
-            for (uint32_t i = threadIdx.x; i < 4 * kBcVolLevels; i += BS)
-                if (s_vol[i]) atomicAdd(&a.lvol[i], s_vol[i]);
+            if (s_hist[l]) s_hist[l] = a.loff[l] + atomicAdd(&a.lfill[l], s_hist[l]);
         __syncthreads();
         for (uint32_t v0 = brank * BS; v0 < n; v0 += nthreads) {
             const uint32_t v = v0 + threadIdx.x;

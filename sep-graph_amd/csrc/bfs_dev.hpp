@@ -321,15 +321,9 @@ __device__ __forceinline__ void pull_group(Round &R, const uint32_t (&wg)[G], co
 // kStage: push rounds stage hub neighbour lists in LDS (the engine option
 // stage_hubs; a separate instantiation, so the default path keeps its
 // register budget).
-// s_lcount (optional, this block's LDS): thread 0 records each level's vertex
-// count there as every block reads it at the top of the level's round
-// (levels below lcap), and *s_nlev the number of rounds begun -- on a
-// finished traversal, the number of levels plus the empty last one. Every
-// block holds the same figures without another barrier.
 template <int BS, bool kStage = false>
 __device__ __forceinline__ bool bfs_run(const BfsArgs &a, uint32_t brank, uint32_t nblocks, uint32_t xcc,
-                                        BarState &gen, uint32_t *s_lcount = nullptr, uint32_t lcap = 0,
-                                        uint32_t *s_nlev = nullptr)
+                                        BarState &gen)
 {
     constexpr int WPB = BS / kWave;
     constexpr uint32_t kLdsQ = 8u * BS;
@@ -420,7 +414,6 @@ __device__ __forceinline__ bool bfs_run(const BfsArgs &a, uint32_t brank, uint32
         pol.next(0, 1, sdeg);  // push, as every round 0
         pushes = 1;
         if (gtid == 0) write_trace(a.trace, 0, false, 1, sdeg, t_start);
-        if (s_lcount && threadIdx.x == 0 && lcap) s_lcount[0] = 1;
         L = 1;
     }
 
@@ -430,10 +423,6 @@ __device__ __forceinline__ bool bfs_run(const BfsArgs &a, uint32_t brank, uint32
         __syncthreads();
         const RoundCounts rc = unpack_counts(s_lvl, a.qcap);
         __syncthreads();
-        if (s_lcount && threadIdx.x == 0) {
-            if (L < lcap) s_lcount[L] = (uint32_t)rc.count;
-            *s_nlev = L + 1;
-        }
         if (L == 1 && local0) {
             // the source's out-neighbours found by block 0 (at most
             // kLocalRound0, so the queue holds them all)
