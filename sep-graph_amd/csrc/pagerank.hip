@@ -289,17 +289,45 @@ __global__ __launch_bounds__(BS) void k_pr_pull(PrArgs a)
             if (dout) cn[v] = xn / (float)dout;
             else csum_add(d_acc, xn);
         };
-        // (A1) chunk sums of the in-hubs
-        for (uint32_t c = gwave; c < g.nhugein; c += nwaves) {
-            const uint2 dsc = g.hugein[c];
-            const uint32_t end = min(g.cstart[dsc.x + 1], dsc.y + kPullChunk);
-            CSum s{0.f, 0.f};
-            for (uint32_t e0 = dsc.y; e0 < end; e0 += kWave) {
-                const uint32_t e = e0 + ln;
-                if (e < end) csum_add(s, contrib(g.csrc[e]));
+        // (A1) chunk sums of the in-hubs, kQ chunks per wave together: every
+        // in-edge load of the kQ chunks is issued before any contribution is
+        // gathered, and every gather before any is added (each lane still
+        // adds its edges of a chunk in edge order, so the sums are the same)
+        constexpr int kQ = 4, kSteps = (int)(kPullChunk / kWave);
+        for (uint32_t c0 = gwave * kQ; c0 < g.nhugein; c0 += nwaves * kQ) {
+            uint32_t cb[kQ], ce[kQ];
+#pragma unroll
+            for (int q = 0; q < kQ; q++) {
+                cb[q] = ce[q] = 0;
+                if (c0 + q < g.nhugein) {
+                    const uint2 dsc = g.hugein[c0 + q];
+                    cb[q] = dsc.y;
+                    ce[q] = min(g.cstart[dsc.x + 1], dsc.y + kPullChunk);
+                }
             }
-            s = wave_csum(s);
-            if (ln == 0) a.hubpart[c] = s;
+            uint32_t u[kQ][kSteps];
+#pragma unroll
+            for (int q = 0; q < kQ; q++)
+#pragma unroll
+                for (int t = 0; t < kSteps; t++) {
+                    const uint32_t e = cb[q] + (uint32_t)(t * kWave + ln);
+                    u[q][t] = e < ce[q] ? g.csrc[e] : 0xFFFFFFFFu;
+                }
+            float cv[kQ][kSteps];
+#pragma unroll
+            for (int q = 0; q < kQ; q++)
+#pragma unroll
+                for (int t = 0; t < kSteps; t++) cv[q][t] = u[q][t] != 0xFFFFFFFFu ? contrib(u[q][t]) : 0.f;
+#pragma unroll
+            for (int q = 0; q < kQ; q++) {
+                if (c0 + q >= g.nhugein) break;  // wave-uniform
+                CSum s{0.f, 0.f};
+#pragma unroll
+                for (int t = 0; t < kSteps; t++)
+                    if (u[q][t] != 0xFFFFFFFFu) csum_add(s, cv[q][t]);
+                s = wave_csum(s);
+                if (ln == 0) a.hubpart[c0 + q] = s;
+            }
         }
         // (A2) every other vertex, a 64-vertex word per wave at a time: from
         // the sliced in-edges (each lane sums its own vertex's row entries in
